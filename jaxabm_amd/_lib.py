@@ -1,0 +1,137 @@
+"""ctypes binding of ``libjaxabm_hip.so`` (declared in ``include/jaxabm_amd.h``).
+
+The engine has one execution path: the HIP library on a ROCm GPU.  Loading
+fails loudly when the library is missing, and every compute entry point
+refuses tensors that are not on the GPU; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+from .build import LIB_PATH
+
+MODE_ORIGINAL = 0
+MODE_PARTITIONABLE = 1
+
+_lock = threading.Lock()
+_lib = None
+
+_p = C.c_void_p
+_i64 = C.c_int64
+_i32 = C.c_int32
+_int = C.c_int
+_f32 = C.c_float
+_u32 = C.c_uint32
+_sz = C.c_size_t
+_u32arr = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_i32arr = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+
+
+class AbmError(RuntimeError):
+    """A non-zero status from the C-ABI, with the library's message."""
+
+
+class SirGraph(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64),
+        ("nnz", C.c_int64),
+        ("row_ptr", C.c_void_p),
+        ("col", C.c_void_p),
+        ("heavy", C.c_void_p),
+        ("n_heavy", C.c_int64),
+    ]
+
+
+# name -> (restype, argtypes).  Every symbol declared in include/jaxabm_amd.h
+# appears here; tests check the two lists agree.
+SIGNATURES = {
+    "abm_last_error": (C.c_char_p, []),
+    "abm_version": (_int, []),
+    "abm_threefry2x32": (_int, [_p, _p, _p, _p, _p, _i64, _p]),
+    "abm_split": (_int, [_p, _i64, _p, _int, _p]),
+    "abm_random_bits32": (_int, [_p, _i64, _p, _int, _p]),
+    "abm_uniform_f32": (_int, [_p, _i64, _f32, _f32, _p, _int, _p]),
+    "abm_normal_f32": (_int, [_p, _i64, _p, _int, _p]),
+    "abm_bernoulli_u8": (_int, [_p, _i64, _f32, _p, _int, _p]),
+    "abm_randint_i32": (_int, [_p, _i64, _i32, _i32, _p, _int, _p]),
+    "abm_split_each": (_int, [_p, _i64, _i64, _p, _int, _p]),
+    "abm_draw_each": (_int, [_p, _i64, _i64, _int, _f32, _f32, _p, _int, _p]),
+    "abm_randint_each": (_int, [_p, _i64, _i64, _i32, _i32, _p, _int, _p]),
+    "abm_permutation_workspace": (_sz, [_i64]),
+    "abm_permutation_rounds": (_int, [_i64]),
+    "abm_permutation": (_int, [_p, _i64, _p, _p, _sz, _int, _p]),
+    "abm_sort_pairs_u32": (_int, [_p, _p, _i64, _p, _sz, _p]),
+    "abm_host_split": (_int, [_u32arr, _i64, _u32arr, _int]),
+    "abm_step_keys": (_int, [_u32arr, _i32, _i32, _i32, _i32, _int, _u32arr]),
+    "abm_ba_num_edges": (_i64, [_i64, _i32]),
+    "abm_ba_edges": (_i64, [_i64, _i32, _u32arr, _i32arr, _i32arr]),
+    "abm_csr_undirected": (_int, [_i64, _i64, _i32arr, _i32arr, _i32arr, _i32arr]),
+    "abm_heavy_rows": (_i64, [_i64, _i32arr, _i32, _p]),
+    "abm_sir_heavy_threshold": (_int, []),
+    "abm_sir_seed": (_int, [_i64, _p, _i64, _p, _p]),
+    "abm_sir_run": (_int, [C.POINTER(SirGraph), _p, _p, _p, _i32, _f32, _f32,
+                           _int, _p, _p]),
+    "abm_sir_step": (_int, [C.POINTER(SirGraph), _p, _p, _p, _f32, _f32, _int, _p, _p]),
+    "abm_unit_threshold": (_u32, [_f32]),
+    "abm_fill_i32": (_int, [_p, _i64, _i32, _p]),
+    "abm_graph_begin": (_int, [_p]),
+    "abm_graph_end": (_int, [_p, C.POINTER(_p)]),
+    "abm_graph_launch": (_int, [_p, _p]),
+    "abm_graph_destroy": (_int, [_p]),
+    "abm_memcpy_d2d": (_int, [_p, _p, _sz, _p]),
+    "abm_event_timer": (_int, [_p, _int, C.POINTER(_f32)]),
+}
+
+
+def lib():
+    """Load the HIP library (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    f"{LIB_PATH} is missing: build it with `python -m jaxabm_amd.build` "
+                    "(the engine has no CPU fallback)")
+            L = C.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(L, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = lib().abm_last_error()
+        msg = msg.decode() if msg else "unknown error"
+        raise AbmError(f"{what or 'jaxabm_amd'} failed (status {rc}): {msg}")
+
+
+def require_gpu() -> None:
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("jaxabm_amd needs a ROCm GPU: torch.cuda.is_available() is False")
+
+
+def ptr(t) -> C.c_void_p:
+    """Raw device pointer of a torch tensor (must be on the GPU)."""
+    if t is None:
+        return C.c_void_p(0)
+    if not t.is_cuda:
+        raise ValueError("jaxabm_amd kernels take GPU tensors only (got a CPU tensor)")
+    return C.c_void_p(t.data_ptr())
+
+
+def stream_handle(stream=None) -> C.c_void_p:
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)

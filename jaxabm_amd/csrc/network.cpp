@@ -1,0 +1,131 @@
+// network.cpp -- host-side Network construction: Barabasi-Albert edge lists
+// and undirected CSR.
+//
+// Replaces the reference's edge-at-a-time Network (jaxabm/agentpy.py:559-582:
+// one jnp.concatenate per add_edge, O(E^2) to build) with a bulk builder whose
+// observable neighbour order matches Network.get_neighbors
+// (jaxabm/agentpy.py:584-615: ascending unique ids for undirected networks).
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "common.hpp"
+#include "jaxabm_amd.h"
+
+using namespace abm;
+
+extern "C" {
+
+int64_t abm_ba_num_edges(int64_t n, int32_t m) {
+  if (n <= m) return n > 0 ? n - 1 : 0;
+  return (int64_t)m * (n - m);
+}
+
+// Barabasi-Albert (n, m): star on nodes 0..m centred on 0, then each node
+// s = m+1 .. n-1 attaches to m distinct nodes drawn from the degree-repeated
+// node list; draw a of node s picks index floor(r * len / 2^32) with
+// r = threefry(key, (s, a)).y0.
+int64_t abm_ba_edges(int64_t n, int32_t m, const uint32_t key[2], int32_t *src,
+                     int32_t *dst) {
+  if (n <= 1 || m < 1 || !key || !src || !dst) {
+    if (n > 1 && m < 1) set_error("abm_ba_edges: m must be >= 1");
+    return n <= 1 ? 0 : -1;
+  }
+  int64_t e = 0;
+  if (n <= m) {
+    for (int64_t i = 1; i < n; ++i, ++e) {
+      src[e] = 0;
+      dst[e] = (int32_t)i;
+    }
+    return e;
+  }
+  std::vector<int32_t> rep;
+  rep.reserve((size_t)(2 * abm_ba_num_edges(n, m) + 2));
+  for (int32_t i = 1; i <= m; ++i, ++e) {
+    src[e] = 0;
+    dst[e] = i;
+  }
+  rep.insert(rep.end(), (size_t)m, 0);
+  for (int32_t i = 1; i <= m; ++i) rep.push_back(i);
+  std::vector<int32_t> pick((size_t)m);
+  for (int64_t s = m + 1; s < n; ++s) {
+    int32_t have = 0;
+    uint32_t a = 0;
+    const uint64_t len = rep.size();
+    while (have < m) {
+      uint32_t y0, y1;
+      threefry(key[0], key[1], (uint32_t)s, a++, y0, y1);
+      const int32_t t = rep[(size_t)(((uint64_t)y0 * len) >> 32)];
+      bool seen = false;
+      for (int32_t q = 0; q < have; ++q) seen |= (pick[q] == t);
+      if (!seen) pick[have++] = t;
+    }
+    for (int32_t q = 0; q < m; ++q, ++e) {
+      src[e] = (int32_t)s;
+      dst[e] = pick[q];
+      rep.push_back(pick[q]);
+    }
+    rep.insert(rep.end(), (size_t)m, (int32_t)s);
+  }
+  return e;
+}
+
+// Undirected CSR with ascending neighbour ids per row: both directions of
+// every edge, ordered by two stable counting sorts (by neighbour, then by
+// row), O(n + E).
+int abm_csr_undirected(int64_t n, int64_t n_edges, const int32_t *src,
+                       const int32_t *dst, int32_t *row_ptr, int32_t *col) {
+  ABM_REQUIRE(n >= 0 && n_edges >= 0, "abm_csr_undirected: bad sizes");
+  ABM_REQUIRE(2 * n_edges < (1ll << 31), "abm_csr_undirected: too many edges");
+  ABM_REQUIRE(row_ptr && (n_edges == 0 || (src && dst && col)),
+              "abm_csr_undirected: null pointer");
+  const int64_t nnz = 2 * n_edges;
+  for (int64_t e = 0; e < n_edges; ++e) {
+    if (src[e] < 0 || src[e] >= n || dst[e] < 0 || dst[e] >= n) {
+      set_error("abm_csr_undirected: edge endpoint out of range");
+      return kErrInvalid;
+    }
+  }
+  std::vector<int32_t> ra((size_t)nnz), ca((size_t)nnz);
+  // pass 1: counting sort of directed entries (r, c) by c
+  std::vector<int64_t> cnt((size_t)n + 1, 0);
+  for (int64_t e = 0; e < n_edges; ++e) {
+    cnt[(size_t)dst[e] + 1]++;
+    cnt[(size_t)src[e] + 1]++;
+  }
+  for (int64_t v = 0; v < n; ++v) cnt[(size_t)v + 1] += cnt[(size_t)v];
+  for (int64_t e = 0; e < n_edges; ++e) {
+    int64_t p = cnt[(size_t)dst[e]]++;
+    ra[(size_t)p] = src[e];
+    ca[(size_t)p] = dst[e];
+    p = cnt[(size_t)src[e]]++;
+    ra[(size_t)p] = dst[e];
+    ca[(size_t)p] = src[e];
+  }
+  // pass 2: stable counting sort by r
+  std::fill(cnt.begin(), cnt.end(), 0);
+  for (int64_t i = 0; i < nnz; ++i) cnt[(size_t)ra[(size_t)i] + 1]++;
+  for (int64_t v = 0; v < n; ++v) cnt[(size_t)v + 1] += cnt[(size_t)v];
+  for (int64_t v = 0; v <= n; ++v) row_ptr[v] = (int32_t)cnt[(size_t)v];
+  for (int64_t i = 0; i < nnz; ++i) col[cnt[(size_t)ra[(size_t)i]]++] = ca[(size_t)i];
+  return kOk;
+}
+
+// Rows whose degree exceeds `threshold`, ascending.  Returns the count; with
+// out == NULL only counts.
+int64_t abm_heavy_rows(int64_t n, const int32_t *row_ptr, int32_t threshold,
+                       int32_t *out) {
+  if (!row_ptr || n < 0) return -1;
+  int64_t k = 0;
+  for (int64_t v = 0; v < n; ++v) {
+    if (row_ptr[v + 1] - row_ptr[v] > threshold) {
+      if (out) out[k] = (int32_t)v;
+      ++k;
+    }
+  }
+  return k;
+}
+
+}  // extern "C"

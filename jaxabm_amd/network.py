@@ -1,0 +1,79 @@
+"""Bulk Network construction: Barabasi-Albert edge lists and undirected CSR.
+
+The reference's ``Network`` grows an ``(E, 2)`` int32 edge list one
+``jnp.concatenate`` per ``add_edge`` and answers ``get_neighbors`` with O(E)
+masks (jaxabm/agentpy.py:545-615).  Here the edge list is built in bulk by the
+native library and stored as CSR (``row_ptr[n+1]``, ``col[2E]``, rows
+ascending), which is also what ``get_neighbors`` returns for undirected
+networks (ascending unique ids).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from ._lib import check, lib
+
+
+def ba_edges(n: int, m: int, graph_seed: int) -> tuple[np.ndarray, np.ndarray]:
+    """Barabasi-Albert(n, m) edges from key ``PRNGKey(graph_seed)``."""
+    E = int(lib().abm_ba_num_edges(int(n), int(m)))
+    src = np.zeros(max(E, 1), np.int32)
+    dst = np.zeros(max(E, 1), np.int32)
+    key = np.array([0, int(graph_seed) & 0xFFFFFFFF], np.uint32)
+    got = int(lib().abm_ba_edges(int(n), int(m), key, src, dst))
+    if got != E:
+        check(1, "ba_edges")
+    return src[:E], dst[:E]
+
+
+def csr_undirected(n: int, src: np.ndarray, dst: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    src = np.ascontiguousarray(src, np.int32)
+    dst = np.ascontiguousarray(dst, np.int32)
+    E = len(src)
+    row_ptr = np.zeros(n + 1, np.int32)
+    col = np.zeros(max(2 * E, 1), np.int32)
+    check(lib().abm_csr_undirected(int(n), E, src if E else np.zeros(1, np.int32),
+                                   dst if E else np.zeros(1, np.int32), row_ptr, col),
+          "csr_undirected")
+    return row_ptr, col[: 2 * E]
+
+
+def heavy_rows(row_ptr: np.ndarray, threshold: int) -> np.ndarray:
+    n = len(row_ptr) - 1
+    rp = np.ascontiguousarray(row_ptr, np.int32)
+    k = int(lib().abm_heavy_rows(n, rp, int(threshold), None))
+    out = np.zeros(max(k, 1), np.int32)
+    import ctypes as C
+
+    lib().abm_heavy_rows(n, rp, int(threshold), C.c_void_p(out.ctypes.data))
+    return out[:k]
+
+
+@dataclass
+class DeviceCSR:
+    """An undirected graph resident in HBM."""
+
+    n: int
+    nnz: int
+    row_ptr: torch.Tensor  # int32 [n+1]
+    col: torch.Tensor  # int32 [nnz]
+    host_row_ptr: np.ndarray
+    host_col: np.ndarray
+
+    @classmethod
+    def from_host(cls, row_ptr: np.ndarray, col: np.ndarray, device="cuda") -> "DeviceCSR":
+        rp = torch.from_numpy(np.ascontiguousarray(row_ptr, np.int32)).to(device)
+        cl = torch.from_numpy(np.ascontiguousarray(col if len(col) else np.zeros(1, np.int32),
+                                                   np.int32)).to(device)
+        return cls(len(row_ptr) - 1, int(row_ptr[-1]), rp, cl, row_ptr, col)
+
+    def neighbors(self, v: int) -> np.ndarray:
+        return self.host_col[self.host_row_ptr[v]:self.host_row_ptr[v + 1]].copy()
+
+
+def ba_csr(n: int, m: int, graph_seed: int) -> tuple[np.ndarray, np.ndarray]:
+    src, dst = ba_edges(n, m, graph_seed)
+    return csr_undirected(n, src, dst)

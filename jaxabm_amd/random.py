@@ -1,0 +1,257 @@
+"""JAX-stream PRNG on the GPU: ``jax.random``-shaped functions over the HIP
+threefry kernels.
+
+Reference call sites replaced: jaxabm/model.py:46 (PRNGKey), 129/156/164/183
+(split), jaxabm/agent.py:115/156 (split(key, N)), jaxabm/agentpy.py:509-513
+(randint), jaxabm/analysis.py:81/87 (uniform, permutation).
+
+Keys are int32 tensors of shape (2,) holding the two uint32 words of a JAX
+raw key; draws reproduce JAX's bits in either stream mode:
+
+* ``MODE_PARTITIONABLE`` (default) -- ``jax_threefry_partitionable=True``,
+  the JAX >= 0.5 default;
+* ``MODE_ORIGINAL`` -- the JAX < 0.5 stream.
+
+Batched keys of shape (n, 2) are accepted by the ``*_each`` helpers, which are
+the vmap of the scalar draw over the keys (what AgentCollection.update does
+with the agent keys, jaxabm/agent.py:156-177).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import MODE_ORIGINAL, MODE_PARTITIONABLE, check, lib, ptr, stream_handle
+
+__all__ = [
+    "MODE_ORIGINAL", "MODE_PARTITIONABLE", "PRNGKey", "key_data", "split",
+    "bits", "uniform", "normal", "bernoulli", "randint", "permutation",
+    "split_each", "uniform_each", "normal_each", "bernoulli_each",
+    "randint_each", "bits_each", "set_threefry_partitionable", "default_mode",
+]
+
+_default_mode = MODE_PARTITIONABLE
+
+
+def set_threefry_partitionable(flag: bool) -> None:
+    """Select the JAX stream (mirrors ``jax.config.update('jax_threefry_partitionable', flag)``)."""
+    global _default_mode
+    _default_mode = MODE_PARTITIONABLE if flag else MODE_ORIGINAL
+
+
+def default_mode() -> int:
+    return _default_mode
+
+
+def _mode(mode):
+    return _default_mode if mode is None else int(mode)
+
+
+def _device(device=None):
+    _lib.require_gpu()
+    return torch.device(device) if device is not None else torch.device("cuda")
+
+
+def _shape(shape) -> tuple:
+    if shape is None:
+        return ()
+    if isinstance(shape, int):
+        return (shape,)
+    return tuple(int(s) for s in shape)
+
+
+def _numel(shape) -> int:
+    n = 1
+    for s in shape:
+        n *= s
+    return n
+
+
+def _as_key(key) -> torch.Tensor:
+    if isinstance(key, torch.Tensor):
+        if key.shape[-1:] != (2,):
+            raise ValueError(f"a PRNG key has trailing dimension 2, got shape {tuple(key.shape)}")
+        if not key.is_cuda:
+            key = key.to("cuda")
+        return key.to(torch.int32).contiguous()
+    arr = np.asarray(key, dtype=np.uint32)
+    if arr.shape[-1:] != (2,):
+        raise ValueError(f"a PRNG key has trailing dimension 2, got shape {arr.shape}")
+    return torch.from_numpy(arr.view(np.int32).copy()).to(_device())
+
+
+def PRNGKey(seed: int, device=None) -> torch.Tensor:
+    """``jax.random.PRNGKey(seed)`` for 32-bit seeds: ``[0, seed mod 2^32]``."""
+    s = int(seed) & 0xFFFFFFFF
+    host = np.array([0, s], dtype=np.uint32).view(np.int32)
+    return torch.from_numpy(host.copy()).to(_device(device))
+
+
+key = PRNGKey
+
+
+def key_data(key) -> np.ndarray:
+    """Host uint32 view of a key (or key array)."""
+    if isinstance(key, torch.Tensor):
+        return key.detach().to("cpu").contiguous().numpy().view(np.uint32).copy()
+    return np.asarray(key, dtype=np.uint32)
+
+
+def _single(key) -> torch.Tensor:
+    k = _as_key(key)
+    if k.dim() != 1:
+        raise ValueError("expected a single key of shape (2,); use the *_each helpers for key arrays")
+    return k
+
+
+def split(key, num: int = 2, mode=None) -> torch.Tensor:
+    """``jax.random.split(key, num)`` -> (num, 2)."""
+    k = _single(key)
+    out = torch.empty((int(num), 2), dtype=torch.int32, device=k.device)
+    check(lib().abm_split(ptr(k), int(num), ptr(out), _mode(mode), stream_handle()), "split")
+    return out
+
+
+def bits(key, shape=(), mode=None) -> torch.Tensor:
+    """``jax.random.bits(key, shape, uint32)`` (returned as int32 storage)."""
+    k = _single(key)
+    shp = _shape(shape)
+    out = torch.empty(shp, dtype=torch.int32, device=k.device)
+    check(lib().abm_random_bits32(ptr(k), _numel(shp), ptr(out), _mode(mode), stream_handle()), "bits")
+    return out
+
+
+def uniform(key, shape=(), dtype=torch.float32, minval=0.0, maxval=1.0, mode=None) -> torch.Tensor:
+    """``jax.random.uniform(key, shape, float32, minval, maxval)``."""
+    if dtype not in (torch.float32, None):
+        raise NotImplementedError("uniform: float32 only (the reference runs with jax_enable_x64 off)")
+    k = _single(key)
+    shp = _shape(shape)
+    out = torch.empty(shp, dtype=torch.float32, device=k.device)
+    check(lib().abm_uniform_f32(ptr(k), _numel(shp), float(minval), float(maxval), ptr(out),
+                                _mode(mode), stream_handle()), "uniform")
+    return out
+
+
+def normal(key, shape=(), dtype=torch.float32, mode=None) -> torch.Tensor:
+    """``jax.random.normal(key, shape, float32)``."""
+    k = _single(key)
+    shp = _shape(shape)
+    out = torch.empty(shp, dtype=torch.float32, device=k.device)
+    check(lib().abm_normal_f32(ptr(k), _numel(shp), ptr(out), _mode(mode), stream_handle()), "normal")
+    return out
+
+
+def bernoulli(key, p=0.5, shape=(), mode=None) -> torch.Tensor:
+    """``jax.random.bernoulli(key, p, shape)`` -> bool."""
+    k = _single(key)
+    shp = _shape(shape)
+    out = torch.empty(shp, dtype=torch.uint8, device=k.device)
+    check(lib().abm_bernoulli_u8(ptr(k), _numel(shp), float(p), ptr(out), _mode(mode),
+                                 stream_handle()), "bernoulli")
+    return out.bool()
+
+
+def randint(key, shape, minval, maxval, dtype=torch.int32, mode=None) -> torch.Tensor:
+    """``jax.random.randint(key, shape, minval, maxval, int32)``."""
+    k = _single(key)
+    shp = _shape(shape)
+    lo = int(np.clip(int(minval), -2**31, 2**31 - 1))
+    hi = int(np.clip(int(maxval), -2**31, 2**31 - 1))
+    out = torch.empty(shp, dtype=torch.int32, device=k.device)
+    check(lib().abm_randint_i32(ptr(k), _numel(shp), lo, hi, ptr(out), _mode(mode),
+                                stream_handle()), "randint")
+    return out
+
+
+def permutation(key, x, mode=None) -> torch.Tensor:
+    """``jax.random.permutation(key, n)`` for an integer n, or of a 1-D tensor."""
+    k = _single(key)
+    if isinstance(x, torch.Tensor):
+        n = x.shape[0]
+    else:
+        n = int(x)
+    out = torch.empty((n,), dtype=torch.int32, device=k.device)
+    if n:
+        nbytes = int(lib().abm_permutation_workspace(n))
+        ws = torch.empty((nbytes,), dtype=torch.uint8, device=k.device)
+        check(lib().abm_permutation(ptr(k), n, ptr(out), ptr(ws), nbytes, _mode(mode),
+                                    stream_handle()), "permutation")
+    if isinstance(x, torch.Tensor):
+        return x[out.long()]
+    return out
+
+
+# ---------------------------------------------------------------- vmapped draws
+def _keys(keys) -> torch.Tensor:
+    k = _as_key(keys)
+    return k.reshape(-1, 2)
+
+
+def split_each(keys, num: int = 2, mode=None) -> torch.Tensor:
+    """``vmap(lambda k: split(k, num))(keys)`` -> (n, num, 2)."""
+    k = _keys(keys)
+    out = torch.empty((k.shape[0], int(num), 2), dtype=torch.int32, device=k.device)
+    check(lib().abm_split_each(ptr(k), k.shape[0], int(num), ptr(out), _mode(mode),
+                               stream_handle()), "split_each")
+    return out
+
+
+def _draw_each(keys, shape, dist, lo, hi, dtype, mode):
+    k = _keys(keys)
+    shp = _shape(shape)
+    out = torch.empty((k.shape[0], *shp), dtype=dtype, device=k.device)
+    check(lib().abm_draw_each(ptr(k), k.shape[0], _numel(shp), dist, float(lo), float(hi),
+                              ptr(out), _mode(mode), stream_handle()), "draw_each")
+    return out
+
+
+def bits_each(keys, shape=(), mode=None):
+    return _draw_each(keys, shape, 0, 0.0, 0.0, torch.int32, mode)
+
+
+def uniform_each(keys, shape=(), minval=0.0, maxval=1.0, mode=None):
+    return _draw_each(keys, shape, 1, minval, maxval, torch.float32, mode)
+
+
+def normal_each(keys, shape=(), mode=None):
+    return _draw_each(keys, shape, 2, 0.0, 0.0, torch.float32, mode)
+
+
+def bernoulli_each(keys, p=0.5, shape=(), mode=None):
+    return _draw_each(keys, shape, 3, p, 0.0, torch.uint8, mode).bool()
+
+
+def randint_each(keys, shape, minval, maxval, mode=None):
+    k = _keys(keys)
+    shp = _shape(shape)
+    out = torch.empty((k.shape[0], *shp), dtype=torch.int32, device=k.device)
+    check(lib().abm_randint_each(ptr(k), k.shape[0], _numel(shp), int(minval), int(maxval),
+                                 ptr(out), _mode(mode), stream_handle()), "randint_each")
+    return out
+
+
+# ---------------------------------------------------------------- host key math
+def host_split(key, num: int = 2, mode=None) -> np.ndarray:
+    """Split a host uint32[2] key on the host (no device work) -> (num, 2)."""
+    k = np.ascontiguousarray(np.asarray(key, dtype=np.uint32).reshape(2))
+    out = np.zeros(2 * int(num), np.uint32)
+    check(lib().abm_host_split(k, int(num), out, _mode(mode)), "host_split")
+    return out.reshape(int(num), 2)
+
+
+def host_prng_key(seed: int) -> np.ndarray:
+    return np.array([0, int(seed) & 0xFFFFFFFF], dtype=np.uint32)
+
+
+def step_key_table(rng: np.ndarray, steps: int, n_collections: int, has_env: bool,
+                   split_env: bool, mode=None) -> np.ndarray:
+    """Host key schedule of ``steps`` JaxModel.step calls; advances ``rng`` in place."""
+    width = 2 * n_collections + ((4 if split_env else 2) if has_env else 0)
+    out = np.zeros(max(steps, 1) * max(width, 1), np.uint32)
+    check(lib().abm_step_keys(rng, int(steps), int(n_collections), int(bool(has_env)),
+                              int(bool(split_env)), _mode(mode), out), "step_keys")
+    return out[: steps * width].reshape(steps, width)

@@ -1,0 +1,75 @@
+"""GPU parity: network SIR program against the oracle, bit-exact."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+MODES = [O.MODE_ORIGINAL, O.MODE_PARTITIONABLE]
+
+
+@pytest.fixture(scope="module")
+def sir():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from jaxabm_amd.programs import sir
+
+    return sir
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("n,m,steps,beta,gamma,seed", [
+    (2000, 4, 60, 0.05, 0.1, 0),
+    (5000, 2, 40, 0.3, 0.05, 3),
+    (3000, 80, 25, 0.02, 0.2, 1),   # dense rows: every row is heavy
+    (1000, 4, 30, 0.0, 0.1, 2),     # beta = 0: no new infection
+    (1000, 4, 30, 1.0, 0.0, 2),     # beta = 1, gamma = 0
+])
+def test_sir_matches_oracle(sir, mode, n, m, steps, beta, gamma, seed):
+    row_ptr, col = O.ba_csr(n, m, graph_seed=seed + 100)
+    eng = sir.SIRNetworkEngine(n, csr=(row_ptr, col), beta=beta, gamma=gamma,
+                               n_seed_infected=10, seed=seed, mode=mode)
+    st, rng = O.sir_init(n, 10, seed, mode)
+    np.testing.assert_array_equal(eng.state_host(), st)
+    np.testing.assert_array_equal(eng.rng, rng)
+    want = O.sir_run(row_ptr, col, st, rng, beta, gamma, steps, mode)
+    got = eng.run(steps)
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(eng.state_host(), st)
+    np.testing.assert_array_equal(eng.rng, rng)
+    assert (got.sum(axis=1) == n).all()
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_sir_two_runs_continue(sir, mode):
+    """A second run() continues from the current state (jaxabm/model.py:227-235)."""
+    n = 3000
+    row_ptr, col = O.ba_csr(n, 4, graph_seed=9)
+    eng = sir.SIRNetworkEngine(n, csr=(row_ptr, col), n_seed_infected=20, seed=4, mode=mode)
+    st, rng = O.sir_init(n, 20, 4, mode)
+    a = eng.run(13)
+    b = eng.run(17)
+    want = O.sir_run(row_ptr, col, st, rng, 0.05, 0.1, 30, mode)
+    np.testing.assert_array_equal(np.concatenate([a, b]), want)
+
+
+def test_sir_full_size_invariants_and_prefix(sir):
+    """1M-agent BA(m=4) at full size: exact against the oracle on a prefix of
+    steps, then size-independent invariants over the full 365 steps."""
+    n, m = 1_000_000, 4
+    row_ptr, col = O.ba_csr(n, m, graph_seed=0)
+    assert row_ptr[-1] == 2 * m * (n - m)
+    eng = sir.SIRNetworkEngine(n, csr=(row_ptr, col), seed=0)
+    st, rng = O.sir_init(n, 100, 0, O.MODE_PARTITIONABLE)
+    want = O.sir_run(row_ptr, col, st, rng, 0.05, 0.1, 12, O.MODE_PARTITIONABLE)
+    got = eng.run(12)
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(eng.state_host(), st)
+    rest = eng.run(365 - 12)
+    allc = np.concatenate([got, rest])
+    assert (allc.sum(axis=1) == n).all()
+    assert (np.diff(allc[:, 0]) <= 0).all()   # S never grows
+    assert (np.diff(allc[:, 2]) >= 0).all()   # R never shrinks
