@@ -12,9 +12,10 @@ all inputs resident in HBM.
 Also reported:
   * roofline of the step kernel: algorithmic bytes per launch (46 B/agent-step
     x 1M = 45,999,844 B: state read + write, row_ptr, col, one neighbour-state
-    byte per CSR entry) over the kernel's average duration measured with HIP
-    events on the launch stream, against MI355X's 8 TB/s;
-  * the data-dependent bytes the early-exit kernel actually has to touch;
+    byte per CSR entry) over the kernel's average duration, taken as the
+    HIP-event time of the timed graph runs on their stream divided by the
+    number of step kernels, against MI355X's 8 TB/s;
+  * an upper bound of the data-dependent bytes the skipping kernel touches;
   * cpu_baseline: the C oracle (a restatement of the reference semantics, one
     thread) running the same full simulation on the host.
 """
@@ -110,16 +111,22 @@ def main() -> int:
         eng = SIRNetworkEngine(N_AGENTS, csr=(row_ptr, col), beta=BETA, gamma=GAMMA,
                                n_seed_infected=N_SEED, seed=SEED)
         init_state = eng.state.clone()
-        rng0 = eng.rng.copy()
+        init_bitmaps = eng.bitmaps.clone()
+        init_mark = eng.mark.clone()
         keys = eng.key_table(SIM_STEPS)
         counts = torch.zeros((SIM_STEPS, 3), dtype=torch.int32, device="cuda")
+        slots = eng.slot_buffer(SIM_STEPS)
     stream.synchronize()
     sh = C.c_void_p(stream.cuda_stream)
     nbytes = C.c_size_t(N_AGENTS)
+    nbytes_bm = C.c_size_t(eng.bitmaps.numel() * 4)
+    nbytes_mark = C.c_size_t(eng.mark.numel() * 2)
 
     def enqueue_run():
         check(lib().abm_memcpy_d2d(ptr(eng.state), ptr(init_state), nbytes, sh), "reset")
-        eng.launch(keys, SIM_STEPS, counts, stream=stream)
+        check(lib().abm_memcpy_d2d(ptr(eng.bitmaps), ptr(init_bitmaps), nbytes_bm, sh), "reset")
+        check(lib().abm_memcpy_d2d(ptr(eng.mark), ptr(init_mark), nbytes_mark, sh), "reset")
+        eng.launch(keys, SIM_STEPS, counts, slots=slots, t0=0, stream=stream)
 
     graph = None
     if not args.no_graph:
@@ -158,29 +165,43 @@ def main() -> int:
     assert np.array_equal(counts.cpu().numpy(), ref_counts)
     assert (ref_counts.sum(axis=1) == N_AGENTS).all()
 
-    # ---- per-kernel timing: one instrumented run, events around each step kernel
+    # ---- step-kernel average: the event-timed graph runs hold one reset copy
+    # pair, one slot clear and one count reduction besides the 365 step
+    # kernels; all of it is charged to the step kernel (conservative).
+    avg_kernel_s = gpu_s / (args.steps * SIM_STEPS)
+
+    # ---- data-dependent bytes: one instrumented step-by-step replay
     nnz = int(row_ptr[-1])
     deg = torch.from_numpy(np.diff(row_ptr).astype(np.int64)).to("cuda")
+    idx = torch.arange(N_AGENTS, device="cuda")
     with torch.cuda.stream(stream):
         eng.state.copy_(init_state)
-        c1 = torch.zeros((SIM_STEPS, 3), dtype=torch.int32, device="cuda")
-        bufs = [eng.state, eng.scratch]
-        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-               for _ in range(SIM_STEPS)]
+        eng.bitmaps.copy_(init_bitmaps)
+        eng.mark.copy_(init_mark)
+        sl1 = eng.slot_buffer(SIM_STEPS).zero_()
         dd_bytes = []
         for t in range(SIM_STEPS):
-            cur = bufs[t & 1]
-            s_mask = cur == 0
-            dd_bytes.append(2 * N_AGENTS + 8 * s_mask.sum() + 5 * (deg * s_mask).sum())
-            evs[t][0].record(stream)
-            check(lib().abm_sir_step(C.byref(eng._g), ptr(cur), ptr(bufs[(t + 1) & 1]),
-                                     ptr(keys[t]), eng.beta, eng.gamma, eng.mode, ptr(c1[t]), sh),
+            old = eng.state & 3
+            s_mask = old == 0
+            risk = s_mask & ((((eng.mark.to(torch.int32) - t) & 0xFFFF)) <= 8)
+            check(lib().abm_sir_step(C.byref(eng._g), ptr(eng.state), ptr(eng.mark),
+                                     ptr(eng.bitmaps[t & 1]), ptr(eng.bitmaps[(t + 1) & 1]), t,
+                                     ptr(keys[t]), eng.beta, eng.gamma, eng.mode,
+                                     ptr(sl1[t - 1]) if t else None, ptr(sl1[t]), sh),
                   "sir_step")
-            evs[t][1].record(stream)
+            new = eng.state & 3
+            marking = (new == 1) & ((old == 0) | (((t + idx) % 8) == 0))
+            # bytes the kernel touches (upper bound): state read, changed-state
+            # writes, marks of S agents, the bitmap staged once per XCD-resident
+            # copy and written once, row_ptr + col of scanned rows, and row_ptr
+            # + col + 2-byte mark stores of marking rows
+            dd_bytes.append(N_AGENTS + (new != old).sum() + 2 * s_mask.sum() + 8 * eng.words
+                            + 8 * risk.sum() + 4 * (deg * risk).sum()
+                            + 8 * marking.sum() + 6 * (deg * marking).sum())
+        c1 = torch.empty((SIM_STEPS, 3), dtype=torch.int32, device="cuda")
+        check(lib().abm_sir_reduce_counts(ptr(sl1), SIM_STEPS, ptr(c1), sh), "reduce")
     stream.synchronize()
-    kern_ms = [a.elapsed_time(b) for a, b in evs]
     assert np.array_equal(c1.cpu().numpy(), ref_counts)
-    avg_kernel_s = sum(kern_ms) / len(kern_ms) / 1e3
     algo = algorithmic_bytes(N_AGENTS, nnz)
     achieved = algo / avg_kernel_s / 1e9
     dd_avg = float(torch.stack(dd_bytes).double().mean().item())

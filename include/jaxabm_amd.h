@@ -89,31 +89,64 @@ int64_t abm_ba_edges(int64_t n, int32_t m, const uint32_t key[2], int32_t *src,
 /* Undirected CSR (both directions, rows ascending). */
 int abm_csr_undirected(int64_t n, int64_t n_edges, const int32_t *src, const int32_t *dst,
                        int32_t *row_ptr, int32_t *col);
-/* Rows with degree > threshold (ascending); out may be NULL to count. */
-int64_t abm_heavy_rows(int64_t n, const int32_t *row_ptr, int32_t threshold, int32_t *out);
+/* Contiguous agent chunks of about `target` cost (cost(v) = node_cost +
+ * degree(v), plus heavy_cost when degree(v) > heavy_deg) and at most
+ * max_nodes agents, with boundaries that are multiples of `align` (except the
+ * last); writes chunk_ptr[n_chunks+1] (may be NULL to count); returns
+ * n_chunks or -1. */
+int64_t abm_node_chunks(int64_t n, const int32_t *row_ptr, int32_t align, int64_t node_cost,
+                        int32_t heavy_deg, int64_t heavy_cost, int64_t target,
+                        int32_t max_nodes, int32_t *chunk_ptr);
 
 /* ---- network SIR program: the env-phase callback of JaxModel.step
- *      (jaxabm/model.py:182-190) + metrics_fn counts (model.py:194-200) ---- */
+ *      (jaxabm/model.py:182-190) + metrics_fn counts (model.py:194-200) ----
+ * Device state per instance: state uint8[n] (low 2 bits S=0/I=1/R=2, bit 2 =
+ * row degree > 32; read public codes with abm_sir_states), updated in place;
+ * mark uint16[n], neighbour-infection step marks that let a step skip S
+ * agents with no infected neighbour; and two infected-set bitmaps
+ * uint32[2][abm_sir_bitmap_words(n)] (buffer 0 holds the current one between
+ * runs).  Alignment: state 8 bytes, mark and bitmaps 16 bytes.  `t`
+ * arguments are absolute step indices. */
 typedef struct abm_sir_graph {
-  int64_t n;              /* agents */
-  int64_t nnz;            /* CSR entries (2 x undirected edges) */
-  const int32_t *row_ptr; /* device [n+1] */
-  const int32_t *col;     /* device [nnz] */
-  const int32_t *heavy;   /* device [n_heavy]: rows with degree > abm_sir_heavy_threshold() */
-  int64_t n_heavy;
+  int64_t n;                /* agents */
+  int64_t nnz;              /* CSR entries (2 x undirected edges) */
+  const int32_t *row_ptr;   /* device [n+1] */
+  const int32_t *col;       /* device [nnz] */
+  const int32_t *chunk_ptr; /* device [n_chunks+1]: abm_node_chunks with the
+                               parameters of abm_sir_chunk_params */
+  int64_t n_chunks;
 } abm_sir_graph;
-int abm_sir_heavy_threshold(void);
-/* state[n] = 0; state[perm[i]] = 1 for i < n0. */
-int abm_sir_seed(int64_t n, const int32_t *perm, int64_t n0, uint8_t *state, void *stream);
+/* Chunking the step kernel expects: alignment, per-agent cost, heavy-row
+ * degree and extra cost, chunks per block, the persistent block count
+ * (target cost = total / (blocks * chunks_per_block)) and the largest chunk
+ * in agents. */
+int abm_sir_chunk_params(int32_t *align, int64_t *node_cost, int32_t *heavy_deg,
+                         int64_t *heavy_cost, int32_t *chunks_per_block, int32_t *blocks,
+                         int32_t *max_nodes);
+/* Words of one infected-set bitmap (a multiple of 4). */
+int64_t abm_sir_bitmap_words(int64_t n);
+/* Count slots per step (one step's `slots` block holds slots*3 ints). */
+int abm_sir_slots(void);
+/* Initial state at step t0: all S with heavy-row flags, agents perm[0..n0)
+ * infected, marks and bitmap (buffer 0) built from it. */
+int abm_sir_seed(const abm_sir_graph *g, const int32_t *perm, int64_t n0, uint8_t *state,
+                 uint16_t *mark, uint32_t *bitmap, uint32_t t0, void *stream);
 /* Runs `steps` synchronous steps.  keys: device [steps][4] = (k_inf, k_rec)
- * per step (abm_step_keys with split_env).  counts: device [steps][3] S/I/R
- * after each step.  scratch: device [n]. */
-int abm_sir_run(const abm_sir_graph *g, uint8_t *state, uint8_t *scratch, const uint32_t *keys,
-                int32_t steps, float beta, float gamma, int mode, int32_t *counts, void *stream);
-/* One step in -> out (in != out); counts3 must be zeroed by the caller
- * (it accumulates S, I, R). */
-int abm_sir_step(const abm_sir_graph *g, const uint8_t *in, uint8_t *out, const uint32_t *keys4,
-                 float beta, float gamma, int mode, int32_t *counts3, void *stream);
+ * per step.  slots: device scratch [steps][abm_sir_slots()][3].  counts:
+ * device [steps][3] S/I/R after each step. */
+int abm_sir_run(const abm_sir_graph *g, uint8_t *state, uint16_t *mark, uint32_t *bitmaps,
+                uint32_t t0, const uint32_t *keys, int32_t steps, float beta, float gamma,
+                int mode, int32_t *slots, int32_t *counts, void *stream);
+/* One step: reads bitmap bm_in, writes bm_out, updates state in place;
+ * accumulates into a zeroed slots[abm_sir_slots()][3].  prev_slots: the
+ * previous step's slots (lets a step with no infected agent skip its work),
+ * or NULL. */
+int abm_sir_step(const abm_sir_graph *g, uint8_t *state, uint16_t *mark, const uint32_t *bm_in,
+                 uint32_t *bm_out, uint32_t t, const uint32_t *keys4, float beta, float gamma,
+                 int mode, const int32_t *prev_slots, int32_t *slots, void *stream);
+int abm_sir_reduce_counts(const int32_t *slots, int32_t steps, int32_t *counts, void *stream);
+/* out[i] = public S/I/R code of state[i]. */
+int abm_sir_states(const uint8_t *state, int64_t n, uint8_t *out, void *stream);
 /* ceil(p * 2^23): integer form of `uniform < p`. */
 uint32_t abm_unit_threshold(float p);
 int abm_fill_i32(int32_t *p, int64_t n, int32_t v, void *stream);

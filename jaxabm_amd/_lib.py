@@ -41,8 +41,8 @@ class SirGraph(C.Structure):
         ("nnz", C.c_int64),
         ("row_ptr", C.c_void_p),
         ("col", C.c_void_p),
-        ("heavy", C.c_void_p),
-        ("n_heavy", C.c_int64),
+        ("chunk_ptr", C.c_void_p),
+        ("n_chunks", C.c_int64),
     ]
 
 
@@ -70,12 +70,19 @@ SIGNATURES = {
     "abm_ba_num_edges": (_i64, [_i64, _i32]),
     "abm_ba_edges": (_i64, [_i64, _i32, _u32arr, _i32arr, _i32arr]),
     "abm_csr_undirected": (_int, [_i64, _i64, _i32arr, _i32arr, _i32arr, _i32arr]),
-    "abm_heavy_rows": (_i64, [_i64, _i32arr, _i32, _p]),
-    "abm_sir_heavy_threshold": (_int, []),
-    "abm_sir_seed": (_int, [_i64, _p, _i64, _p, _p]),
-    "abm_sir_run": (_int, [C.POINTER(SirGraph), _p, _p, _p, _i32, _f32, _f32,
-                           _int, _p, _p]),
-    "abm_sir_step": (_int, [C.POINTER(SirGraph), _p, _p, _p, _f32, _f32, _int, _p, _p]),
+    "abm_node_chunks": (_i64, [_i64, _i32arr, _i32, _i64, _i32, _i64, _i64, _i32, _p]),
+    "abm_sir_chunk_params": (_int, [C.POINTER(_i32), C.POINTER(_i64), C.POINTER(_i32),
+                                    C.POINTER(_i64), C.POINTER(_i32), C.POINTER(_i32),
+                                    C.POINTER(_i32)]),
+    "abm_sir_bitmap_words": (_i64, [_i64]),
+    "abm_sir_slots": (_int, []),
+    "abm_sir_seed": (_int, [C.POINTER(SirGraph), _p, _i64, _p, _p, _p, _u32, _p]),
+    "abm_sir_run": (_int, [C.POINTER(SirGraph), _p, _p, _p, _u32, _p, _i32, _f32, _f32,
+                           _int, _p, _p, _p]),
+    "abm_sir_step": (_int, [C.POINTER(SirGraph), _p, _p, _p, _p, _u32, _p, _f32, _f32,
+                            _int, _p, _p, _p]),
+    "abm_sir_reduce_counts": (_int, [_p, _i32, _p, _p]),
+    "abm_sir_states": (_int, [_p, _i64, _p, _p]),
     "abm_unit_threshold": (_u32, [_f32]),
     "abm_fill_i32": (_int, [_p, _i64, _i32, _p]),
     "abm_graph_begin": (_int, [_p]),

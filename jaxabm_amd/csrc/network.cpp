@@ -113,19 +113,36 @@ int abm_csr_undirected(int64_t n, int64_t n_edges, const int32_t *src,
   return kOk;
 }
 
-// Rows whose degree exceeds `threshold`, ascending.  Returns the count; with
-// out == NULL only counts.
-int64_t abm_heavy_rows(int64_t n, const int32_t *row_ptr, int32_t threshold,
-                       int32_t *out) {
-  if (!row_ptr || n < 0) return -1;
-  int64_t k = 0;
+// Contiguous node chunks of roughly equal cost (cost(v) = node_cost +
+// degree(v), plus heavy_cost when degree(v) > heavy_deg) and at most
+// max_nodes nodes, whose boundaries are multiples of `align` (except the last;
+// max_nodes must be a multiple of align): chunk c covers nodes chunk_ptr[c] ..
+// chunk_ptr[c+1].  Returns the number of chunks; chunk_ptr may be NULL to
+// count.
+int64_t abm_node_chunks(int64_t n, const int32_t *row_ptr, int32_t align,
+                        int64_t node_cost, int32_t heavy_deg, int64_t heavy_cost,
+                        int64_t target, int32_t max_nodes, int32_t *chunk_ptr) {
+  if (!row_ptr || n < 0 || align < 1 || target < 1 || max_nodes < align ||
+      max_nodes % align)
+    return -1;
+  int64_t c = 0, acc = 0, start = 0;
+  if (chunk_ptr) chunk_ptr[0] = 0;
   for (int64_t v = 0; v < n; ++v) {
-    if (row_ptr[v + 1] - row_ptr[v] > threshold) {
-      if (out) out[k] = (int32_t)v;
-      ++k;
+    const int64_t d = row_ptr[v + 1] - row_ptr[v];
+    acc += node_cost + d + (d > heavy_deg ? heavy_cost : 0);
+    const bool full = v + 1 - start >= max_nodes;
+    if ((acc >= target || full) && (v + 1) % align == 0 && v + 1 < n) {
+      ++c;
+      if (chunk_ptr) chunk_ptr[c] = (int32_t)(v + 1);
+      acc = 0;
+      start = v + 1;
     }
   }
-  return k;
+  if (n > 0) {
+    ++c;
+    if (chunk_ptr) chunk_ptr[c] = (int32_t)n;
+  }
+  return c;
 }
 
 }  // extern "C"

@@ -41,15 +41,22 @@ def csr_undirected(n: int, src: np.ndarray, dst: np.ndarray) -> tuple[np.ndarray
     return row_ptr, col[: 2 * E]
 
 
-def heavy_rows(row_ptr: np.ndarray, threshold: int) -> np.ndarray:
-    n = len(row_ptr) - 1
-    rp = np.ascontiguousarray(row_ptr, np.int32)
-    k = int(lib().abm_heavy_rows(n, rp, int(threshold), None))
-    out = np.zeros(max(k, 1), np.int32)
+def node_chunks(row_ptr: np.ndarray, align: int, node_cost: int, target: int,
+                max_nodes: int, heavy_deg: int = 1 << 30, heavy_cost: int = 0) -> np.ndarray:
+    """Chunk pointer over contiguous agent ranges of about ``target`` cost
+    (cost = node_cost + degree, plus heavy_cost for degree > heavy_deg)."""
     import ctypes as C
 
-    lib().abm_heavy_rows(n, rp, int(threshold), C.c_void_p(out.ctypes.data))
-    return out[:k]
+    rp = np.ascontiguousarray(row_ptr, np.int32)
+    n = len(rp) - 1
+    k = int(lib().abm_node_chunks(n, rp, int(align), int(node_cost), int(heavy_deg),
+                                  int(heavy_cost), int(target), int(max_nodes), None))
+    if k < 0:
+        check(1, "node_chunks")
+    out = np.zeros(k + 1, np.int32)
+    lib().abm_node_chunks(n, rp, int(align), int(node_cost), int(heavy_deg), int(heavy_cost),
+                          int(target), int(max_nodes), C.c_void_p(out.ctypes.data))
+    return out
 
 
 @dataclass

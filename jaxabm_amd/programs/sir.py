@@ -22,7 +22,7 @@ import torch
 
 from .. import random as jr
 from .._lib import SirGraph, check, lib, ptr, require_gpu, stream_handle
-from ..network import DeviceCSR, ba_csr, heavy_rows
+from ..network import DeviceCSR, ba_csr, node_chunks
 
 S, I, R = 0, 1, 2
 
@@ -44,13 +44,28 @@ class SIRNetworkEngine:
             csr = ba_csr(self.n, m, graph_seed)
         row_ptr, col = csr
         self.graph = DeviceCSR.from_host(row_ptr, col, self.device)
-        hv = heavy_rows(row_ptr, int(lib().abm_sir_heavy_threshold()))
-        self.heavy = torch.from_numpy(hv if len(hv) else np.zeros(1, np.int32)).to(self.device)
-        self.n_heavy = len(hv)
+        align, node_cost, heavy_deg, heavy_cost = C.c_int32(), C.c_int64(), C.c_int32(), C.c_int64()
+        per_block, blocks, max_nodes = C.c_int32(), C.c_int32(), C.c_int32()
+        lib().abm_sir_chunk_params(C.byref(align), C.byref(node_cost), C.byref(heavy_deg),
+                                   C.byref(heavy_cost), C.byref(per_block), C.byref(blocks),
+                                   C.byref(max_nodes))
+        deg = np.diff(row_ptr)
+        total = (node_cost.value * self.n + int(row_ptr[-1])
+                 + heavy_cost.value * int((deg > heavy_deg.value).sum()))
+        target = max(1, -(-total // (blocks.value * per_block.value)))
+        cp = node_chunks(row_ptr, align.value, node_cost.value, target, max_nodes.value,
+                         heavy_deg.value, heavy_cost.value)
+        self.chunks = torch.from_numpy(cp).to(self.device)
+        self.n_chunks = len(cp) - 1
         self._g = SirGraph(self.n, self.graph.nnz, self.graph.row_ptr.data_ptr(),
-                           self.graph.col.data_ptr(), self.heavy.data_ptr(), self.n_heavy)
+                           self.graph.col.data_ptr(), self.chunks.data_ptr(), self.n_chunks)
+        self.slots_per_step = int(lib().abm_sir_slots())
         self.state = torch.zeros(self.n, dtype=torch.uint8, device=self.device)
-        self.scratch = torch.zeros(self.n, dtype=torch.uint8, device=self.device)
+        self.words = int(lib().abm_sir_bitmap_words(self.n))
+        # two infected-set bitmaps; buffer 0 holds the current one between runs
+        self.bitmaps = torch.zeros((2, self.words), dtype=torch.int32, device=self.device)
+        self.mark = torch.zeros(self.n, dtype=torch.int16, device=self.device)
+        self.t = 0
         self.n_seed_infected = int(min(n_seed_infected, self.n))
         self.seed = int(seed)
         self.rng = np.zeros(2, np.uint32)
@@ -62,21 +77,37 @@ class SIRNetworkEngine:
             self.seed = int(seed)
         keys = jr.host_split(jr.host_prng_key(self.seed), 2, self.mode)
         self.rng = keys[0].copy()
+        self.t = 0
         init_key = torch.from_numpy(keys[1].view(np.int32).copy()).to(self.device)
         perm = jr.permutation(init_key, self.n, mode=self.mode)
-        check(lib().abm_sir_seed(self.n, ptr(perm), self.n_seed_infected, ptr(self.state),
-                                 stream_handle()), "sir_seed")
+        check(lib().abm_sir_seed(C.byref(self._g), ptr(perm), self.n_seed_infected,
+                                 ptr(self.state), ptr(self.mark), ptr(self.bitmaps), self.t,
+                                 stream_handle()),
+              "sir_seed")
 
     def key_table(self, steps: int) -> torch.Tensor:
         """Per-step (k_inf, k_rec) keys for the next ``steps`` steps; advances rng."""
         tab = jr.step_key_table(self.rng, steps, 1, True, True, self.mode)
         return torch.from_numpy(np.ascontiguousarray(tab[:, 2:6]).view(np.int32)).to(self.device)
 
-    def launch(self, keys: torch.Tensor, steps: int, counts: torch.Tensor, stream=None) -> None:
-        """Enqueue ``steps`` steps reading ``keys`` [steps, 4] and writing ``counts`` [steps, 3]."""
-        check(lib().abm_sir_run(C.byref(self._g), ptr(self.state), ptr(self.scratch), ptr(keys),
-                                int(steps), self.beta, self.gamma, self.mode, ptr(counts),
+    def slot_buffer(self, steps: int) -> torch.Tensor:
+        return torch.empty((max(steps, 1), self.slots_per_step, 3), dtype=torch.int32,
+                           device=self.device)
+
+    def launch(self, keys: torch.Tensor, steps: int, counts: torch.Tensor,
+               slots: torch.Tensor | None = None, t0: int | None = None, stream=None) -> None:
+        """Enqueue ``steps`` steps from absolute step ``t0`` (default: the
+        engine's clock, which then advances) reading ``keys`` [steps, 4] and
+        writing ``counts`` [steps, 3]."""
+        if slots is None:
+            slots = self.slot_buffer(steps)
+        start = self.t if t0 is None else int(t0)
+        check(lib().abm_sir_run(C.byref(self._g), ptr(self.state), ptr(self.mark),
+                                ptr(self.bitmaps), start & 0xFFFFFFFF, ptr(keys), int(steps),
+                                self.beta, self.gamma, self.mode, ptr(slots), ptr(counts),
                                 stream_handle(stream)), "sir_run")
+        if t0 is None:
+            self.t += int(steps)
 
     def run(self, steps: int) -> np.ndarray:
         """Run ``steps`` steps; returns host counts [steps, 3] (S, I, R)."""
@@ -85,5 +116,12 @@ class SIRNetworkEngine:
         self.launch(keys, steps, counts)
         return counts[:steps].cpu().numpy()
 
+    def states(self) -> torch.Tensor:
+        """Public S/I/R codes (uint8) on the device."""
+        out = torch.empty_like(self.state)
+        check(lib().abm_sir_states(ptr(self.state), self.n, ptr(out), stream_handle()),
+              "sir_states")
+        return out
+
     def state_host(self) -> np.ndarray:
-        return self.state.cpu().numpy()
+        return self.states().cpu().numpy()

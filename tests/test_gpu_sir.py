@@ -27,6 +27,7 @@ def sir():
     (3000, 80, 25, 0.02, 0.2, 1),   # dense rows: every row is heavy
     (1000, 4, 30, 0.0, 0.1, 2),     # beta = 0: no new infection
     (1000, 4, 30, 1.0, 0.0, 2),     # beta = 1, gamma = 0
+    (2000, 3, 300, 0.02, 0.01, 5),  # > 256 steps: risk marks wrap around
 ])
 def test_sir_matches_oracle(sir, mode, n, m, steps, beta, gamma, seed):
     row_ptr, col = O.ba_csr(n, m, graph_seed=seed + 100)

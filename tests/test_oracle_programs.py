@@ -1,0 +1,277 @@
+"""Pins of the oracle's model programs (CPU).
+
+Each program is checked against an independent dense numpy re-evaluation of
+one step built from the oracle's PRNG pieces (pinned in test_oracle_rng.py),
+and against invariants a wrong index or term would break:
+
+* BA/CSR: E = m(n-m), symmetric, sorted rows, no self loops or duplicates;
+* SIR: dense any-over-edges step vs the early-exit oracle; S+I+R = n,
+  S non-increasing, R non-decreasing;
+* wealth: dense bincount step; sum(wealth) = n, mean = 1, wealth >= 0; torus
+  positions in [0, 1];
+* Gini: pairwise |xi - xj| identity and the f32 rounding of the reference
+  formula;
+* Schelling: dense Moore-8 step; type counts conserved, one agent per cell;
+* Deffuant: sum of opinions conserved, dense pair step; adjoint vs central
+  finite differences of the float64 forward.  The differences converge to
+  the adjoint as h shrinks (7 digits at h = 1e-6); at h = 1e-3 the loss is too
+  curved (sigmoid gates of scale 0.02 compounded over the steps) for a
+  central difference to be accurate to better than ~20 %, so the pin uses
+  h = 1e-6 and checks the convergence sequence.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+MODES = [O.MODE_ORIGINAL, O.MODE_PARTITIONABLE]
+
+
+def step_keys(rng, mode):
+    rng2, sk = O.split(rng, 2, mode)
+    sk2, ck = O.split(sk, 2, mode)
+    _, uk = O.split(sk2, 2, mode)
+    return rng2, ck, uk
+
+
+# ---------------------------------------------------------------- network
+@pytest.mark.parametrize("n,m", [(2, 1), (5, 4), (6, 4), (1000, 4), (3000, 7)])
+def test_ba_csr_structure(n, m):
+    src, dst = O.ba_edges(n, m, O.prng_key(5))
+    assert len(src) == (m * (n - m) if n > m else n - 1)
+    assert (src != dst).all()
+    pairs = {(min(a, b), max(a, b)) for a, b in zip(src, dst)}
+    assert len(pairs) == len(src)
+    rp, col = O.csr_undirected(n, src, dst)
+    assert rp[-1] == 2 * len(src)
+    for v in range(n):
+        row = col[rp[v]:rp[v + 1]]
+        assert (np.diff(row) > 0).all()
+        for u in row:
+            assert v in col[rp[u]:rp[u + 1]]
+    if n > m:
+        assert (np.diff(rp)[m + 1:] >= m).all()
+
+
+# ---------------------------------------------------------------- SIR
+def dense_sir_step(rp, col, state, rng, beta, gamma, mode):
+    n, nnz = len(state), len(col)
+    rng, ck, uk = step_keys(rng, mode)
+    kinf, krec = O.split(uk, 2, mode)
+    u = O.uniform(kinf, nnz, mode=mode)
+    r = O.uniform(krec, n, mode=mode)
+    row = np.repeat(np.arange(n), np.diff(rp))
+    hit = (state[col] == 1) & (u < np.float32(beta))
+    any_hit = np.zeros(n, bool)
+    np.logical_or.at(any_hit, row, hit)
+    new = state.copy()
+    new[(state == 0) & any_hit] = 1
+    new[(state == 1) & (r < np.float32(gamma))] = 2
+    return new, rng
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_sir_matches_dense_step(mode):
+    n = 3000
+    rp, col = O.ba_csr(n, 4, 11)
+    st, rng = O.sir_init(n, 30, 2, mode)
+    assert (st == 1).sum() == 30
+    ref, ref_rng = st.copy(), rng.copy()
+    for t in range(25):
+        ref, ref_rng = dense_sir_step(rp, col, ref, ref_rng, 0.08, 0.1, mode)
+        c = O.sir_run(rp, col, st, rng, 0.08, 0.1, 1, mode)[0]
+        np.testing.assert_array_equal(st, ref)
+        np.testing.assert_array_equal(rng, ref_rng)
+        np.testing.assert_array_equal(c, np.bincount(ref, minlength=3))
+
+
+def test_sir_invariants():
+    n = 5000
+    rp, col = O.ba_csr(n, 3, 1)
+    st, rng = O.sir_init(n, 50, 7, O.MODE_PARTITIONABLE)
+    c = O.sir_run(rp, col, st, rng, 0.05, 0.1, 200, O.MODE_PARTITIONABLE)
+    assert (c.sum(1) == n).all()
+    assert (np.diff(c[:, 0]) <= 0).all() and (np.diff(c[:, 2]) >= 0).all()
+    assert c[-1, 2] > 0
+
+
+def test_sir_seeding_is_permutation_prefix():
+    for mode in MODES:
+        st, _ = O.sir_init(1000, 17, 4, mode)
+        ik = O.split(O.prng_key(4), 2, mode)[1]
+        np.testing.assert_array_equal(np.sort(np.nonzero(st)[0]), np.sort(O.permutation(ik, 1000, mode)[:17]))
+
+
+# ---------------------------------------------------------------- wealth
+def dense_wealth_step(x, y, w, rng, mode):
+    n = len(w)
+    rng, ck, uk = step_keys(rng, mode)
+    ak = O.split(ck, n, mode)
+    dx = np.array([O.normal(O.split(k, 2, mode)[0], 1, mode)[0] for k in ak], np.float32)
+    dy = np.array([O.normal(O.split(k, 2, mode)[1], 1, mode)[0] for k in ak], np.float32)
+    x = np.mod(x + np.float32(0.01) * dx, np.float32(1.0)).astype(np.float32)
+    y = np.mod(y + np.float32(0.01) * dy, np.float32(1.0)).astype(np.float32)
+    t = O.randint(uk, n, 0, n - 1, mode)
+    t = t + (t >= np.arange(n))
+    give = (w > 0).astype(np.int32)
+    w = w - give + np.bincount(t, weights=give, minlength=n).astype(np.int32)
+    return x, y, w, rng
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_wealth_matches_dense_step(mode):
+    n = 300
+    x, y, w, rng = O.wealth_init(n, 0, mode)
+    ak = O.split(O.split(O.prng_key(0), 2, mode)[1], n, mode)
+    np.testing.assert_array_equal(x, [O.uniform(O.split(k, 2, mode)[0], 1, mode=mode)[0] for k in ak])
+    rx, ry, rw, rr = x.copy(), y.copy(), w.copy(), rng.copy()
+    for _ in range(5):
+        rx, ry, rw, rr = dense_wealth_step(rx, ry, rw, rr, mode)
+        mean, gini = O.wealth_run(x, y, w, rng, 1, mode)
+        np.testing.assert_array_equal(w, rw)
+        np.testing.assert_array_equal(rng, rr)
+        # numpy's float32 mod equals jnp.mod (fmod plus divisor on sign change)
+        np.testing.assert_array_equal(x, rx)
+        np.testing.assert_array_equal(y, ry)
+        assert mean[0] == np.float32(1.0)
+        assert gini[0] == O.gini_i32(rw)
+
+
+def test_wealth_invariants():
+    n = 1000
+    x, y, w, rng = O.wealth_init(n, 0, O.MODE_PARTITIONABLE)
+    mean, gini = O.wealth_run(x, y, w, rng, 100, O.MODE_PARTITIONABLE)
+    assert w.sum() == n and (w >= 0).all()
+    assert (mean == np.float32(1.0)).all()
+    assert (gini > 0).all() and (gini < 1).all() and gini[-1] > gini[0]
+    assert (x >= 0).all() and (x <= 1).all() and (y >= 0).all() and (y <= 1).all()
+
+
+@pytest.mark.parametrize("v", [[1, 1, 1, 1], [0, 0, 0, 5], [3, 1, 4, 1, 5, 9, 2, 6], list(range(100)), [0, 0]])
+def test_gini_pairwise_identity(v):
+    v = np.array(v, np.int32)
+    n, s = len(v), int(v.sum())
+    if s == 0:
+        assert O.gini_i32(v) == 0.0
+        return
+    exact = np.abs(v[:, None].astype(np.int64) - v[None, :]).sum() / (2.0 * n * s)
+    assert abs(float(O.gini_i32(v)) - exact) < 5e-7
+    # the reference's float32 form: f32(2 S) / f32(n T) - f32((n + 1) / n)
+    xs = np.sort(v)
+    S = int((np.arange(1, n + 1) * xs).sum())
+    want = np.float32(np.float32(2 * S) / np.float32(n * s)) - np.float32((n + 1) / n)
+    assert O.gini_i32(v) == want
+
+
+# ---------------------------------------------------------------- Schelling
+def dense_schelling_step(H, W, cells, pos, typ, rng, mode):
+    rng, ck, uk = step_keys(rng, mode)
+    na = len(pos)
+    ak = O.split(ck, na, mode)
+    grid = cells.reshape(H, W)
+    occ = np.zeros((H, W), int)
+    same = {1: np.zeros((H, W), int), 2: np.zeros((H, W), int)}
+    for dr in (-1, 0, 1):
+        for dc in (-1, 0, 1):
+            if dr or dc:
+                sh = np.roll(np.roll(grid, -dr, 0), -dc, 1)
+                occ += sh != 0
+                for ty in (1, 2):
+                    same[ty] += sh == ty
+    r, c = pos // W, pos % W
+    n_occ = occ[r, c]
+    n_same = np.where(typ == 1, same[1][r, c], same[2][r, c])
+    happy = 8 * n_same >= 3 * n_occ
+    empty = np.nonzero(cells == 0)[0]
+    target = np.full(na, -1)
+    for k in np.nonzero(~happy)[0]:
+        j = O.randint(ak[k], 1, 0, len(empty), mode)[0]
+        target[k] = empty[j]
+    claim = {}
+    for k in range(na):
+        if target[k] >= 0 and target[k] not in claim:
+            claim[target[k]] = k
+    cells = cells.copy()
+    pos = pos.copy()
+    for cell, k in claim.items():
+        cells[pos[k]] = 0
+        cells[cell] = typ[k]
+        pos[k] = cell
+    return cells, pos, rng, np.float32(np.float32(happy.sum()) / np.float32(na)), len(claim)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_schelling_matches_dense_step(mode):
+    H, W = 24, 20
+    cells, pos, typ, rng = O.schelling_init(H, W, 3, mode)
+    u = O.uniform(O.prng_key(3), H * W, mode=mode)
+    np.testing.assert_array_equal(cells, np.where(u < np.float32(0.45), 1, np.where(u < np.float32(0.9), 2, 0)))
+    rc, rp, rr = cells.copy(), pos.copy(), rng.copy()
+    for _ in range(6):
+        rc, rp, rr, hf, mv = dense_schelling_step(H, W, rc, rp, typ, rr, mode)
+        h, m = O.schelling_run(H, W, cells, pos, typ, rng, 1, mode)
+        np.testing.assert_array_equal(cells, rc)
+        np.testing.assert_array_equal(pos, rp)
+        assert h[0] == hf and m[0] == mv
+
+
+def test_schelling_invariants():
+    H = W = 64
+    cells, pos, typ, rng = O.schelling_init(H, W, 0, O.MODE_PARTITIONABLE)
+    n1, n2 = (cells == 1).sum(), (cells == 2).sum()
+    hf, mv = O.schelling_run(H, W, cells, pos, typ, rng, 50, O.MODE_PARTITIONABLE)
+    assert (cells == 1).sum() == n1 and (cells == 2).sum() == n2
+    assert len(np.unique(pos)) == len(pos)
+    np.testing.assert_array_equal(cells[pos], typ)
+    assert hf[-1] > hf[0] and (mv >= 0).all()
+
+
+# ---------------------------------------------------------------- Deffuant
+def test_deffuant_dense_step_and_conservation():
+    n, B = 1000, 3
+    x0, rng = O.deffuant_init(n, B, 1, O.MODE_PARTITIONABLE)
+    eps = np.array([0.1, 0.2, 0.5], np.float32)
+    mu = np.array([0.3, 0.5, 0.1], np.float32)
+    x = x0.copy()
+    r2 = rng.copy()
+    O.deffuant_forward(x, eps, mu, r2, 1, O.MODE_PARTITIONABLE)
+    _, ck, uk = step_keys(rng, O.MODE_PARTITIONABLE)
+    perm = O.permutation(uk, n, O.MODE_PARTITIONABLE)
+    a, b = perm[0::2], perm[1::2]
+    d = x0[b] - x0[a]
+    g = 1.0 / (1.0 + np.exp(-((eps - np.abs(d)) / np.float32(0.02)).astype(np.float32)))
+    delta = (mu * g.astype(np.float32)) * d
+    want = x0.copy()
+    want[a] = x0[a] + delta
+    want[b] = x0[b] - delta
+    np.testing.assert_allclose(x, want, rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(x.sum(0), x0.sum(0), rtol=1e-5)
+
+
+def test_deffuant_adjoint_matches_finite_differences():
+    n, steps = 512, 40
+    x0, rng0 = O.deffuant_init(n, 1, 0, O.MODE_PARTITIONABLE)
+    xt = x0.copy()
+    O.deffuant_forward(xt, np.float32([0.2]), np.float32([0.3]), rng0.copy(), steps,
+                       O.MODE_PARTITIONABLE)
+    target = O.deffuant_stats(xt)[0]
+    eps, mu = np.array([0.27]), np.array([0.21])
+    loss, ge, gm = O.deffuant_grad(x0, eps, mu, rng0, steps, target, O.MODE_PARTITIONABLE)
+    def fd(h):
+        lp = O.deffuant_grad(x0, eps + h, mu, rng0, steps, target, O.MODE_PARTITIONABLE)[0]
+        lm = O.deffuant_grad(x0, eps - h, mu, rng0, steps, target, O.MODE_PARTITIONABLE)[0]
+        mp = O.deffuant_grad(x0, eps, mu + h, rng0, steps, target, O.MODE_PARTITIONABLE)[0]
+        mm = O.deffuant_grad(x0, eps, mu - h, rng0, steps, target, O.MODE_PARTITIONABLE)[0]
+        return (lp - lm) / (2 * h), (mp - mm) / (2 * h)
+
+    assert loss[0] > 0
+    fd_e, fd_m = fd(1e-6)
+    np.testing.assert_allclose(ge, fd_e, rtol=1e-5)
+    np.testing.assert_allclose(gm, fd_m, rtol=1e-5)
+    # the central difference converges to the adjoint as h shrinks
+    errs = [max(abs(fd(h)[0][0] - ge[0]), abs(fd(h)[1][0] - gm[0])) for h in (1e-3, 1e-4, 1e-5)]
+    assert errs[2] < errs[1] < errs[0]
+    # at the target parameters the loss (same seed, same pairing) is ~0
+    l0 = O.deffuant_grad(x0, np.array([0.2]), np.array([0.3]), rng0, steps, target,
+                         O.MODE_PARTITIONABLE)[0]
+    assert l0[0] < 1e-10
