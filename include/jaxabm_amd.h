@@ -151,6 +151,50 @@ int abm_sir_states(const uint8_t *state, int64_t n, uint8_t *out, void *stream);
 uint32_t abm_unit_threshold(float p);
 int abm_fill_i32(int32_t *p, int64_t n, int32_t v, void *stream);
 
+/* ---- wealth-exchange program: per-agent update of AgentCollection.update
+ *      (jaxabm/agent.py:132-177) + env-phase exchange + record-time mean and
+ *      Gini (examples/models/advanced_economic_model.py:1798-1811) ---- */
+typedef struct abm_wealth_inst {
+  int64_t n;                  /* agents, >= 2 */
+  float *x, *y;               /* device [n] */
+  int32_t *w;                 /* device [n] wealth */
+  int32_t *inc;               /* device [n] scratch, zero on entry, left zero */
+  int32_t *hist;              /* device [n+1] scratch, zero on entry, left zero */
+  const uint32_t *keys;       /* device [steps][4] = (ck, uk) per step */
+  int32_t steps;
+  float *mean_out, *gini_out; /* device [steps] */
+} abm_wealth_inst;
+/* x, y, wealth from the collection's init key (device uint32[2]). */
+int abm_wealth_init(const uint32_t *init_key, int64_t n, float *x, float *y, int32_t *w,
+                    int mode, void *stream);
+/* Runs every instance of the device array insts[count] for its steps, one
+ * workgroup per instance. */
+int abm_wealth_run_batch(const abm_wealth_inst *insts, int32_t count, int mode, void *stream);
+
+/* ---- Schelling program on a periodic Grid (jaxabm/agentpy.py:465-527
+ *      rebuilt with a Moore-8 step) ---- */
+typedef struct abm_schelling_grid {
+  int32_t height, width;
+  int64_t n_agents;      /* set by abm_schelling_init */
+  int8_t *cells;         /* device [H*W]: 0 empty, 1 / 2 agent type */
+  int32_t *id;           /* device [H*W]: agent on the cell or -1 */
+  int32_t *claim;        /* device [H*W]: INT32_MAX when unclaimed */
+  int32_t *pos;          /* device [H*W]: cell of agent k (first n_agents) */
+  int8_t *type;          /* device [H*W]: type of agent k */
+  int32_t *target;       /* device [H*W]: scratch */
+  int32_t *empty;        /* device [H*W]: scratch (empty-cell list) */
+  int32_t *block_counts; /* device [abm_schelling_block_count(H*W)] */
+} abm_schelling_grid;
+int64_t abm_schelling_block_count(int64_t n_cells);
+/* Cells from uniform(key, (H*W,)) and the row-major agent numbering; sets
+ * g->n_agents (synchronises the stream once). */
+int abm_schelling_init(abm_schelling_grid *g, const uint32_t *key, int mode, int32_t *n_agents_out,
+                       void *stream);
+/* Runs `steps` steps; keys: device [steps][2] collection keys ck.  counters:
+ * device scratch [steps][2]; happy_frac, moves: device [steps]. */
+int abm_schelling_run(const abm_schelling_grid *g, const uint32_t *keys, int32_t steps, int mode,
+                      int32_t *counters, float *happy_frac, int32_t *moves, void *stream);
+
 /* ---- step-loop capture (replaces the Python loop of JaxModel.run,
  *      jaxabm/model.py:240-241) ---- */
 typedef struct abm_graph abm_graph;

@@ -35,6 +35,15 @@ class AbmError(RuntimeError):
     """A non-zero status from the C-ABI, with the library's message."""
 
 
+class SchellingGrid(C.Structure):
+    _fields_ = [
+        ("height", C.c_int32), ("width", C.c_int32), ("n_agents", C.c_int64),
+        ("cells", C.c_void_p), ("id", C.c_void_p), ("claim", C.c_void_p),
+        ("pos", C.c_void_p), ("type", C.c_void_p), ("target", C.c_void_p),
+        ("empty", C.c_void_p), ("block_counts", C.c_void_p),
+    ]
+
+
 class SirGraph(C.Structure):
     _fields_ = [
         ("n", C.c_int64),
@@ -44,6 +53,13 @@ class SirGraph(C.Structure):
         ("chunk_ptr", C.c_void_p),
         ("n_chunks", C.c_int64),
     ]
+
+
+# Device-side descriptor layouts (numpy structured dtypes with C alignment).
+WEALTH_INST = np.dtype([("n", np.int64), ("x", np.uint64), ("y", np.uint64), ("w", np.uint64),
+                        ("inc", np.uint64), ("hist", np.uint64), ("keys", np.uint64),
+                        ("steps", np.int32), ("mean_out", np.uint64), ("gini_out", np.uint64)],
+                       align=True)
 
 
 # name -> (restype, argtypes).  Every symbol declared in include/jaxabm_amd.h
@@ -83,6 +99,11 @@ SIGNATURES = {
                             _int, _p, _p, _p]),
     "abm_sir_reduce_counts": (_int, [_p, _i32, _p, _p]),
     "abm_sir_states": (_int, [_p, _i64, _p, _p]),
+    "abm_wealth_init": (_int, [_p, _i64, _p, _p, _p, _int, _p]),
+    "abm_wealth_run_batch": (_int, [_p, _i32, _int, _p]),
+    "abm_schelling_block_count": (_i64, [_i64]),
+    "abm_schelling_init": (_int, [C.POINTER(SchellingGrid), _p, _int, C.POINTER(_i32), _p]),
+    "abm_schelling_run": (_int, [C.POINTER(SchellingGrid), _p, _i32, _int, _p, _p, _p, _p]),
     "abm_unit_threshold": (_u32, [_f32]),
     "abm_fill_i32": (_int, [_p, _i64, _i32, _p]),
     "abm_graph_begin": (_int, [_p]),

@@ -1,0 +1,323 @@
+// schelling.hip -- Schelling segregation on a periodic Grid.
+//
+// Program (reference Grid stores only shape and the periodic flag,
+// jaxabm/agentpy.py:465-527; step schedule jaxabm/model.py:146-216):
+//  init   u = uniform(PRNGKey(seed), (H*W,)); cell type 1 if u < 0.45, 2 if
+//         u < 0.9, else empty; agent k = k-th occupied cell in row-major order
+//  step   empty list = empty cells of the pre-step grid in row-major order;
+//         agent phase (agent keys = split(ck, n_agents)): Moore-8 on the torus,
+//         happy iff 8 * same >= 3 * occupied (same/occupied >= 0.375, an
+//         isolated agent is happy); an unhappy agent draws
+//         j = randint(agent_key, (), 0, n_empty) and claims empty[j];
+//         env phase: each claimed cell goes to its lowest-id claimant, winners
+//         move (their old cell is not reusable until the next step)
+//  record fraction happy (agent phase) and number of moves
+//
+// Kernels per step (captured with the rest of the run):
+//   k_cells_count / k_cells_scatter  row-major compaction of the empty cells
+//                                    (per-1024-cell block counts, then block
+//                                    offsets and an in-block scan)
+//   k_schelling_agents               32x32 cell tiles + 1-cell torus halo in
+//                                    LDS; happiness, draw, atomicMin claim
+//   k_schelling_moves                winners move; claims reset for the next step
+// Layout (HBM): cells int8[H*W], id int32[H*W] (agent on a cell or -1),
+// claim int32[H*W] (INT32_MAX when unclaimed), pos int32[n_agents],
+// type int8[n_agents], target int32[n_agents], empty int32[H*W].
+#include <hip/hip_runtime.h>
+
+#include <climits>
+
+#include "common.hpp"
+#include "jaxabm_amd.h"
+
+namespace abm {
+
+constexpr int kTileDim = 32;
+constexpr int kCellsPerBlock = 1024;
+
+// number of cells with (cells[c] == 0) == want_empty per 1024-cell block
+__global__ void __launch_bounds__(kCellsPerBlock)
+    k_cells_count(const int8_t *__restrict__ cells, int64_t nc, int want_empty,
+                  int32_t *__restrict__ block_counts) {
+  const int64_t c = (int64_t)blockIdx.x * kCellsPerBlock + threadIdx.x;
+  const bool hit = c < nc && ((cells[c] == 0) == (want_empty != 0));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __shared__ int32_t wc[16];
+  const uint64_t m = __ballot(hit);
+  if (lane == 0) wc[wave] = (int32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t s = 0;
+    for (int w = 0; w < 16; ++w) s += wc[w];
+    block_counts[blockIdx.x] = s;
+  }
+}
+
+// Writes the matching cells in row-major order: out[rank] = cell.  When
+// agent arrays are given (init), also numbers the agents: pos, type, id.
+__global__ void __launch_bounds__(kCellsPerBlock)
+    k_cells_scatter(const int8_t *__restrict__ cells, int64_t nc, int want_empty,
+                    const int32_t *__restrict__ block_counts, int32_t *__restrict__ out,
+                    int8_t *__restrict__ type, int32_t *__restrict__ id) {
+  __shared__ int32_t base;
+  __shared__ int32_t wc[16];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // block offset = sum of the counts of earlier blocks
+  int32_t part = 0;
+  for (int32_t b = tid; b < (int32_t)blockIdx.x; b += kCellsPerBlock) part += block_counts[b];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
+  if (lane == 0) wc[wave] = part;
+  __syncthreads();
+  if (tid == 0) {
+    int32_t s = 0;
+    for (int w = 0; w < 16; ++w) s += wc[w];
+    base = s;
+  }
+  __syncthreads();
+  const int64_t c = (int64_t)blockIdx.x * kCellsPerBlock + tid;
+  const bool hit = c < nc && ((cells[c] == 0) == (want_empty != 0));
+  const uint64_t m = __ballot(hit);
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  __syncthreads();
+  if (lane == 0) wc[wave] = (int32_t)__popcll(m);
+  __syncthreads();
+  int32_t before = base;
+  for (int w = 0; w < wave; ++w) before += wc[w];
+  if (hit) {
+    const int32_t r = before + (int32_t)__popcll(m & lt);
+    out[r] = (int32_t)c;
+    if (type) type[r] = cells[c];
+    if (id) id[c] = r;
+  } else if (id && c < nc) {
+    id[c] = -1;
+  }
+}
+
+struct SchellingStep {
+  int32_t H, W;
+  int64_t n_agents, n_empty;
+  const int8_t *cells;
+  const int32_t *id, *empty;
+  int32_t *claim, *target;
+  const uint32_t *ck;
+  int mode;
+  int32_t *happy_count;
+};
+
+__global__ void __launch_bounds__(kTileDim * kTileDim) k_schelling_agents(SchellingStep a) {
+  __shared__ int8_t tile[kTileDim + 2][kTileDim + 2];
+  __shared__ int32_t wc[16];
+  const int tx = threadIdx.x % kTileDim, ty = threadIdx.x / kTileDim;
+  const int r0 = blockIdx.y * kTileDim, c0 = blockIdx.x * kTileDim;
+  // stage the tile and its torus halo: 34 x 34 cells
+  for (int q = threadIdx.x; q < (kTileDim + 2) * (kTileDim + 2); q += blockDim.x) {
+    const int lr = q / (kTileDim + 2), lc = q % (kTileDim + 2);
+    int gr = r0 + lr - 1, gc = c0 + lc - 1;
+    gr = (gr % a.H + a.H) % a.H;
+    gc = (gc % a.W + a.W) % a.W;
+    tile[lr][lc] = a.cells[(int64_t)gr * a.W + gc];
+  }
+  __syncthreads();
+  const int r = r0 + ty, c = c0 + tx;
+  bool happy = false;
+  if (r < a.H && c < a.W) {
+    const int8_t me = tile[ty + 1][tx + 1];
+    if (me) {
+      int occ = 0, same = 0;
+#pragma unroll
+      for (int dr = -1; dr <= 1; ++dr)
+#pragma unroll
+        for (int dc = -1; dc <= 1; ++dc) {
+          if (!dr && !dc) continue;
+          const int8_t v = tile[ty + 1 + dr][tx + 1 + dc];
+          occ += v != 0;
+          same += v == me;
+        }
+      happy = 8 * same >= 3 * occ;
+      const int64_t cell = (int64_t)r * a.W + c;
+      const int32_t k = a.id[cell];
+      int32_t tg = -1;
+      if (!happy && a.n_empty > 0) {
+        Key k1, k2;
+        split2(split_at(Key{a.ck[0], a.ck[1]}, a.n_agents, k, a.mode), k1, k2, a.mode);
+        const uint32_t j = randint_offset(bits_scalar(k1, a.mode), bits_scalar(k2, a.mode),
+                                          (uint32_t)a.n_empty);
+        tg = a.empty[j];
+        atomicMin(&a.claim[tg], k);
+      }
+      a.target[k] = tg;
+    }
+  }
+  const uint64_t m = __ballot(happy);
+  if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = (int32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t s = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += wc[w];
+    if (s) atomicAdd(a.happy_count, s);
+  }
+}
+
+__global__ void k_schelling_moves(int64_t n_agents, int8_t *__restrict__ cells,
+                                  int32_t *__restrict__ id, int32_t *__restrict__ claim,
+                                  int32_t *__restrict__ pos, const int8_t *__restrict__ type,
+                                  const int32_t *__restrict__ target,
+                                  int32_t *__restrict__ moves) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool moved = false;
+  if (k < n_agents) {
+    const int32_t tg = target[k];
+    if (tg >= 0 && claim[tg] == (int32_t)k) {
+      const int32_t old = pos[k];
+      cells[old] = 0;
+      id[old] = -1;
+      cells[tg] = type[k];
+      id[tg] = (int32_t)k;
+      pos[k] = tg;
+      claim[tg] = INT_MAX;
+      moved = true;
+    }
+  }
+  __shared__ int32_t wc[4];
+  const uint64_t m = __ballot(moved);
+  if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = (int32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int32_t s = wc[0] + wc[1] + wc[2] + wc[3];
+    if (s) atomicAdd(moves, s);
+  }
+}
+
+// record[t] = (happy fraction, moves) from the integer counters
+__global__ void k_schelling_record(const int32_t *__restrict__ counters, int32_t steps,
+                                   int64_t n_agents, float *__restrict__ happy_frac,
+                                   int32_t *__restrict__ moves) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= steps) return;
+  happy_frac[t] = (float)counters[2 * t] / (float)n_agents;
+  moves[t] = counters[2 * t + 1];
+}
+
+__global__ void k_schelling_init_cells(const uint32_t *__restrict__ key, int64_t nc,
+                                       int8_t *__restrict__ cells, int32_t *__restrict__ claim,
+                                       int mode) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const float u = unit_float(bits_at(Key{key[0], key[1]}, nc, c, mode));
+  cells[c] = u < 0.45f ? 1 : (u < 0.9f ? 2 : 0);
+  claim[c] = INT_MAX;
+}
+
+__global__ void k_fill_i32_s(int32_t *__restrict__ p, int64_t n, int32_t v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+int check_grid(const abm_schelling_grid *g, const char *who) {
+  if (!g || g->height < 1 || g->width < 1 ||
+      (int64_t)g->height * g->width >= (1ll << 31) || !g->cells || !g->id || !g->claim ||
+      !g->pos || !g->type || !g->target || !g->empty || !g->block_counts ||
+      g->n_agents < 0 || g->n_agents > (int64_t)g->height * g->width) {
+    set_error(std::string(who) + ": bad grid descriptor");
+    return kErrInvalid;
+  }
+  return kOk;
+}
+
+}  // namespace abm
+
+using namespace abm;
+
+extern "C" {
+
+int64_t abm_schelling_block_count(int64_t n_cells) {
+  return (n_cells + kCellsPerBlock - 1) / kCellsPerBlock;
+}
+
+int abm_schelling_init(abm_schelling_grid *g, const uint32_t *key, int mode,
+                       int32_t *n_agents_out, void *stream) {
+  if (!g || g->height < 1 || g->width < 1) {
+    set_error("abm_schelling_init: bad grid");
+    return kErrInvalid;
+  }
+  ABM_REQUIRE(key && n_agents_out && (mode == 0 || mode == 1),
+              "abm_schelling_init: bad arguments");
+  const int64_t nc = (int64_t)g->height * g->width;
+  g->n_agents = 0;
+  if (int rc = check_grid(g, "abm_schelling_init")) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_schelling_init_cells, dim3(nblk(nc, 256)), dim3(256), 0, s, key, nc,
+                     g->cells, g->claim, mode);
+  const unsigned nb = nblk(nc, kCellsPerBlock);
+  hipLaunchKernelGGL(k_cells_count, dim3(nb), dim3(kCellsPerBlock), 0, s, g->cells, nc, 0,
+                     g->block_counts);
+  hipLaunchKernelGGL(k_cells_scatter, dim3(nb), dim3(kCellsPerBlock), 0, s, g->cells, nc, 0,
+                     g->block_counts, g->pos, g->type, g->id);
+  ABM_LAUNCH_CHECK("abm_schelling_init");
+  // agent count: sum of the block counts (host read; set-up only)
+  int32_t *host = new int32_t[nb];
+  hipError_t e = hipMemcpyAsync(host, g->block_counts, nb * sizeof(int32_t),
+                                hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    delete[] host;
+    return hip_fail(e, "abm_schelling_init");
+  }
+  int64_t na = 0;
+  for (unsigned b = 0; b < nb; ++b) na += host[b];
+  delete[] host;
+  g->n_agents = na;
+  *n_agents_out = (int32_t)na;
+  return kOk;
+}
+
+int abm_schelling_run(const abm_schelling_grid *g, const uint32_t *keys, int32_t steps,
+                      int mode, int32_t *counters, float *happy_frac, int32_t *moves,
+                      void *stream) {
+  if (int rc = check_grid(g, "abm_schelling_run")) return rc;
+  ABM_REQUIRE(keys && counters && happy_frac && moves && steps >= 0 &&
+                  (mode == 0 || mode == 1),
+              "abm_schelling_run: bad arguments");
+  if (steps == 0) return kOk;
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t nc = (int64_t)g->height * g->width;
+  const int64_t n_empty = nc - g->n_agents;
+  const unsigned nb = nblk(nc, kCellsPerBlock);
+  hipLaunchKernelGGL(k_fill_i32_s, dim3(nblk(2 * (int64_t)steps, 256)), dim3(256), 0, s,
+                     counters, 2 * (int64_t)steps, 0);
+  const dim3 tiles((unsigned)((g->width + kTileDim - 1) / kTileDim),
+                   (unsigned)((g->height + kTileDim - 1) / kTileDim));
+  for (int32_t t = 0; t < steps; ++t) {
+    hipLaunchKernelGGL(k_cells_count, dim3(nb), dim3(kCellsPerBlock), 0, s, g->cells, nc, 1,
+                       g->block_counts);
+    hipLaunchKernelGGL(k_cells_scatter, dim3(nb), dim3(kCellsPerBlock), 0, s, g->cells, nc, 1,
+                       g->block_counts, g->empty, (int8_t *)nullptr, (int32_t *)nullptr);
+    SchellingStep a;
+    a.H = g->height;
+    a.W = g->width;
+    a.n_agents = g->n_agents;
+    a.n_empty = n_empty;
+    a.cells = g->cells;
+    a.id = g->id;
+    a.empty = g->empty;
+    a.claim = g->claim;
+    a.target = g->target;
+    a.ck = keys + 2 * (int64_t)t;
+    a.mode = mode;
+    a.happy_count = counters + 2 * (int64_t)t;
+    hipLaunchKernelGGL(k_schelling_agents, tiles, dim3(kTileDim * kTileDim), 0, s, a);
+    if (g->n_agents > 0)
+      hipLaunchKernelGGL(k_schelling_moves, dim3(nblk(g->n_agents, 256)), dim3(256), 0, s,
+                         g->n_agents, g->cells, g->id, g->claim, g->pos, g->type, g->target,
+                         counters + 2 * (int64_t)t + 1);
+  }
+  hipLaunchKernelGGL(k_schelling_record, dim3(nblk(steps, 256)), dim3(256), 0, s, counters,
+                     steps, g->n_agents > 0 ? g->n_agents : 1, happy_frac, moves);
+  ABM_LAUNCH_CHECK("abm_schelling_run");
+  return kOk;
+}
+
+}  // extern "C"

@@ -1,0 +1,176 @@
+// wealth.hip -- wealth-exchange random walk: one workgroup runs a whole
+// simulation of one instance; a launch runs a batch of instances.
+//
+// Program (reference API, jaxabm/model.py:118-216, jaxabm/agent.py:92-177):
+//  init   agent keys = split(init_key, n); kx, ky = split(agent_key);
+//         x = uniform(kx), y = uniform(ky), wealth = 1
+//  step   agent phase (agent keys = split(ck, n)): k1, k2 = split(agent_key);
+//           x = mod(x + 0.01 * normal(k1), 1), y = mod(y + 0.01 * normal(k2), 1)
+//         env phase (uk): t = randint(uk, (n,), 0, n-1); t += (t >= i);
+//           give = wealth > 0; wealth += bincount(t, give) - give
+//  record mean(wealth) and Gini (examples/models/advanced_economic_model.py:
+//         1798-1811): x = sort(wealth); S = sum(i * x_i) and T = sum(x) in
+//         int32; G = f32(2 S) / f32(n T) - f32((n+1)/n) if T > 1e-6.
+//
+// The sort is replaced by a histogram over wealth values: with c_w agents of
+// wealth w and b_w agents below, their ranks sum to c_w * b_w + c_w(c_w+1)/2,
+// so S = sum_w w * ranksum_w (mod 2^32, the int32 wrap of the reference).
+// Whole run in one launch: every phase boundary is a workgroup barrier.
+// Scatter-adds are integer atomics (exact, order-free); the histogram and
+// increments are cleared by the kernel after use.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "jaxabm_amd.h"
+
+namespace abm {
+
+constexpr int kWealthThreads = 1024;
+
+__device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t *red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  uint32_t s = 0;
+  if (threadIdx.x < 64) {
+    s = threadIdx.x < (blockDim.x >> 6) ? red[threadIdx.x] : 0u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  }
+  __syncthreads();
+  return s;  // valid in wave 0
+}
+
+__global__ void __launch_bounds__(kWealthThreads)
+    k_wealth_run(const abm_wealth_inst *__restrict__ insts, int mode) {
+  const abm_wealth_inst in = insts[blockIdx.x];
+  const int64_t n = in.n;
+  const int tid = threadIdx.x;
+  __shared__ uint32_t red[16];
+  __shared__ uint32_t carry;
+  const uint32_t span = n > 1 ? (uint32_t)(n - 1) : 1u;
+  for (int32_t t = 0; t < in.steps; ++t) {
+    const Key ck{in.keys[4 * t + 0], in.keys[4 * t + 1]};
+    const Key uk{in.keys[4 * t + 2], in.keys[4 * t + 3]};
+    Key kk1, kk2;
+    split2(uk, kk1, kk2, mode);
+    // agent phase + exchange targets
+    for (int64_t i = tid; i < n; i += kWealthThreads) {
+      Key k1, k2;
+      split2(split_at(ck, n, i, mode), k1, k2, mode);
+      const float dx = normal_from_bits(bits_scalar(k1, mode));
+      const float dy = normal_from_bits(bits_scalar(k2, mode));
+      const float sx = in.x[i] + 0.01f * dx;
+      const float sy = in.y[i] + 0.01f * dy;
+      in.x[i] = mod1(sx);
+      in.y[i] = mod1(sy);
+      if (in.w[i] > 0) {
+        uint32_t ti = randint_offset(bits_at(kk1, n, i, mode), bits_at(kk2, n, i, mode), span);
+        ti += ti >= (uint32_t)i ? 1u : 0u;
+        atomicAdd(&in.inc[ti], 1);
+      }
+    }
+    __syncthreads();
+    // apply the exchange, histogram of wealth values
+    uint32_t total = 0;
+    for (int64_t i = tid; i < n; i += kWealthThreads) {
+      const int32_t w0 = in.w[i];
+      const int32_t wi = w0 - (w0 > 0 ? 1 : 0) + in.inc[i];
+      in.w[i] = wi;
+      in.inc[i] = 0;
+      atomicAdd(&in.hist[wi], 1);  // 0 <= wi <= n: wealth is conserved
+      total += (uint32_t)wi;
+    }
+    const uint32_t T = block_sum_u32(total, red);
+    __syncthreads();
+    // S = sum_w w * (c_w b_w + c_w (c_w + 1) / 2): chunked exclusive scan of
+    // the histogram, one bin per thread per chunk
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    uint32_t S_part = 0;
+    for (int64_t base = 0; base <= n; base += kWealthThreads) {
+      const int64_t w = base + tid;
+      const uint32_t c = w <= n ? (uint32_t)in.hist[w] : 0u;
+      // inclusive wave scan of c, then across waves
+      uint32_t inc = c;
+      const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(inc, off);
+        if (lane >= off) inc += y;
+      }
+      if (lane == 63) red[wave] = inc;
+      __syncthreads();
+      uint32_t wave_off = 0;
+      for (int q = 0; q < wave; ++q) wave_off += red[q];
+      const uint32_t before = carry + wave_off + inc - c;
+      if (c) {
+        const uint64_t ranksum = (uint64_t)c * before + (uint64_t)c * (c + 1) / 2;
+        S_part += (uint32_t)w * (uint32_t)ranksum;
+        in.hist[w] = 0;
+      }
+      __syncthreads();
+      if (tid == kWealthThreads - 1) carry = before + c;
+      __syncthreads();
+    }
+    const uint32_t S = block_sum_u32(S_part, red);
+    if (tid == 0) {
+      in.mean_out[t] = (float)(int32_t)T / (float)n;
+      const int32_t Ti = (int32_t)T;
+      float g = 0.0f;
+      if ((float)Ti > 1e-6f) {
+        const int32_t num = (int32_t)(2u * S);
+        const int32_t den = (int32_t)((uint32_t)n * T);
+        const float q = (float)num / (float)den;
+        const float cst = (float)((double)(n + 1) / (double)n);
+        g = q - cst;
+      }
+      in.gini_out[t] = g;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_wealth_init(const uint32_t *__restrict__ init_key, int64_t n,
+                              float *__restrict__ x, float *__restrict__ y,
+                              int32_t *__restrict__ w, int mode) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Key kx, ky;
+  split2(split_at(Key{init_key[0], init_key[1]}, n, i, mode), kx, ky, mode);
+  x[i] = unit_float(bits_scalar(kx, mode));
+  y[i] = unit_float(bits_scalar(ky, mode));
+  w[i] = 1;
+}
+
+}  // namespace abm
+
+using namespace abm;
+
+extern "C" {
+
+int abm_wealth_init(const uint32_t *init_key, int64_t n, float *x, float *y, int32_t *w,
+                    int mode, void *stream) {
+  ABM_REQUIRE(init_key && x && y && w && n > 1 && n < (1ll << 31) &&
+                  (mode == 0 || mode == 1),
+              "abm_wealth_init: bad arguments");
+  hipLaunchKernelGGL(k_wealth_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, init_key, n, x, y, w, mode);
+  ABM_LAUNCH_CHECK("abm_wealth_init");
+  return kOk;
+}
+
+int abm_wealth_run_batch(const abm_wealth_inst *insts, int32_t count, int mode,
+                         void *stream) {
+  ABM_REQUIRE(insts && count >= 0 && (mode == 0 || mode == 1),
+              "abm_wealth_run_batch: bad arguments");
+  if (count == 0) return kOk;
+  hipLaunchKernelGGL(k_wealth_run, dim3((unsigned)count), dim3(kWealthThreads), 0,
+                     (hipStream_t)stream, insts, mode);
+  ABM_LAUNCH_CHECK("abm_wealth_run_batch");
+  return kOk;
+}
+
+}  // extern "C"

@@ -1,0 +1,57 @@
+"""GPU parity: Schelling program against the oracle, bit-exact (grid,
+positions, fraction happy, moves)."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+MODES = [O.MODE_ORIGINAL, O.MODE_PARTITIONABLE]
+
+
+@pytest.fixture(scope="module")
+def schelling():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from jaxabm_amd.programs import schelling
+
+    return schelling
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("H,W,steps,seed", [(64, 64, 30, 0), (100, 37, 20, 3), (5, 7, 10, 1),
+                                            (128, 96, 12, 2)])
+def test_schelling_matches_oracle(schelling, mode, H, W, steps, seed):
+    eng = schelling.SchellingEngine(H, W, seed=seed, mode=mode)
+    cells, pos, typ, rng = O.schelling_init(H, W, seed, mode)
+    assert eng.n_agents == len(pos)
+    np.testing.assert_array_equal(eng.grid().reshape(-1), cells)
+    np.testing.assert_array_equal(eng.positions(), pos)
+    hf, mv = eng.run(steps)
+    ohf, omv = O.schelling_run(H, W, cells, pos, typ, rng, steps, mode)
+    np.testing.assert_array_equal(mv, omv)
+    np.testing.assert_array_equal(hf, ohf)
+    np.testing.assert_array_equal(eng.grid().reshape(-1), cells)
+    np.testing.assert_array_equal(eng.positions(), pos)
+    np.testing.assert_array_equal(eng.rng, rng)
+
+
+def test_schelling_full_size(schelling):
+    """512 x 512 torus, 200 steps: exact against the oracle on a prefix,
+    conservation invariants over the whole run."""
+    H = W = 512
+    eng = schelling.SchellingEngine(H, W, seed=0)
+    cells, pos, typ, rng = O.schelling_init(H, W, 0, O.MODE_PARTITIONABLE)
+    hf, mv = eng.run(10)
+    ohf, omv = O.schelling_run(H, W, cells, pos, typ, rng, 10, O.MODE_PARTITIONABLE)
+    np.testing.assert_array_equal(hf, ohf)
+    np.testing.assert_array_equal(mv, omv)
+    counts = np.bincount(eng.grid().reshape(-1), minlength=3)
+    hf2, _ = eng.run(190)
+    np.testing.assert_array_equal(np.bincount(eng.grid().reshape(-1), minlength=3), counts)
+    p = eng.positions()
+    assert len(np.unique(p)) == len(p)
+    assert hf2[-1] > hf[0]
