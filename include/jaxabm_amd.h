@@ -68,6 +68,11 @@ size_t abm_permutation_workspace(int64_t n);
 int abm_permutation_rounds(int64_t n);
 int abm_permutation(const uint32_t *key, int64_t n, int32_t *out, void *ws, size_t ws_bytes,
                     int mode, void *stream);
+/* permutation(keys[s * key_stride .. +2], n) for s < count into out[count][n]
+ * (all steps' pairings of a run in one batched sort). */
+size_t abm_permutation_batch_workspace(int64_t n, int32_t count);
+int abm_permutation_batch(const uint32_t *keys, int64_t key_stride, int32_t count, int64_t n,
+                          int32_t *out, void *ws, size_t ws_bytes, int mode, void *stream);
 /* Stable ascending sort of (keys, vals) in place; workspace as permutation. */
 int abm_sort_pairs_u32(uint32_t *keys, int32_t *vals, int64_t n, void *ws, size_t ws_bytes,
                        void *stream);
@@ -194,6 +199,46 @@ int abm_schelling_init(abm_schelling_grid *g, const uint32_t *key, int mode, int
  * device scratch [steps][2]; happy_frac, moves: device [steps]. */
 int abm_schelling_run(const abm_schelling_grid *g, const uint32_t *keys, int32_t steps, int mode,
                       int32_t *counters, float *happy_frac, int32_t *moves, void *stream);
+
+/* ---- differentiable Deffuant model, batch of (eps, mu) instances
+ *      (loss and gradient set-up jaxabm/analysis.py:489-531; step schedule
+ *      jaxabm/model.py:146-216) ---- */
+typedef struct abm_deffuant_problem {
+  int64_t n;               /* agents */
+  int32_t batch;           /* instances B */
+  int32_t steps;
+  int32_t ckpt_every;      /* checkpoint interval k */
+  int32_t pair_blocks;     /* workgroup rows of the step grids (<= 65535) */
+  int32_t stat_chunks;     /* row chunks of the statistics pass (<= 65535) */
+  int32_t reserved;
+  const int32_t *perms;    /* device [steps][n]: pairing of each step */
+  const float *eps, *mu;   /* device [B] */
+  float *work;             /* device [n][B]: final state after the forward */
+  float *ckpt;             /* device [ceil(steps/k)][n][B]; ckpt[0] = initial state */
+  float *traj;             /* device [k-1][n][B] (backward) */
+  float *lam;              /* device [n][B] (backward) */
+  double *acc;             /* device [pair_blocks][2][B] (backward) */
+  double *partial;         /* device [stat_chunks][34][B] */
+  double *sums;            /* device [34][B] */
+  double *coef;            /* device [34][B] */
+  const double *target;    /* device [34]: target [mean, var, hist_0..31] */
+  double *stats;           /* device [B][34] */
+  double *loss;            /* device [B] */
+  double *grads;           /* device [2][B]: dL/deps, dL/dmu (backward) */
+} abm_deffuant_problem;
+/* x[i][b] = uniform(split(init_key, n)[i]) for every instance b. */
+int abm_deffuant_init(const uint32_t *init_key, int64_t n, int32_t batch, float *x, int mode,
+                      void *stream);
+/* One step of the whole batch with pairing perm[n]; src == dst in place. */
+int abm_deffuant_step(const float *src, float *dst, int64_t n, int32_t batch,
+                      const int32_t *perm, const float *eps, const float *mu,
+                      int32_t pair_blocks, void *stream);
+/* Steps from ckpt[0], keeping the checkpoints; then stats, loss and the
+ * dL/dx coefficients of the final state. */
+int abm_deffuant_forward(const abm_deffuant_problem *p, void *stream);
+/* Reverse sweep after abm_deffuant_forward on the same descriptor:
+ * recomputes each checkpoint segment, then grads = dL/d(eps, mu). */
+int abm_deffuant_backward(const abm_deffuant_problem *p, void *stream);
 
 /* ---- step-loop capture (replaces the Python loop of JaxModel.run,
  *      jaxabm/model.py:240-241) ---- */

@@ -44,6 +44,19 @@ class SchellingGrid(C.Structure):
     ]
 
 
+class DeffuantProblem(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64), ("batch", C.c_int32), ("steps", C.c_int32),
+        ("ckpt_every", C.c_int32), ("pair_blocks", C.c_int32), ("stat_chunks", C.c_int32),
+        ("reserved", C.c_int32),
+        ("perms", C.c_void_p), ("eps", C.c_void_p), ("mu", C.c_void_p), ("work", C.c_void_p),
+        ("ckpt", C.c_void_p), ("traj", C.c_void_p), ("lam", C.c_void_p), ("acc", C.c_void_p),
+        ("partial", C.c_void_p), ("sums", C.c_void_p), ("coef", C.c_void_p),
+        ("target", C.c_void_p), ("stats", C.c_void_p), ("loss", C.c_void_p),
+        ("grads", C.c_void_p),
+    ]
+
+
 class SirGraph(C.Structure):
     _fields_ = [
         ("n", C.c_int64),
@@ -101,6 +114,12 @@ SIGNATURES = {
     "abm_sir_states": (_int, [_p, _i64, _p, _p]),
     "abm_wealth_init": (_int, [_p, _i64, _p, _p, _p, _int, _p]),
     "abm_wealth_run_batch": (_int, [_p, _i32, _int, _p]),
+    "abm_permutation_batch_workspace": (_sz, [_i64, _i32]),
+    "abm_permutation_batch": (_int, [_p, _i64, _i32, _i64, _p, _p, _sz, _int, _p]),
+    "abm_deffuant_init": (_int, [_p, _i64, _i32, _p, _int, _p]),
+    "abm_deffuant_step": (_int, [_p, _p, _i64, _i32, _p, _p, _p, _i32, _p]),
+    "abm_deffuant_forward": (_int, [C.POINTER(DeffuantProblem), _p]),
+    "abm_deffuant_backward": (_int, [C.POINTER(DeffuantProblem), _p]),
     "abm_schelling_block_count": (_i64, [_i64]),
     "abm_schelling_init": (_int, [C.POINTER(SchellingGrid), _p, _int, C.POINTER(_i32), _p]),
     "abm_schelling_run": (_int, [C.POINTER(SchellingGrid), _p, _i32, _int, _p, _p, _p, _p]),

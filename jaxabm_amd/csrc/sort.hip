@@ -27,6 +27,9 @@ constexpr int kWaves = kSortThreads / 64;
 __global__ void __launch_bounds__(kSortThreads)
     k_digit_hist(const uint32_t *__restrict__ keys, int64_t n, int shift,
                  uint32_t *__restrict__ hist, int tiles) {
+  // segment blockIdx.y: keys[seg * n ..), hist[seg * 256 * tiles ..)
+  keys += (int64_t)blockIdx.y * n;
+  hist += (int64_t)blockIdx.y * 256 * tiles;
   __shared__ uint32_t cnt[256];
   cnt[threadIdx.x] = 0;
   __syncthreads();
@@ -39,9 +42,10 @@ __global__ void __launch_bounds__(kSortThreads)
   hist[(int64_t)threadIdx.x * tiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// Exclusive scan of `len` counts by a single block.
+// Exclusive scan of `len` counts by a single block per segment (blockIdx.y).
 __global__ void __launch_bounds__(1024)
     k_scan_exclusive(uint32_t *__restrict__ data, int64_t len) {
+  data += (int64_t)blockIdx.y * len;
   __shared__ uint32_t part[1024];
   const int t = threadIdx.x;
   int64_t per = (len + 1023) / 1024;
@@ -70,6 +74,14 @@ __global__ void __launch_bounds__(kSortThreads)
                     const uint32_t *__restrict__ offsets, int tiles,
                     uint32_t *__restrict__ keys_out,
                     int32_t *__restrict__ vals_out) {
+  {
+    const int64_t seg = (int64_t)blockIdx.y * n;
+    keys_in += seg;
+    vals_in += seg;
+    keys_out += seg;
+    vals_out += seg;
+    offsets += (int64_t)blockIdx.y * 256 * tiles;
+  }
   __shared__ uint32_t base[256];
   __shared__ uint32_t wc[kWaves][256];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -108,17 +120,21 @@ __global__ void __launch_bounds__(kSortThreads)
   }
 }
 
+// out[s * n + i] = i for segment s = blockIdx.y
 __global__ void k_iota(int32_t *__restrict__ out, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = (int32_t)i;
+  if (i < n) out[(int64_t)blockIdx.y * n + i] = (int32_t)i;
 }
 
 // Round r's sort keys: k_0 = key; k_{j+1}, sub_j = split(k_j); bits(sub_r).
-__global__ void k_perm_round_bits(const uint32_t *__restrict__ key, int round,
-                                  int64_t n, uint32_t *__restrict__ out,
+// Segment s = blockIdx.y takes its key at key[s * key_stride].
+__global__ void k_perm_round_bits(const uint32_t *__restrict__ key, int64_t key_stride,
+                                  int round, int64_t n, uint32_t *__restrict__ out,
                                   int mode) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  key += (int64_t)blockIdx.y * key_stride;
+  out += (int64_t)blockIdx.y * n;
   Key k{key[0], key[1]}, sub;
   for (int j = 0; j <= round; ++j) {
     Key nk;
@@ -128,36 +144,39 @@ __global__ void k_perm_round_bits(const uint32_t *__restrict__ key, int round,
   out[i] = bits_at(sub, n, i, mode);
 }
 
-size_t sort_workspace_bytes(int64_t n) {
+size_t sort_workspace_bytes(int64_t n, int64_t segs = 1) {
   int64_t tiles = (n + kTile - 1) / kTile;
   // keys x2, vals x1 (the second value buffer is the caller's), hist
   size_t b = 0;
-  b += (size_t)n * 4 * 3;
-  b += (size_t)tiles * 256 * 4;
+  b += (size_t)n * (size_t)segs * 4 * 3;
+  b += (size_t)tiles * 256 * 4 * (size_t)segs;
   return (b + 255) & ~(size_t)255;
 }
 
-// Sort keys[] (destroyed) with vals[] in place, using ws.
+// Sort each of `segs` consecutive segments of n (key, value) pairs, keys[]
+// destroyed, vals[] in place, using ws.
 int radix_sort_pairs(uint32_t *keys, int32_t *vals, int64_t n, void *ws,
-                     hipStream_t s) {
-  if (n <= 1) return kOk;
+                     hipStream_t s, int64_t segs = 1) {
+  if (n <= 1 || segs < 1) return kOk;
   int tiles = (int)((n + kTile - 1) / kTile);
+  const int64_t total = n * segs;
   char *p = (char *)ws;
   uint32_t *k2 = (uint32_t *)p;
-  p += (size_t)n * 4;
+  p += (size_t)total * 4;
   int32_t *v2 = (int32_t *)p;
-  p += (size_t)n * 4;
-  p += (size_t)n * 4;  // reserved
+  p += (size_t)total * 4;
+  p += (size_t)total * 4;  // reserved
   uint32_t *hist = (uint32_t *)p;
   uint32_t *ka = keys, *kb = k2;
   int32_t *va = vals, *vb = v2;
+  const dim3 grid((unsigned)tiles, (unsigned)segs);
   for (int shift = 0; shift < 32; shift += 8) {
-    hipLaunchKernelGGL(k_digit_hist, dim3(tiles), dim3(kSortThreads), 0, s, ka,
-                       n, shift, hist, tiles);
-    hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, hist,
+    hipLaunchKernelGGL(k_digit_hist, grid, dim3(kSortThreads), 0, s, ka, n, shift, hist,
+                       tiles);
+    hipLaunchKernelGGL(k_scan_exclusive, dim3(1, (unsigned)segs), dim3(1024), 0, s, hist,
                        (int64_t)tiles * 256);
-    hipLaunchKernelGGL(k_digit_scatter, dim3(tiles), dim3(kSortThreads), 0, s,
-                       ka, va, n, shift, hist, tiles, kb, vb);
+    hipLaunchKernelGGL(k_digit_scatter, grid, dim3(kSortThreads), 0, s, ka, va, n, shift,
+                       hist, tiles, kb, vb);
     uint32_t *tk = ka;
     ka = kb;
     kb = tk;
@@ -167,6 +186,24 @@ int radix_sort_pairs(uint32_t *keys, int32_t *vals, int64_t n, void *ws,
   }
   // 4 passes: the result is back in (keys, vals)
   ABM_LAUNCH_CHECK("radix_sort_pairs");
+  return kOk;
+}
+
+// permutation(key_s, n) for segs keys (key_s at keys[s * key_stride]), the
+// results in out[s * n ..).
+int permutation_segments(const uint32_t *keys, int64_t key_stride, int64_t segs, int64_t n,
+                         int32_t *out, void *ws, int mode, hipStream_t s) {
+  const dim3 g((unsigned)((n + 255) / 256), (unsigned)segs);
+  hipLaunchKernelGGL(k_iota, g, dim3(256), 0, s, out, n);
+  uint32_t *bits = (uint32_t *)((char *)ws + sort_workspace_bytes(n, segs));
+  const int rounds = (int)__builtin_ceil(3.0 * __builtin_log((double)(n > 1 ? n : 1)) /
+                                         __builtin_log(4294967295.0));
+  for (int r = 0; r < rounds; ++r) {
+    hipLaunchKernelGGL(k_perm_round_bits, g, dim3(256), 0, s, keys, key_stride, r, n, bits,
+                       mode);
+    int rc = radix_sort_pairs(bits, out, n, ws, s, segs);
+    if (rc) return rc;
+  }
   return kOk;
 }
 
@@ -190,20 +227,32 @@ int abm_permutation(const uint32_t *key, int64_t n, int32_t *out, void *ws,
   ABM_REQUIRE(n >= 0 && n < (1ll << 31), "abm_permutation: n out of range");
   ABM_REQUIRE(mode == 0 || mode == 1, "abm_permutation: bad mode");
   if (n == 0) return kOk;
-  ABM_REQUIRE(ws && ws_bytes >= abm_permutation_workspace(n),
+  ABM_REQUIRE(key && out && ws && ws_bytes >= abm_permutation_workspace(n),
               "abm_permutation: workspace too small");
-  hipStream_t s = (hipStream_t)stream;
-  unsigned g = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(k_iota, dim3(g), dim3(256), 0, s, out, n);
-  uint32_t *bits = (uint32_t *)((char *)ws + sort_workspace_bytes(n));
-  int rounds = abm_permutation_rounds(n);
-  for (int r = 0; r < rounds; ++r) {
-    hipLaunchKernelGGL(k_perm_round_bits, dim3(g), dim3(256), 0, s, key, r, n,
-                       bits, mode);
-    int rc = radix_sort_pairs(bits, out, n, ws, s);
-    if (rc) return rc;
-  }
+  int rc = permutation_segments(key, 0, 1, n, out, ws, mode, (hipStream_t)stream);
+  if (rc) return rc;
   ABM_LAUNCH_CHECK("abm_permutation");
+  return kOk;
+}
+
+size_t abm_permutation_batch_workspace(int64_t n, int32_t count) {
+  if (n < 0 || count < 0) return 0;
+  return sort_workspace_bytes(n, count) +
+         (((size_t)n * (size_t)count * 4 + 255) & ~(size_t)255);
+}
+
+int abm_permutation_batch(const uint32_t *keys, int64_t key_stride, int32_t count, int64_t n,
+                          int32_t *out, void *ws, size_t ws_bytes, int mode, void *stream) {
+  ABM_REQUIRE(n >= 0 && n < (1ll << 31) && count >= 0 && count < 65536 &&
+                  n * (int64_t)count < (1ll << 31) && key_stride >= 2,
+              "abm_permutation_batch: sizes out of range");
+  ABM_REQUIRE(mode == 0 || mode == 1, "abm_permutation_batch: bad mode");
+  if (n == 0 || count == 0) return kOk;
+  ABM_REQUIRE(keys && out && ws && ws_bytes >= abm_permutation_batch_workspace(n, count),
+              "abm_permutation_batch: workspace too small");
+  int rc = permutation_segments(keys, key_stride, count, n, out, ws, mode, (hipStream_t)stream);
+  if (rc) return rc;
+  ABM_LAUNCH_CHECK("abm_permutation_batch");
   return kOk;
 }
 

@@ -28,7 +28,7 @@ from ._lib import MODE_ORIGINAL, MODE_PARTITIONABLE, check, lib, ptr, stream_han
 
 __all__ = [
     "MODE_ORIGINAL", "MODE_PARTITIONABLE", "PRNGKey", "key_data", "split",
-    "bits", "uniform", "normal", "bernoulli", "randint", "permutation",
+    "bits", "uniform", "normal", "bernoulli", "randint", "permutation", "permutation_each",
     "split_each", "uniform_each", "normal_each", "bernoulli_each",
     "randint_each", "bits_each", "set_threefry_partitionable", "default_mode",
 ]
@@ -182,6 +182,20 @@ def permutation(key, x, mode=None) -> torch.Tensor:
                                     stream_handle()), "permutation")
     if isinstance(x, torch.Tensor):
         return x[out.long()]
+    return out
+
+
+def permutation_each(keys, n: int, mode=None) -> torch.Tensor:
+    """``vmap(lambda k: permutation(k, n))(keys)`` for keys (count, 2) ->
+    (count, n) int32, computed by one batched segmented sort."""
+    k = _as_key(keys).reshape(-1, 2).contiguous()
+    count = k.shape[0]
+    out = torch.empty((count, n), dtype=torch.int32, device=k.device)
+    if n and count:
+        nbytes = int(lib().abm_permutation_batch_workspace(n, count))
+        ws = torch.empty((nbytes,), dtype=torch.uint8, device=k.device)
+        check(lib().abm_permutation_batch(ptr(k), 2, count, n, ptr(out), ptr(ws), nbytes,
+                                          _mode(mode), stream_handle()), "permutation_each")
     return out
 
 
