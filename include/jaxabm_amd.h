@@ -240,6 +240,65 @@ int abm_deffuant_forward(const abm_deffuant_problem *p, void *stream);
  * recomputes each checkpoint segment, then grads = dL/d(eps, mu). */
 int abm_deffuant_backward(const abm_deffuant_problem *p, void *stream);
 
+/* ---- continuous-batching simulation server (replaces the serial runners
+ *      jaxabm/analysis.py:97-165, jaxabm/utils.py:128-176,
+ *      jaxabm/agentpy.py:1116-1152) ---- */
+#define ABM_SERVE_WEALTH 0
+#define ABM_SERVE_SIR 1
+#define ABM_SERVE_QUEUED 0
+#define ABM_SERVE_PREPARED 1
+#define ABM_SERVE_UPLOADING 2
+#define ABM_SERVE_ACTIVE 3
+#define ABM_SERVE_DONE 4
+#define ABM_SERVE_FAILED 5
+typedef struct abm_serve_config {
+  uint64_t hbm_budget;        /* device bytes for instance state (one arena) */
+  int64_t max_agents;         /* largest request accepted */
+  int32_t max_active;         /* instances advanced together (1: one at a time) */
+  int32_t prefetch;           /* extra admitted instances uploading meanwhile */
+  int32_t max_inflight;       /* ticks queued on the device (1..3) */
+  int32_t steps_per_tick;     /* K */
+  int32_t prep_threads;       /* host preparation workers */
+  int32_t mode;               /* RNG stream mode */
+  int64_t sir_chunk;          /* stepwise SIR: cost (4 + degree per agent) per range */
+  int64_t wealth_chunk;       /* stepwise wealth: agents per range */
+  int64_t sir_resident_max;   /* n <= this (and <= 65536): resident SIR instance */
+  int64_t wealth_resident_max;/* n <= this (and <= 65536): resident wealth instance */
+  int32_t strict_fifo;        /* 1: no admission past a request that does not fit */
+  int32_t reserved;
+} abm_serve_config;
+typedef struct abm_request {
+  int32_t kind;               /* ABM_SERVE_WEALTH or ABM_SERVE_SIR */
+  int32_t steps;
+  int64_t n;                  /* agents */
+  int64_t seed;               /* model seed: PRNGKey(seed) */
+  int64_t graph_seed;         /* SIR: Barabasi-Albert graph seed */
+  int32_t m;                  /* SIR: edges per new node */
+  int32_t n_infected;         /* SIR: initial infections */
+  float beta, gamma;          /* SIR: infection / recovery probabilities */
+} abm_request;
+typedef struct abm_ticket_info {
+  int32_t status, kind, steps, resident;
+  int64_t n, nnz, device_bytes, result_bytes;
+  double t_submit, t_ready, t_admit, t_start, t_done; /* abm_serve_clock() seconds */
+} abm_ticket_info;
+typedef struct abm_serve_stats {
+  int64_t ticks, completed, agent_steps, peak_active, peak_device_bytes, outstanding;
+} abm_serve_stats;
+void abm_serve_default_config(abm_serve_config *cfg);
+double abm_serve_clock(void);
+int abm_serve_create(const abm_serve_config *cfg, void **server);
+int abm_serve_submit(void *server, const abm_request *req, int64_t *ticket);
+int abm_serve_poll(void *server, int64_t ticket, int32_t *status);
+int abm_serve_wait(void *server, int64_t ticket, double timeout_s, int32_t *status);
+/* Result records: SIR int32 counts[steps][3] (S, I, R); wealth float32
+ * mean[steps] then gini[steps].  out may be NULL to read only the info. */
+int abm_serve_collect(void *server, int64_t ticket, void *out, size_t out_bytes,
+                      abm_ticket_info *info);
+int abm_serve_drain(void *server, double timeout_s);
+int abm_serve_get_stats(void *server, abm_serve_stats *st);
+int abm_serve_destroy(void *server);
+
 /* ---- step-loop capture (replaces the Python loop of JaxModel.run,
  *      jaxabm/model.py:240-241) ---- */
 typedef struct abm_graph abm_graph;

@@ -57,6 +57,38 @@ class DeffuantProblem(C.Structure):
     ]
 
 
+class ServeConfig(C.Structure):
+    _fields_ = [
+        ("hbm_budget", C.c_uint64), ("max_agents", C.c_int64), ("max_active", C.c_int32),
+        ("prefetch", C.c_int32), ("max_inflight", C.c_int32), ("steps_per_tick", C.c_int32),
+        ("prep_threads", C.c_int32), ("mode", C.c_int32), ("sir_chunk", C.c_int64),
+        ("wealth_chunk", C.c_int64), ("sir_resident_max", C.c_int64),
+        ("wealth_resident_max", C.c_int64), ("strict_fifo", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class ServeRequest(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("steps", C.c_int32), ("n", C.c_int64), ("seed", C.c_int64),
+        ("graph_seed", C.c_int64), ("m", C.c_int32), ("n_infected", C.c_int32),
+        ("beta", C.c_float), ("gamma", C.c_float),
+    ]
+
+
+class TicketInfo(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("kind", C.c_int32), ("steps", C.c_int32), ("resident", C.c_int32),
+        ("n", C.c_int64), ("nnz", C.c_int64), ("device_bytes", C.c_int64),
+        ("result_bytes", C.c_int64), ("t_submit", C.c_double), ("t_ready", C.c_double),
+        ("t_admit", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double),
+    ]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [(f, C.c_int64) for f in ("ticks", "completed", "agent_steps", "peak_active",
+                                         "peak_device_bytes", "outstanding")]
+
+
 class SirGraph(C.Structure):
     _fields_ = [
         ("n", C.c_int64),
@@ -120,6 +152,16 @@ SIGNATURES = {
     "abm_deffuant_step": (_int, [_p, _p, _i64, _i32, _p, _p, _p, _i32, _p]),
     "abm_deffuant_forward": (_int, [C.POINTER(DeffuantProblem), _p]),
     "abm_deffuant_backward": (_int, [C.POINTER(DeffuantProblem), _p]),
+    "abm_serve_default_config": (None, [C.POINTER(ServeConfig)]),
+    "abm_serve_clock": (C.c_double, []),
+    "abm_serve_create": (_int, [C.POINTER(ServeConfig), C.POINTER(C.c_void_p)]),
+    "abm_serve_submit": (_int, [_p, C.POINTER(ServeRequest), C.POINTER(C.c_int64)]),
+    "abm_serve_poll": (_int, [_p, _i64, C.POINTER(C.c_int32)]),
+    "abm_serve_wait": (_int, [_p, _i64, C.c_double, C.POINTER(C.c_int32)]),
+    "abm_serve_collect": (_int, [_p, _i64, _p, _sz, C.POINTER(TicketInfo)]),
+    "abm_serve_drain": (_int, [_p, C.c_double]),
+    "abm_serve_get_stats": (_int, [_p, C.POINTER(ServeStats)]),
+    "abm_serve_destroy": (_int, [_p]),
     "abm_schelling_block_count": (_i64, [_i64]),
     "abm_schelling_init": (_int, [C.POINTER(SchellingGrid), _p, _int, C.POINTER(_i32), _p]),
     "abm_schelling_run": (_int, [C.POINTER(SchellingGrid), _p, _i32, _int, _p, _p, _p, _p]),
