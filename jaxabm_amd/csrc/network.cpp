@@ -50,17 +50,25 @@ int64_t abm_ba_edges(int64_t n, int32_t m, const uint32_t key[2], int32_t *src,
   rep.insert(rep.end(), (size_t)m, 0);
   for (int32_t i = 1; i <= m; ++i) rep.push_back(i);
   std::vector<int32_t> pick((size_t)m);
+  // draws are a pure function of (s, attempt): compute them four at a time
+  // (independent chains), consume them in attempt order
+  constexpr int kBatch = 4;
+  uint32_t draw[kBatch];
   for (int64_t s = m + 1; s < n; ++s) {
     int32_t have = 0;
-    uint32_t a = 0;
     const uint64_t len = rep.size();
-    while (have < m) {
-      uint32_t y0, y1;
-      threefry(key[0], key[1], (uint32_t)s, a++, y0, y1);
-      const int32_t t = rep[(size_t)(((uint64_t)y0 * len) >> 32)];
-      bool seen = false;
-      for (int32_t q = 0; q < have; ++q) seen |= (pick[q] == t);
-      if (!seen) pick[have++] = t;
+    for (uint32_t a = 0; have < m; a += kBatch) {
+      for (int j = 0; j < kBatch; ++j) {
+        uint32_t y0, y1;
+        threefry(key[0], key[1], (uint32_t)s, a + (uint32_t)j, y0, y1);
+        draw[j] = y0;
+      }
+      for (int j = 0; j < kBatch && have < m; ++j) {
+        const int32_t t = rep[(size_t)(((uint64_t)draw[j] * len) >> 32)];
+        bool seen = false;
+        for (int32_t q = 0; q < have; ++q) seen |= (pick[q] == t);
+        if (!seen) pick[have++] = t;
+      }
     }
     for (int32_t q = 0; q < m; ++q, ++e) {
       src[e] = (int32_t)s;
@@ -72,44 +80,49 @@ int64_t abm_ba_edges(int64_t n, int32_t m, const uint32_t key[2], int32_t *src,
   return e;
 }
 
-// Undirected CSR with ascending neighbour ids per row: both directions of
-// every edge, ordered by two stable counting sorts (by neighbour, then by
-// row), O(n + E).
+// Undirected CSR with ascending neighbour ids per row (duplicates kept):
+// degree count, scatter of both directions of every edge, then a sort of each
+// row (rows are short; insertion sort below 32 entries).
 int abm_csr_undirected(int64_t n, int64_t n_edges, const int32_t *src,
                        const int32_t *dst, int32_t *row_ptr, int32_t *col) {
   ABM_REQUIRE(n >= 0 && n_edges >= 0, "abm_csr_undirected: bad sizes");
   ABM_REQUIRE(2 * n_edges < (1ll << 31), "abm_csr_undirected: too many edges");
   ABM_REQUIRE(row_ptr && (n_edges == 0 || (src && dst && col)),
               "abm_csr_undirected: null pointer");
-  const int64_t nnz = 2 * n_edges;
   for (int64_t e = 0; e < n_edges; ++e) {
     if (src[e] < 0 || src[e] >= n || dst[e] < 0 || dst[e] >= n) {
       set_error("abm_csr_undirected: edge endpoint out of range");
       return kErrInvalid;
     }
   }
-  std::vector<int32_t> ra((size_t)nnz), ca((size_t)nnz);
-  // pass 1: counting sort of directed entries (r, c) by c
-  std::vector<int64_t> cnt((size_t)n + 1, 0);
+  std::fill(row_ptr, row_ptr + n + 1, 0);
   for (int64_t e = 0; e < n_edges; ++e) {
-    cnt[(size_t)dst[e] + 1]++;
-    cnt[(size_t)src[e] + 1]++;
+    row_ptr[src[e] + 1]++;
+    row_ptr[dst[e] + 1]++;
   }
-  for (int64_t v = 0; v < n; ++v) cnt[(size_t)v + 1] += cnt[(size_t)v];
+  for (int64_t v = 0; v < n; ++v) row_ptr[v + 1] += row_ptr[v];
+  std::vector<int32_t> pos(row_ptr, row_ptr + n);
   for (int64_t e = 0; e < n_edges; ++e) {
-    int64_t p = cnt[(size_t)dst[e]]++;
-    ra[(size_t)p] = src[e];
-    ca[(size_t)p] = dst[e];
-    p = cnt[(size_t)src[e]]++;
-    ra[(size_t)p] = dst[e];
-    ca[(size_t)p] = src[e];
+    col[pos[(size_t)src[e]]++] = dst[e];
+    col[pos[(size_t)dst[e]]++] = src[e];
   }
-  // pass 2: stable counting sort by r
-  std::fill(cnt.begin(), cnt.end(), 0);
-  for (int64_t i = 0; i < nnz; ++i) cnt[(size_t)ra[(size_t)i] + 1]++;
-  for (int64_t v = 0; v < n; ++v) cnt[(size_t)v + 1] += cnt[(size_t)v];
-  for (int64_t v = 0; v <= n; ++v) row_ptr[v] = (int32_t)cnt[(size_t)v];
-  for (int64_t i = 0; i < nnz; ++i) col[cnt[(size_t)ra[(size_t)i]]++] = ca[(size_t)i];
+  for (int64_t v = 0; v < n; ++v) {
+    int32_t *a = col + row_ptr[v];
+    const int32_t len = row_ptr[v + 1] - row_ptr[v];
+    if (len > 32) {
+      std::sort(a, a + len);
+      continue;
+    }
+    for (int32_t i = 1; i < len; ++i) {
+      const int32_t x = a[i];
+      int32_t j = i - 1;
+      while (j >= 0 && a[j] > x) {
+        a[j + 1] = a[j];
+        --j;
+      }
+      a[j + 1] = x;
+    }
+  }
   return kOk;
 }
 

@@ -798,19 +798,31 @@ void Server::prep_loop() {
   }
 }
 
-// permutation(key, n)[:n0] on the host: stable LSD radix sorts of (bits, x)
-// for ceil(3 ln n / ln(2^32 - 1)) rounds.
+// permutation(key, n)[:n0] on the host: ceil(3 ln n / ln(2^32 - 1)) rounds
+// of a stable sort of x by fresh 32-bit keys.  All rounds but the last are
+// stable LSD radix sorts; the last only needs its n0 smallest (key, position)
+// pairs, so it is a selection followed by a sort of those n0.
 std::vector<int32_t> host_permutation_prefix(Key key, int64_t n, int32_t n0, int mode) {
   std::vector<int32_t> x(n), x2(n);
   std::vector<uint32_t> k(n), k2(n);
   for (int64_t i = 0; i < n; ++i) x[i] = (int32_t)i;
   const int rounds = (int)__builtin_ceil(3.0 * __builtin_log((double)(n > 1 ? n : 1)) /
                                          __builtin_log(4294967295.0));
+  const int64_t take = std::min<int64_t>(n0, n);
   for (int r = 0; r < rounds; ++r) {
     Key nk, sub;
     split2(key, nk, sub, mode);
     key = nk;
     for (int64_t i = 0; i < n; ++i) k[i] = bits_at(sub, n, i, mode);
+    if (r + 1 == rounds) {
+      std::vector<uint64_t> kp((size_t)n);
+      for (int64_t i = 0; i < n; ++i) kp[i] = ((uint64_t)k[i] << 32) | (uint64_t)i;
+      if (take < n) std::nth_element(kp.begin(), kp.begin() + take, kp.end());
+      std::sort(kp.begin(), kp.begin() + take);
+      std::vector<int32_t> out((size_t)take);
+      for (int64_t i = 0; i < take; ++i) out[i] = x[(size_t)(kp[i] & 0xFFFFFFFFu)];
+      return out;
+    }
     for (int shift = 0; shift < 32; shift += 8) {
       size_t cnt[257] = {0};
       for (int64_t i = 0; i < n; ++i) ++cnt[((k[i] >> shift) & 255u) + 1];
@@ -824,7 +836,7 @@ std::vector<int32_t> host_permutation_prefix(Key key, int64_t n, int32_t n0, int
       x.swap(x2);
     }
   }
-  x.resize((size_t)std::min<int64_t>(n0, n));
+  x.resize((size_t)take);
   return x;
 }
 
@@ -1323,7 +1335,7 @@ void abm_serve_default_config(abm_serve_config *cfg) {
   cfg->prefetch = 2;
   cfg->max_inflight = 2;
   cfg->steps_per_tick = 16;
-  cfg->prep_threads = 4;
+  cfg->prep_threads = 12;
   cfg->mode = kModePartitionable;
   cfg->sir_chunk = 16384;
   cfg->wealth_chunk = 2048;

@@ -225,6 +225,8 @@ class ReplayReport:
     hbm_gbps: float
     peak_active: int
     peak_device_bytes: int
+    prep_s: float = 0.0       # last request prepared - trace start
+    queue_ms: float = 0.0     # mean prepared -> started on the device
     tickets: list = field(default_factory=list, repr=False)
     latencies_ms: list = field(default_factory=list, repr=False)
 
@@ -251,13 +253,15 @@ def replay(server: SimulationServer, trace: list[SimRequest], time_scale: float 
             time.sleep(min(dt, 0.002) if dt > 0.0005 else 0)
         tickets.append(server.submit(r))
     server.drain(timeout)
-    lat, t_end = [], t0
+    lat, t_end, t_ready, queue = [], t0, t0, []
     for tk, ta in zip(tickets, arrivals):
         info = server.info(tk)
         if info.status != DONE:
             raise AbmError(f"request {tk} did not complete ({STATUS[info.status]})")
         lat.append((info.t_done - ta) * 1e3)
         t_end = max(t_end, info.t_done)
+        t_ready = max(t_ready, info.t_ready)
+        queue.append((info.t_start - info.t_ready) * 1e3)
     span = max(t_end - t0, 1e-9)
     st = server.stats()
     agent_steps = sum(r.n * r.steps for r in trace)
@@ -268,16 +272,17 @@ def replay(server: SimulationServer, trace: list[SimRequest], time_scale: float 
         p50_ms=float(np.percentile(lat_a, 50)), p99_ms=float(np.percentile(lat_a, 99)),
         mean_ms=float(lat_a.mean()), agent_steps=agent_steps, algorithmic_bytes=alg,
         hbm_gbps=alg / span / 1e9, peak_active=st["peak_active"],
-        peak_device_bytes=st["peak_device_bytes"], tickets=tickets, latencies_ms=lat)
+        peak_device_bytes=st["peak_device_bytes"], prep_s=t_ready - t0,
+        queue_ms=float(np.mean(queue)), tickets=tickets, latencies_ms=lat)
 
 
 # ---------------------------------------------------------------- autotuner
 SEARCH_SPACE = {
-    "steps_per_tick": [4, 8, 16, 32, 64],
-    "sir_resident_max": [0, 2048, 8192, 32768, 65536],
-    "wealth_resident_max": [0, 1024, 4096, 16384],
-    "sir_chunk": [4096, 16384, 65536],
-    "wealth_chunk": [512, 2048, 8192],
+    "steps_per_tick": [2, 4, 8, 16, 32, 64],
+    "sir_resident_max": [0, 1024, 2048, 4096, 8192, 32768],
+    "wealth_resident_max": [0, 512, 1024, 2048, 4096, 16384],
+    "sir_chunk": [1024, 2048, 4096, 8192, 16384],
+    "wealth_chunk": [128, 256, 512, 1024, 2048, 4096],
 }
 
 
@@ -286,7 +291,7 @@ def _measure(cfg: dict, trace: list[SimRequest]) -> float:
         return replay(s, trace, time_scale=0.0).sims_per_s
 
 
-def autotune(profile: str = "batched", n_requests: int = 96, seed: int = 12345,
+def autotune(profile: str = "batched", n_requests: int = 192, seed: int = 12345,
              rounds: int = 2, write: bool = True, log=print) -> dict:
     """Coordinate search over ``SEARCH_SPACE`` maximising saturated sims/s on a
     sample trace (its own seed, not the benchmark's).  ``profile`` "solo"
@@ -339,3 +344,40 @@ def submit_concurrently(server: SimulationServer, traces: list[list[SimRequest]]
     for th in threads:
         th.join()
     return outs
+
+
+def _records_equal(a: dict, b: dict) -> bool:
+    return a.keys() == b.keys() and all(np.array_equal(a[k], b[k]) for k in a)
+
+
+def benchmark(n_requests: int = 400, rate: float = 200.0, seed: int = 0,
+              solo: dict | None = None, batched: dict | None = None, log=None) -> dict:
+    """The serving workload on one trace: one request at a time (max_active =
+    1, its own tuned configuration) against continuous batching.
+
+    For each server: a saturated replay (every request submitted at once:
+    sims/s is the capacity) and a replay at the trace's arrival times
+    (``rate`` per second: latency percentiles arrival -> result).  Every
+    request's records are compared between the two servers."""
+    trace = make_trace(n_requests, rate=rate, seed=seed)
+    out = {"requests": n_requests, "rate": rate, "trace_seed": seed,
+           "trace_checksum": trace_checksum(trace)}
+    results = {}
+    for name, cfg in (("solo", solo or tuned_config("solo")),
+                      ("batched", batched or tuned_config("batched"))):
+        with SimulationServer(cfg) as s:
+            sat = replay(s, trace, time_scale=0.0)
+            results[name] = [s.result(t) for t in sat.tickets]
+        with SimulationServer(cfg) as s:
+            live = replay(s, trace, time_scale=1.0)
+        out[name] = {"config": {k: cfg[k] for k in SEARCH_SPACE if k in cfg} |
+                     {"max_active": cfg["max_active"]},
+                     "saturated": sat.summary(), "at_rate": live.summary()}
+        if log:
+            log(f"[serving] {name}: saturated {sat.sims_per_s:.1f} sims/s, at {rate:g}/s "
+                f"p50 {live.p50_ms:.2f} ms p99 {live.p99_ms:.2f} ms")
+    out["identical"] = all(_records_equal(a, b) for a, b in zip(results["solo"],
+                                                                 results["batched"]))
+    out["speedup"] = (out["batched"]["saturated"]["sims_per_s"] /
+                      out["solo"]["saturated"]["sims_per_s"])
+    return out
