@@ -17,7 +17,13 @@ Also reported:
     number of step kernels, against MI355X's 8 TB/s;
   * an upper bound of the data-dependent bytes the skipping kernel touches;
   * cpu_baseline: the C oracle (a restatement of the reference semantics, one
-    thread) running the same full simulation on the host.
+    thread) running the same full simulation on the host;
+  * serving: the batched-serving workload (seeded trace of wealth-exchange and
+    SIR requests, agent counts log-uniform in [1k, 256k], 100-365 steps,
+    Poisson arrivals at 200/s) through the continuous-batching server against
+    the same server running one request at a time: saturated sims/s of each,
+    arrival-to-result p50/p99 at 200 requests/s, and whether every request's
+    records are identical between the two.
 """
 from __future__ import annotations
 
@@ -80,6 +86,31 @@ def cpu_baseline(row_ptr, col, runs: int) -> dict:
     }
 
 
+def serving_summary(n_requests: int) -> dict:
+    from jaxabm_amd import serving
+
+    r = serving.benchmark(n_requests=n_requests, rate=200.0, seed=0)
+    b, s = r["batched"], r["solo"]
+    return {
+        "metric": "sims/s, batched serving vs one request at a time (same GPU, same kernels)",
+        "sims_per_s": b["saturated"]["sims_per_s"],
+        "solo_sims_per_s": s["saturated"]["sims_per_s"],
+        "speedup": r["speedup"],
+        "identical_results": r["identical"],
+        "p50_ms": b["at_rate"]["p50_ms"],
+        "p99_ms": b["at_rate"]["p99_ms"],
+        "solo_p50_ms": s["at_rate"]["p50_ms"],
+        "solo_p99_ms": s["at_rate"]["p99_ms"],
+        "arrival_rate": r["rate"],
+        "requests": r["requests"],
+        "trace_checksum": r["trace_checksum"],
+        "algorithmic_GBs": b["saturated"]["hbm_gbps"],
+        "peak_active": b["saturated"]["peak_active"],
+        "batched_config": b["config"],
+        "solo_config": s["config"],
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +118,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-runs", type=int, default=2, help="oracle runs for cpu_baseline (0 = skip)")
     ap.add_argument("--no-graph", action="store_true", help="launch kernels directly instead of a hipGraph")
+    ap.add_argument("--serving-requests", type=int, default=400,
+                    help="requests in the serving trace (0 = skip the serving measurement)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -243,6 +276,8 @@ def main() -> int:
     }
     if args.cpu_runs > 0:
         out["cpu_baseline"] = cpu_baseline(row_ptr, col, args.cpu_runs)
+    if args.serving_requests > 0:
+        out["serving"] = serving_summary(args.serving_requests)
     if graph is not None:
         lib().abm_graph_destroy(graph)
     print(json.dumps(out))
