@@ -103,6 +103,20 @@ int64_t abm_node_chunks(int64_t n, const int32_t *row_ptr, int32_t align, int64_
                         int32_t heavy_deg, int64_t heavy_cost, int64_t target,
                         int32_t max_nodes, int32_t *chunk_ptr);
 
+/* ---- generic CSR gather / scatter (neighbour aggregation for Network
+ *      models; replaces the masked queries of jaxabm/agentpy.py:584-615) ---- */
+/* out[v] = sum over the CSR row of v of x[col[e]] (float64 accumulation). */
+int abm_csr_gather_sum_f32(int64_t n, const int32_t *row_ptr, const int32_t *col, const float *x,
+                           float *out, void *stream);
+/* out[v] = number of entries e in row v with mask[col[e]] != 0. */
+int abm_csr_count_u8(int64_t n, const int32_t *row_ptr, const int32_t *col, const uint8_t *mask,
+                     int32_t *out, void *stream);
+/* out[idx[i]] += val[i] for i < m (atomic; idx within out). */
+int abm_scatter_add_f32(int64_t m, const int32_t *idx, const float *val, float *out,
+                        void *stream);
+int abm_scatter_add_i32(int64_t m, const int32_t *idx, const int32_t *val, int32_t *out,
+                        void *stream);
+
 /* ---- network SIR program: the env-phase callback of JaxModel.step
  *      (jaxabm/model.py:182-190) + metrics_fn counts (model.py:194-200) ----
  * Device state per instance: state uint8[n] (low 2 bits S=0/I=1/R=2, bit 2 =
@@ -225,6 +239,7 @@ typedef struct abm_deffuant_problem {
   double *stats;           /* device [B][34] */
   double *loss;            /* device [B] */
   double *grads;           /* device [2][B]: dL/deps, dL/dmu (backward) */
+  const double *weights;   /* device [34] per-statistic loss weights, or NULL (all 1) */
 } abm_deffuant_problem;
 /* x[i][b] = uniform(split(init_key, n)[i]) for every instance b. */
 int abm_deffuant_init(const uint32_t *init_key, int64_t n, int32_t batch, float *x, int mode,

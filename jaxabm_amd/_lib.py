@@ -53,7 +53,7 @@ class DeffuantProblem(C.Structure):
         ("ckpt", C.c_void_p), ("traj", C.c_void_p), ("lam", C.c_void_p), ("acc", C.c_void_p),
         ("partial", C.c_void_p), ("sums", C.c_void_p), ("coef", C.c_void_p),
         ("target", C.c_void_p), ("stats", C.c_void_p), ("loss", C.c_void_p),
-        ("grads", C.c_void_p),
+        ("grads", C.c_void_p), ("weights", C.c_void_p),
     ]
 
 
@@ -162,6 +162,10 @@ SIGNATURES = {
     "abm_serve_drain": (_int, [_p, C.c_double]),
     "abm_serve_get_stats": (_int, [_p, C.POINTER(ServeStats)]),
     "abm_serve_destroy": (_int, [_p]),
+    "abm_csr_gather_sum_f32": (_int, [_i64, _p, _p, _p, _p, _p]),
+    "abm_csr_count_u8": (_int, [_i64, _p, _p, _p, _p, _p]),
+    "abm_scatter_add_f32": (_int, [_i64, _p, _p, _p, _p]),
+    "abm_scatter_add_i32": (_int, [_i64, _p, _p, _p, _p]),
     "abm_schelling_block_count": (_i64, [_i64]),
     "abm_schelling_init": (_int, [C.POINTER(SchellingGrid), _p, _int, C.POINTER(_i32), _p]),
     "abm_schelling_run": (_int, [C.POINTER(SchellingGrid), _p, _i32, _int, _p, _p, _p, _p]),

@@ -262,24 +262,28 @@ __global__ void __launch_bounds__(256)
 
 // stats[b][34], loss[b] and the coefficients of dL/dx (coef[34][B]:
 // dL/dmean / n, dL/dvar * 2 / n, dL/dhist_k / n) from the column sums.
+// weights (optional, [34]): L = sum_j w_j (stats_j - target_j)^2.
 __global__ void k_deffuant_loss(const double *__restrict__ sums, int64_t n, int32_t B,
-                                const double *__restrict__ target, double *__restrict__ stats,
+                                const double *__restrict__ target,
+                                const double *__restrict__ weights, double *__restrict__ stats,
                                 double *__restrict__ loss, double *__restrict__ coef) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
   const double dn = (double)n;
   const double m = sums[b] / dn;
   const double v = sums[B + b] / dn - (m - 0.5) * (m - 0.5);
-  double L = (m - target[0]) * (m - target[0]) + (v - target[1]) * (v - target[1]);
+  const double w0 = weights ? weights[0] : 1.0, w1 = weights ? weights[1] : 1.0;
+  double L = w0 * (m - target[0]) * (m - target[0]) + w1 * (v - target[1]) * (v - target[1]);
   stats[(int64_t)b * kStatLen] = m;
   stats[(int64_t)b * kStatLen + 1] = v;
-  coef[b] = 2.0 * (m - target[0]) / dn;
-  coef[B + b] = 2.0 * (v - target[1]) * 2.0 / dn;
+  coef[b] = w0 * 2.0 * (m - target[0]) / dn;
+  coef[B + b] = w1 * 2.0 * (v - target[1]) * 2.0 / dn;
   for (int k = 0; k < kHistBins; ++k) {
     const double hk = sums[(int64_t)(2 + k) * B + b] / dn;
+    const double wk = weights ? weights[2 + k] : 1.0;
     stats[(int64_t)b * kStatLen + 2 + k] = hk;
-    L += (hk - target[2 + k]) * (hk - target[2 + k]);
-    coef[(int64_t)(2 + k) * B + b] = 2.0 * (hk - target[2 + k]) / dn;
+    L += wk * (hk - target[2 + k]) * (hk - target[2 + k]);
+    coef[(int64_t)(2 + k) * B + b] = wk * 2.0 * (hk - target[2 + k]) / dn;
   }
   loss[b] = L;
 }
@@ -364,8 +368,8 @@ void launch_loss(const abm_deffuant_problem *p, const float *x, hipStream_t s) {
   hipLaunchKernelGGL(k_sum_rows, dim3(nblocks(p->batch, 64), (unsigned)kStatLen), dim3(256), 0,
                      s, (const double *)p->partial, chunks, kStatLen, p->batch, p->sums);
   hipLaunchKernelGGL(k_deffuant_loss, dim3(nblocks(p->batch, 256)), dim3(256), 0, s,
-                     (const double *)p->sums, p->n, p->batch, p->target, p->stats, p->loss,
-                     p->coef);
+                     (const double *)p->sums, p->n, p->batch, p->target, p->weights, p->stats,
+                     p->loss, p->coef);
 }
 
 int check_problem(const abm_deffuant_problem *p, const char *who, bool backward) {

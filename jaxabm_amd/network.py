@@ -14,7 +14,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from ._lib import check, lib
+from ._lib import check, lib, ptr, stream_handle
 
 
 def ba_edges(n: int, m: int, graph_seed: int) -> tuple[np.ndarray, np.ndarray]:
@@ -84,3 +84,43 @@ class DeviceCSR:
 def ba_csr(n: int, m: int, graph_seed: int) -> tuple[np.ndarray, np.ndarray]:
     src, dst = ba_edges(n, m, graph_seed)
     return csr_undirected(n, src, dst)
+
+
+def csr_gather_sum(g: DeviceCSR, x: torch.Tensor) -> torch.Tensor:
+    """out[v] = sum of x[u] over the neighbours u of v (float32 result,
+    float64 accumulation)."""
+    x = x.to(device=g.row_ptr.device, dtype=torch.float32).contiguous()
+    if x.numel() < g.n:
+        raise ValueError("x needs one value per node")
+    out = torch.empty(g.n, dtype=torch.float32, device=g.row_ptr.device)
+    check(lib().abm_csr_gather_sum_f32(g.n, ptr(g.row_ptr), ptr(g.col), ptr(x), ptr(out),
+                                       stream_handle()), "csr_gather_sum")
+    return out
+
+
+def csr_count(g: DeviceCSR, mask: torch.Tensor) -> torch.Tensor:
+    """out[v] = number of neighbours u of v with mask[u] (int32, exact)."""
+    m = mask.to(device=g.row_ptr.device, dtype=torch.uint8).contiguous()
+    if m.numel() < g.n:
+        raise ValueError("mask needs one value per node")
+    out = torch.empty(g.n, dtype=torch.int32, device=g.row_ptr.device)
+    check(lib().abm_csr_count_u8(g.n, ptr(g.row_ptr), ptr(g.col), ptr(m), ptr(out),
+                                 stream_handle()), "csr_count")
+    return out
+
+
+def scatter_add(out: torch.Tensor, idx: torch.Tensor, val: torch.Tensor) -> torch.Tensor:
+    """out[idx[i]] += val[i] in place (float32 or int32; idx checked on the
+    host against out's length)."""
+    idx = idx.to(device=out.device, dtype=torch.int32).contiguous()
+    val = val.to(device=out.device, dtype=out.dtype).contiguous()
+    if idx.numel() != val.numel():
+        raise ValueError("idx and val differ in length")
+    if idx.numel() and (int(idx.min()) < 0 or int(idx.max()) >= out.numel()):
+        raise IndexError("scatter_add index out of range")
+    fn = {torch.float32: lib().abm_scatter_add_f32,
+          torch.int32: lib().abm_scatter_add_i32}.get(out.dtype)
+    if fn is None:
+        raise TypeError("scatter_add: float32 or int32 output")
+    check(fn(idx.numel(), ptr(idx), ptr(val), ptr(out), stream_handle()), "scatter_add")
+    return out

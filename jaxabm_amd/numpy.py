@@ -1,0 +1,397 @@
+"""``jax.numpy``-shaped array functions over torch, for agent and model code.
+
+Models written against the reference use ``jax.numpy`` inside ``Agent.step``,
+``AgentType.update`` and the env/metrics callbacks (jaxabm/agent.py:19-51,
+jaxabm/model.py:29-30).  This namespace gives the same names on GPU tensors,
+with JAX's default dtypes (float32, int32: the reference runs with
+jax_enable_x64 off), so such code runs with ``import jaxabm_amd.numpy as jnp``.
+Every function is a torch op, so per-agent functions run under
+``torch.func.vmap`` exactly as under ``jax.vmap``.
+
+Functional updates: ``x.at[i].set(v)`` has no torch form; use
+:func:`at_set` / :func:`at_add` (out-of-place) instead.
+"""
+from __future__ import annotations
+
+import builtins
+import math
+
+import numpy as _np
+import torch
+
+float32 = torch.float32
+float64 = torch.float64
+int32 = torch.int32
+int64 = torch.int64
+uint8 = torch.uint8
+uint32 = torch.int32  # 32-bit storage; random bits come as int32 bit patterns
+bool_ = torch.bool
+pi = math.pi
+inf = math.inf
+nan = math.nan
+e = math.e
+ndarray = torch.Tensor
+
+
+def _dev():
+    from ._lib import require_gpu
+
+    require_gpu()
+    return torch.device("cuda")
+
+
+def _default_dtype(value):
+    if isinstance(value, bool):
+        return torch.bool
+    if isinstance(value, int):
+        return torch.int32
+    if isinstance(value, float):
+        return torch.float32
+    return None
+
+
+def asarray(a, dtype=None) -> torch.Tensor:
+    """Array from a tensor, numpy array, Python scalar or (nested) sequence."""
+    if isinstance(a, torch.Tensor):
+        return a if dtype is None else a.to(dtype)
+    if isinstance(a, _np.ndarray):
+        t = torch.from_numpy(_np.ascontiguousarray(a))
+        if dtype is None and t.dtype == torch.float64:
+            dtype = torch.float32
+        if dtype is None and t.dtype == torch.int64:
+            dtype = torch.int32
+        return t.to(device=_dev(), dtype=dtype or t.dtype)
+    if isinstance(a, (list, tuple)) and a and any(isinstance(x, torch.Tensor) for x in a):
+        return torch.stack([asarray(x, dtype) for x in a])
+    if dtype is None:
+        dtype = _default_dtype(a)
+        if dtype is None:
+            probe = _np.asarray(a)
+            dtype = {"f": torch.float32, "i": torch.int32, "u": torch.int32,
+                     "b": torch.bool}.get(probe.dtype.kind, torch.float32)
+    return torch.as_tensor(a, dtype=dtype, device=_dev())
+
+
+array = asarray
+
+
+def _t(x):
+    return x if isinstance(x, torch.Tensor) else asarray(x)
+
+
+def zeros(shape, dtype=float32):
+    return torch.zeros(shape, dtype=dtype, device=_dev())
+
+
+def ones(shape, dtype=float32):
+    return torch.ones(shape, dtype=dtype, device=_dev())
+
+
+def full(shape, fill_value, dtype=None):
+    return torch.full(shape if isinstance(shape, tuple) else (shape,) if isinstance(shape, int)
+                      else tuple(shape), fill_value, dtype=dtype or _default_dtype(fill_value),
+                      device=_dev())
+
+
+def zeros_like(x, dtype=None):
+    return torch.zeros_like(_t(x), dtype=dtype)
+
+
+def ones_like(x, dtype=None):
+    return torch.ones_like(_t(x), dtype=dtype)
+
+
+def full_like(x, fill_value, dtype=None):
+    return torch.full_like(_t(x), fill_value, dtype=dtype)
+
+
+def arange(start, stop=None, step=1, dtype=None):
+    if stop is None:
+        start, stop = 0, start
+    if dtype is None:
+        dtype = torch.int32 if all(isinstance(v, int) for v in (start, stop, step)) else torch.float32
+    return torch.arange(start, stop, step, dtype=dtype, device=_dev())
+
+
+def linspace(start, stop, num=50, endpoint=True, dtype=float32):
+    if endpoint:
+        return torch.linspace(start, stop, num, dtype=dtype, device=_dev())
+    return torch.linspace(start, stop, num + 1, dtype=dtype, device=_dev())[:-1]
+
+
+def eye(n, dtype=float32):
+    return torch.eye(n, dtype=dtype, device=_dev())
+
+
+def where(cond, x=None, y=None):
+    if x is None and y is None:
+        return torch.nonzero(_t(cond), as_tuple=True)
+    c = _t(cond)
+    return torch.where(c, _t(x) if not isinstance(x, (int, float)) else x,
+                       _t(y) if not isinstance(y, (int, float)) else y)
+
+
+def clip(x, a_min=None, a_max=None):
+    return torch.clamp(_t(x), min=a_min, max=a_max)
+
+
+def _unary(fn):
+    return lambda x: fn(_t(x))
+
+
+abs = _unary(torch.abs)
+absolute = abs
+sign = _unary(torch.sign)
+exp = _unary(torch.exp)
+log = _unary(torch.log)
+log1p = _unary(torch.log1p)
+expm1 = _unary(torch.expm1)
+sqrt = _unary(torch.sqrt)
+square = _unary(torch.square)
+sin = _unary(torch.sin)
+cos = _unary(torch.cos)
+tan = _unary(torch.tan)
+tanh = _unary(torch.tanh)
+arctan = _unary(torch.atan)
+floor = _unary(torch.floor)
+ceil = _unary(torch.ceil)
+isnan = _unary(torch.isnan)
+isfinite = _unary(torch.isfinite)
+logical_not = _unary(torch.logical_not)
+negative = _unary(torch.neg)
+
+
+def round(x, decimals=0):
+    return torch.round(_t(x), decimals=decimals)
+
+
+def power(x, y):
+    return torch.pow(_t(x), y)
+
+
+def arctan2(y, x):
+    return torch.atan2(_t(y), _t(x))
+
+
+def mod(x, y):
+    """Floored remainder with the sign of the divisor (jnp.mod)."""
+    return torch.remainder(_t(x), y)
+
+
+remainder = mod
+
+
+def minimum(x, y):
+    return torch.minimum(_t(x), _t(y))
+
+
+def maximum(x, y):
+    return torch.maximum(_t(x), _t(y))
+
+
+def logical_and(x, y):
+    return torch.logical_and(_t(x), _t(y))
+
+
+def logical_or(x, y):
+    return torch.logical_or(_t(x), _t(y))
+
+
+def logical_xor(x, y):
+    return torch.logical_xor(_t(x), _t(y))
+
+
+def _reduce(fn, x, axis, keepdims, **kw):
+    x = _t(x)
+    if axis is None:
+        return fn(x, **kw)
+    return fn(x, dim=axis, keepdim=keepdims, **kw)
+
+
+def sum(x, axis=None, keepdims=False):
+    return _reduce(torch.sum, x, axis, keepdims)
+
+
+def prod(x, axis=None, keepdims=False):
+    x = _t(x)
+    if axis is None:
+        return torch.prod(x)
+    return torch.prod(x, dim=axis, keepdim=keepdims)
+
+
+def mean(x, axis=None, keepdims=False):
+    x = _t(x)
+    if not x.is_floating_point():
+        x = x.to(torch.float32)
+    return _reduce(torch.mean, x, axis, keepdims)
+
+
+def var(x, axis=None, keepdims=False, ddof=0):
+    x = _t(x).to(torch.float32) if not _t(x).is_floating_point() else _t(x)
+    if axis is None:
+        return torch.var(x, correction=ddof)
+    return torch.var(x, dim=axis, keepdim=keepdims, correction=ddof)
+
+
+def std(x, axis=None, keepdims=False, ddof=0):
+    return torch.sqrt(var(x, axis, keepdims, ddof))
+
+
+def min(x, axis=None, keepdims=False):
+    x = _t(x)
+    if axis is None:
+        return torch.min(x)
+    return torch.amin(x, dim=axis, keepdim=keepdims)
+
+
+def max(x, axis=None, keepdims=False):
+    x = _t(x)
+    if axis is None:
+        return torch.max(x)
+    return torch.amax(x, dim=axis, keepdim=keepdims)
+
+
+amin, amax = min, max
+
+
+def argmin(x, axis=None):
+    return torch.argmin(_t(x), dim=axis).to(torch.int32)
+
+
+def argmax(x, axis=None):
+    return torch.argmax(_t(x), dim=axis).to(torch.int32)
+
+
+def any(x, axis=None, keepdims=False):
+    x = _t(x).bool()
+    if axis is None:
+        return torch.any(x)
+    return torch.any(x, dim=axis, keepdim=keepdims)
+
+
+def all(x, axis=None, keepdims=False):
+    x = _t(x).bool()
+    if axis is None:
+        return torch.all(x)
+    return torch.all(x, dim=axis, keepdim=keepdims)
+
+
+def count_nonzero(x, axis=None):
+    return torch.count_nonzero(_t(x), dim=axis).to(torch.int32)
+
+
+def cumsum(x, axis=None):
+    x = _t(x)
+    if axis is None:
+        x, axis = x.reshape(-1), 0
+    return torch.cumsum(x, dim=axis)
+
+
+def diff(x, axis=-1):
+    return torch.diff(_t(x), dim=axis)
+
+
+def sort(x, axis=-1):
+    return torch.sort(_t(x), dim=axis, stable=True).values
+
+
+def argsort(x, axis=-1):
+    return torch.argsort(_t(x), dim=axis, stable=True).to(torch.int32)
+
+
+def unique(x):
+    return torch.unique(_t(x), sorted=True)
+
+
+def bincount(x, weights=None, minlength=0, length=None):
+    x = _t(x).long()
+    n = length if length is not None else minlength
+    out = torch.bincount(x, weights=None if weights is None else _t(weights), minlength=n)
+    if length is not None:
+        out = out[:length]
+    return out.to(torch.int32) if weights is None else out
+
+
+def stack(arrays, axis=0):
+    return torch.stack([_t(a) for a in arrays], dim=axis)
+
+
+def concatenate(arrays, axis=0):
+    return torch.cat([_t(a) for a in arrays], dim=axis)
+
+
+def column_stack(arrays):
+    return torch.stack([_t(a) for a in arrays], dim=1)
+
+
+def reshape(x, shape):
+    return _t(x).reshape(shape)
+
+
+def transpose(x, axes=None):
+    x = _t(x)
+    return x.permute(*axes) if axes is not None else x.permute(*reversed(range(x.dim())))
+
+
+def expand_dims(x, axis):
+    return _t(x).unsqueeze(axis)
+
+
+def squeeze(x, axis=None):
+    return _t(x).squeeze() if axis is None else _t(x).squeeze(axis)
+
+
+def take(x, indices, axis=None):
+    x = _t(x)
+    idx = _t(indices).long()
+    if axis is None:
+        return x.reshape(-1)[idx]
+    return torch.index_select(x, axis, idx.reshape(-1)).reshape(
+        *x.shape[:axis], *idx.shape, *x.shape[axis + 1:])
+
+
+def dot(a, b):
+    return torch.matmul(_t(a), _t(b))
+
+
+matmul = dot
+
+
+def outer(a, b):
+    return torch.outer(_t(a), _t(b))
+
+
+def allclose(a, b, rtol=1e-5, atol=1e-8):
+    return builtins.bool(torch.allclose(_t(a).to(torch.float64), _t(b).to(torch.float64),
+                                        rtol=rtol, atol=atol))
+
+
+def array_equal(a, b):
+    a, b = _t(a), _t(b)
+    return builtins.bool(a.shape == b.shape and torch.equal(a, b.to(a.dtype)))
+
+
+def at_set(x, index, value):
+    """``x.at[index].set(value)`` (out of place)."""
+    y = _t(x).clone()
+    y[index] = value
+    return y
+
+
+def at_add(x, index, value):
+    """``x.at[index].add(value)`` (out of place, repeated indices accumulate)."""
+    y = _t(x).clone()
+    idx = index if not isinstance(index, torch.Tensor) else index.long()
+    y.index_put_((idx,) if not isinstance(idx, tuple) else idx,
+                  torch.as_tensor(value, dtype=y.dtype, device=y.device).expand_as(y[idx]),
+                  accumulate=True)
+    return y
+
+
+class linalg:  # noqa: N801 -- mirrors jnp.linalg
+    @staticmethod
+    def norm(x, ord=None, axis=None, keepdims=False):
+        x = _t(x)
+        if not x.is_floating_point():
+            x = x.to(torch.float32)
+        return torch.linalg.norm(x, ord=ord, dim=axis, keepdim=keepdims)

@@ -90,6 +90,7 @@ class DeffuantModel:
         self.loss = torch.empty((B,), dtype=f64, device=dev)
         self.grads = torch.empty((2, B), dtype=f64, device=dev)
         self.target = torch.zeros((STATS_LEN,), dtype=f64, device=dev)
+        self.weights = torch.ones((STATS_LEN,), dtype=f64, device=dev)
         self.eps = torch.zeros((B,), dtype=torch.float32, device=dev)
         self.mu = torch.zeros((B,), dtype=torch.float32, device=dev)
         self.version = 0
@@ -108,12 +109,14 @@ class DeffuantModel:
         p.ckpt_every, p.pair_blocks, p.stat_chunks = (self.ckpt_every, self.pair_blocks,
                                                       self.stat_chunks)
         for name in ("perms", "eps", "mu", "work", "ckpt", "traj", "lam", "acc", "partial",
-                     "sums", "coef", "target", "stats", "loss", "grads"):
+                     "sums", "coef", "target", "stats", "loss", "grads", "weights"):
             setattr(p, name, getattr(self, name).data_ptr())
         return p
 
-    def forward(self, eps, mu, target) -> torch.Tensor:
-        """Run the batch; returns the per-instance float64 losses (device)."""
+    def forward(self, eps, mu, target, weights=None) -> torch.Tensor:
+        """Run the batch; returns the per-instance float64 losses (device).
+        ``weights`` (34 values, default all 1) weight each statistic's squared
+        error."""
         eps = torch.as_tensor(eps, dtype=torch.float32, device=self.device).reshape(-1)
         mu = torch.as_tensor(mu, dtype=torch.float32, device=self.device).reshape(-1)
         if eps.numel() != self.batch or mu.numel() != self.batch:
@@ -121,6 +124,8 @@ class DeffuantModel:
         self.eps.copy_(eps)
         self.mu.copy_(mu)
         self.target.copy_(torch.as_tensor(np.asarray(target, np.float64).reshape(STATS_LEN)))
+        w = np.ones(STATS_LEN) if weights is None else np.asarray(weights, np.float64)
+        self.weights.copy_(torch.as_tensor(w.reshape(STATS_LEN)))
         check(lib().abm_deffuant_forward(C.byref(self._problem()), stream_handle()),
               "deffuant_forward")
         self.version += 1

@@ -28,7 +28,7 @@ from ._lib import MODE_ORIGINAL, MODE_PARTITIONABLE, check, lib, ptr, stream_han
 
 __all__ = [
     "MODE_ORIGINAL", "MODE_PARTITIONABLE", "PRNGKey", "key_data", "split",
-    "bits", "uniform", "normal", "bernoulli", "randint", "permutation", "permutation_each",
+    "bits", "uniform", "normal", "bernoulli", "randint", "choice", "permutation", "permutation_each",
     "split_each", "uniform_each", "normal_each", "bernoulli_each",
     "randint_each", "bits_each", "set_threefry_partitionable", "default_mode",
 ]
@@ -107,16 +107,14 @@ def _single(key) -> torch.Tensor:
     return k
 
 
-def split(key, num: int = 2, mode=None) -> torch.Tensor:
-    """``jax.random.split(key, num)`` -> (num, 2)."""
+def _split_impl(key, num: int = 2, mode=None) -> torch.Tensor:
     k = _single(key)
     out = torch.empty((int(num), 2), dtype=torch.int32, device=k.device)
     check(lib().abm_split(ptr(k), int(num), ptr(out), _mode(mode), stream_handle()), "split")
     return out
 
 
-def bits(key, shape=(), mode=None) -> torch.Tensor:
-    """``jax.random.bits(key, shape, uint32)`` (returned as int32 storage)."""
+def _bits_impl(key, shape=(), mode=None) -> torch.Tensor:
     k = _single(key)
     shp = _shape(shape)
     out = torch.empty(shp, dtype=torch.int32, device=k.device)
@@ -124,10 +122,7 @@ def bits(key, shape=(), mode=None) -> torch.Tensor:
     return out
 
 
-def uniform(key, shape=(), dtype=torch.float32, minval=0.0, maxval=1.0, mode=None) -> torch.Tensor:
-    """``jax.random.uniform(key, shape, float32, minval, maxval)``."""
-    if dtype not in (torch.float32, None):
-        raise NotImplementedError("uniform: float32 only (the reference runs with jax_enable_x64 off)")
+def _uniform_impl(key, shape=(), minval=0.0, maxval=1.0, mode=None) -> torch.Tensor:
     k = _single(key)
     shp = _shape(shape)
     out = torch.empty(shp, dtype=torch.float32, device=k.device)
@@ -136,8 +131,7 @@ def uniform(key, shape=(), dtype=torch.float32, minval=0.0, maxval=1.0, mode=Non
     return out
 
 
-def normal(key, shape=(), dtype=torch.float32, mode=None) -> torch.Tensor:
-    """``jax.random.normal(key, shape, float32)``."""
+def _normal_impl(key, shape=(), mode=None) -> torch.Tensor:
     k = _single(key)
     shp = _shape(shape)
     out = torch.empty(shp, dtype=torch.float32, device=k.device)
@@ -145,8 +139,7 @@ def normal(key, shape=(), dtype=torch.float32, mode=None) -> torch.Tensor:
     return out
 
 
-def bernoulli(key, p=0.5, shape=(), mode=None) -> torch.Tensor:
-    """``jax.random.bernoulli(key, p, shape)`` -> bool."""
+def _bernoulli_impl(key, p=0.5, shape=(), mode=None) -> torch.Tensor:
     k = _single(key)
     shp = _shape(shape)
     out = torch.empty(shp, dtype=torch.uint8, device=k.device)
@@ -155,8 +148,7 @@ def bernoulli(key, p=0.5, shape=(), mode=None) -> torch.Tensor:
     return out.bool()
 
 
-def randint(key, shape, minval, maxval, dtype=torch.int32, mode=None) -> torch.Tensor:
-    """``jax.random.randint(key, shape, minval, maxval, int32)``."""
+def _randint_impl(key, shape, minval, maxval, mode=None) -> torch.Tensor:
     k = _single(key)
     shp = _shape(shape)
     lo = int(np.clip(int(minval), -2**31, 2**31 - 1))
@@ -165,6 +157,136 @@ def randint(key, shape, minval, maxval, dtype=torch.int32, mode=None) -> torch.T
     check(lib().abm_randint_i32(ptr(k), _numel(shp), lo, hi, ptr(out), _mode(mode),
                                 stream_handle()), "randint")
     return out
+
+
+# ---------------------------------------------------------------- vmap-able draws
+# The draws are torch custom ops with vmap rules: called with one key they run
+# the single-key kernels; inside torch.func.vmap (what AgentCollection uses to
+# map a per-agent function over the agents, as jax.vmap does in
+# jaxabm/agent.py:115-177) the batched key array goes to the per-key kernels
+# in one launch -- the same bits JAX produces for a vmapped draw.
+def _batched_keys(info, dim, key) -> torch.Tensor:
+    if dim is None:
+        return key.reshape(1, 2).expand(info.batch_size, 2).contiguous()
+    return key.movedim(dim, 0).contiguous()
+
+
+@torch.library.custom_op("jaxabm::split", mutates_args=(), device_types="cuda")
+def _op_split(key: torch.Tensor, num: int, mode: int) -> torch.Tensor:
+    return _split_impl(key, num, mode)
+
+
+@_op_split.register_fake
+def _(key, num, mode):
+    return key.new_empty((num, 2))
+
+
+@_op_split.register_vmap
+def _(info, in_dims, key, num, mode):
+    return split_each(_batched_keys(info, in_dims[0], key), num, mode), 0
+
+
+_DRAW_DTYPES = {0: torch.int32, 1: torch.float32, 2: torch.float32}
+
+
+@torch.library.custom_op("jaxabm::draw", mutates_args=(), device_types="cuda")
+def _op_draw(key: torch.Tensor, shape: list[int], dist: int, lo: float, hi: float,
+             mode: int) -> torch.Tensor:
+    if dist == 0:
+        return _bits_impl(key, tuple(shape), mode)
+    if dist == 1:
+        return _uniform_impl(key, tuple(shape), lo, hi, mode)
+    return _normal_impl(key, tuple(shape), mode)
+
+
+@_op_draw.register_fake
+def _(key, shape, dist, lo, hi, mode):
+    return key.new_empty(tuple(shape), dtype=_DRAW_DTYPES[dist])
+
+
+@_op_draw.register_vmap
+def _(info, in_dims, key, shape, dist, lo, hi, mode):
+    k = _batched_keys(info, in_dims[0], key)
+    return _draw_each(k, tuple(shape), dist, lo, hi, _DRAW_DTYPES[dist], mode), 0
+
+
+@torch.library.custom_op("jaxabm::randint", mutates_args=(), device_types="cuda")
+def _op_randint(key: torch.Tensor, shape: list[int], lo: int, hi: int, mode: int) -> torch.Tensor:
+    return _randint_impl(key, tuple(shape), lo, hi, mode)
+
+
+@_op_randint.register_fake
+def _(key, shape, lo, hi, mode):
+    return key.new_empty(tuple(shape), dtype=torch.int32)
+
+
+@_op_randint.register_vmap
+def _(info, in_dims, key, shape, lo, hi, mode):
+    return randint_each(_batched_keys(info, in_dims[0], key), tuple(shape), lo, hi, mode), 0
+
+
+def _key_arg(key) -> torch.Tensor:
+    if isinstance(key, torch.Tensor):
+        if key.shape[-1:] != (2,):
+            raise ValueError(f"a PRNG key has trailing dimension 2, got shape {tuple(key.shape)}")
+        if key.dtype != torch.int32:
+            key = key.to(torch.int32)
+        return key if key.is_cuda else key.to(_device())
+    return _as_key(key)
+
+
+def split(key, num: int = 2, mode=None) -> torch.Tensor:
+    """``jax.random.split(key, num)`` -> (num, 2)."""
+    return torch.ops.jaxabm.split(_key_arg(key), int(num), _mode(mode))
+
+
+def bits(key, shape=(), mode=None) -> torch.Tensor:
+    """``jax.random.bits(key, shape, uint32)`` (returned as int32 storage)."""
+    return torch.ops.jaxabm.draw(_key_arg(key), list(_shape(shape)), 0, 0.0, 0.0, _mode(mode))
+
+
+def uniform(key, shape=(), dtype=torch.float32, minval=0.0, maxval=1.0, mode=None) -> torch.Tensor:
+    """``jax.random.uniform(key, shape, float32, minval, maxval)``.  Tensor
+    bounds (e.g. per agent under vmap) scale a [0, 1) draw with torch ops."""
+    if dtype not in (torch.float32, None):
+        raise NotImplementedError("uniform: float32 only (the reference runs with jax_enable_x64 off)")
+    k = _key_arg(key)
+    if isinstance(minval, torch.Tensor) or isinstance(maxval, torch.Tensor):
+        f = torch.ops.jaxabm.draw(k, list(_shape(shape)), 1, 0.0, 1.0, _mode(mode))
+        lo = torch.as_tensor(minval, dtype=torch.float32, device=f.device)
+        hi = torch.as_tensor(maxval, dtype=torch.float32, device=f.device)
+        return torch.maximum(lo, f * (hi - lo) + lo)
+    return torch.ops.jaxabm.draw(k, list(_shape(shape)), 1, float(minval), float(maxval),
+                                 _mode(mode))
+
+
+def normal(key, shape=(), dtype=torch.float32, mode=None) -> torch.Tensor:
+    """``jax.random.normal(key, shape, float32)``."""
+    return torch.ops.jaxabm.draw(_key_arg(key), list(_shape(shape)), 2, 0.0, 0.0, _mode(mode))
+
+
+def bernoulli(key, p=0.5, shape=(), mode=None) -> torch.Tensor:
+    """``jax.random.bernoulli(key, p, shape)`` = ``uniform(key, shape) < p``."""
+    if isinstance(p, torch.Tensor):
+        shp = _shape(shape) if shape not in ((), None) else tuple(p.shape)
+        return uniform(key, shp, mode=mode) < p.to(torch.float32)
+    return uniform(key, shape, mode=mode) < float(p)
+
+
+def randint(key, shape, minval, maxval, dtype=torch.int32, mode=None) -> torch.Tensor:
+    """``jax.random.randint(key, shape, minval, maxval, int32)``."""
+    lo = int(np.clip(int(minval), -2**31, 2**31 - 1))
+    hi = int(np.clip(int(maxval), -2**31, 2**31 - 1))
+    return torch.ops.jaxabm.randint(_key_arg(key), list(_shape(shape)), lo, hi, _mode(mode))
+
+
+def choice(key, a, shape=(), mode=None):
+    """``jax.random.choice(key, a, shape)`` with replacement:
+    ``a[randint(key, shape, 0, len(a))]`` (a an int n or a 1-D tensor)."""
+    if isinstance(a, int):
+        return randint(key, shape, 0, a, mode=mode)
+    idx = randint(key, shape, 0, a.shape[0], mode=mode)
+    return a[idx.long()]
 
 
 def permutation(key, x, mode=None) -> torch.Tensor:

@@ -1,0 +1,394 @@
+"""GPU: the core API and the AgentPy-style façade.
+
+Ports the reference's structural tests (tests/unit/test_agent.py:141-262,
+tests/unit/test_model.py:194-300, tests/unit/test_api.py,
+tests/integration/test_integration.py:290-348) and adds what they leave
+open: the key schedule (agent keys, per-step collection and env keys) and
+the vmapped per-agent draws are checked bit for bit against the oracle's
+restatement of JAX's threefry construction."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+PART = O.MODE_PARTITIONABLE
+
+
+@pytest.fixture(scope="module")
+def J():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import jaxabm_amd
+
+    return jaxabm_amd
+
+
+class EconAgent:
+    """tests/unit/test_agent.py TestAgent."""
+
+    def init_state(self, model_config, key):
+        from jaxabm_amd import random as jr
+
+        k1, k2 = jr.split(key)
+        return {"wealth": jr.uniform(k1, minval=0.0, maxval=100.0),
+                "productivity": jr.uniform(k2, minval=0.5, maxval=1.5)}
+
+    def update(self, state, model_state, model_config, key):
+        income = state["productivity"] * model_state.get("wage_rate", 1.0)
+        return {"wealth": state["wealth"] + income, "productivity": state["productivity"]}
+
+
+def test_collection_init_matches_jax_schedule(J):
+    """init: agent keys = split(key, n); vmap(init_state): each agent's draws
+    are uniform(split(agent_key)[i]) exactly."""
+    import torch
+    from jaxabm_amd import random as jr
+
+    key = jr.PRNGKey(0)
+    agents = J.AgentCollection(EconAgent(), 10)
+    agents.init(key, J.ModelConfig(seed=42, steps=5))
+    st = agents.get_states()
+    assert st["wealth"].shape == (10,) and st["productivity"].shape == (10,)
+    assert bool(torch.all((st["wealth"] >= 0) & (st["wealth"] <= 100)))
+    assert bool(torch.all((st["productivity"] >= 0.5) & (st["productivity"] <= 1.5)))
+    ak = O.split(np.array([0, 0], np.uint32), 10, PART)
+    want_w = np.array([O.uniform(O.split(k, 2, PART)[0], 1, 0.0, 100.0, PART)[0] for k in ak],
+                      np.float32)
+    want_p = np.array([O.uniform(O.split(k, 2, PART)[1], 1, 0.5, 1.5, PART)[0] for k in ak],
+                      np.float32)
+    np.testing.assert_array_equal(st["wealth"].cpu().numpy(), want_w)
+    np.testing.assert_array_equal(st["productivity"].cpu().numpy(), want_p)
+
+
+def test_collection_update_aggregate_filter(J):
+    import torch
+    from jaxabm_amd import numpy as jnp
+    from jaxabm_amd import random as jr
+
+    agents = J.AgentCollection(EconAgent(), 10)
+    cfg = J.ModelConfig(seed=42, steps=5)
+    agents.init(jr.PRNGKey(0), cfg)
+    w0 = agents.get_states()["wealth"].clone()
+    p0 = agents.get_states()["productivity"].clone()
+    agents.update({"wage_rate": 1.0}, jr.PRNGKey(1), cfg)
+    assert torch.allclose(agents.get_states()["wealth"] - w0, p0, rtol=1e-5)
+    assert abs(float(agents.aggregate("wealth", jnp.sum)) -
+               float(agents.get_states()["wealth"].sum())) < 1e-3
+    assert abs(float(agents.aggregate("productivity")) -
+               float(agents.get_states()["productivity"].mean())) < 1e-6
+    with pytest.raises(ValueError, match="not found"):
+        agents.aggregate("nope")
+    rich = agents.filter(lambda s: s["wealth"] > 50.0)
+    assert bool(torch.all(rich.get_states()["wealth"] > 50.0))
+    assert rich.num_agents == int((agents.get_states()["wealth"] > 50).sum())
+
+
+class DummyAgent:
+    """tests/unit/test_model.py DummyAgent."""
+
+    def init_state(self, model_config, key):
+        from jaxabm_amd import random as jr
+
+        return {"value": jr.uniform(key, minval=0.0, maxval=10.0)}
+
+    def update(self, state, model_state, model_config, key):
+        return {"value": state["value"] + model_state["env"].get("increment", 1.0)}
+
+
+def counter_fn(env_state, agent_states, params, key):
+    new = dict(env_state)
+    new["counter"] = env_state.get("counter", 0) + 1
+    return new
+
+
+def metrics_fn(env_state, agent_states, params):
+    from jaxabm_amd import numpy as jnp
+
+    out = {}
+    c = agent_states.get("consumers")
+    if c and "value" in c:
+        out["total_value"] = jnp.sum(c["value"])
+    out["step_counter"] = env_state.get("counter", 0)
+    return out
+
+
+def make_core(J, steps=20, seed=42):
+    m = J.JaxModel(params={"adjustment_rate": 0.1}, config=J.ModelConfig(steps=steps, seed=seed),
+                   update_state_fn=counter_fn, metrics_fn=metrics_fn)
+    m.add_agent_collection("consumers", J.AgentCollection(DummyAgent(), 10))
+    for k, v in {"price_level": 1.0, "interest_rate": 0.05, "increment": 1.0}.items():
+        m.add_env_state(k, v)
+    return m
+
+
+def test_core_model_step_run_and_schedule(J):
+    m = make_core(J)
+    with pytest.raises(RuntimeError):
+        m.step()
+    m.initialize()
+    with pytest.raises(RuntimeError):
+        m.add_agent_collection("more", J.AgentCollection(DummyAgent(), 2))
+    # initialize: keys = split(PRNGKey(42), 2); collection init with keys[1]
+    keys = O.split(np.array([0, 42], np.uint32), 2, PART)
+    ak = O.split(keys[1], 10, PART)
+    want = np.array([O.uniform(k, 1, 0.0, 10.0, PART)[0] for k in ak], np.float32)
+    v0 = m.agent_collections["consumers"].states["value"]
+    np.testing.assert_array_equal(v0.cpu().numpy(), want)
+    np.testing.assert_array_equal(m._rng, keys[0])
+    st = m.model_state()
+    assert st["time_step"] == 0 and "agents_consumers" in st and st["env"]["increment"] == 1.0
+    total0 = float(v0.sum())
+    metrics = m.step()
+    assert m._time_step == 1 and metrics["step_counter"] == 1
+    assert abs(float(metrics["total_value"]) - (total0 + 10.0)) < 1e-3
+    assert m._env_state["counter"] == 1
+    # the step consumed rng, sk = split(rng); sk, ck = split(sk); sk, uk = split(sk)
+    r, _ = O.split(keys[0], 2, PART)
+    np.testing.assert_array_equal(m._rng, r)
+    assert m.step()["step_counter"] == 2
+    m.add_env_state("new_variable", 42.0)
+    assert m.state["env"]["new_variable"] == 42.0
+
+
+def test_core_model_run_history(J, capsys):
+    m = make_core(J)
+    res = m.run(steps=15)
+    assert "Ran 15 steps" in capsys.readouterr().out
+    assert len(m._history) == 15 and len(res["step"]) == 15
+    assert res["step_counter"][0] == 1 and res["step_counter"][-1] == 15
+    again = m.run()  # continues the state, resets the history
+    assert len(again["step"]) == 20 and again["step"][0] == 16
+
+
+def test_per_agent_function_under_vmap(J):
+    """A per-agent function with a per-agent vector variable, a reduction
+    over it, a draw, a jnp.where and a constant output."""
+    import torch
+    from jaxabm_amd import numpy as jnp
+    from jaxabm_amd import random as jr
+
+    class Mover:
+        def init_state(self, cfg, key):
+            return {"pos": jr.uniform(key, (2,)), "tag": 7}
+
+        def update(self, state, model_state, cfg, key):
+            step = jr.normal(key, (2,)) * 0.1
+            pos = jnp.clip(state["pos"] + step, 0.0, 1.0)
+            far = jnp.sum(pos * pos) > 0.5
+            return {"pos": pos, "tag": jnp.where(far, 1, 0)}
+
+    coll = J.AgentCollection(Mover(), 6)
+    cfg = J.ModelConfig()
+    coll.init(jr.PRNGKey(3), cfg)
+    assert coll.states["pos"].shape == (6, 2) and coll.states["tag"].tolist() == [7] * 6
+    pos0 = coll.states["pos"].clone()
+    coll.update({}, jr.PRNGKey(4), cfg)
+    ak = O.split(np.array([0, 4], np.uint32), 6, PART)
+    steps = np.stack([O.normal(k, 2, PART) for k in ak]).astype(np.float32)
+    want = np.clip(pos0.cpu().numpy() + steps * np.float32(0.1), 0, 1)
+    np.testing.assert_allclose(coll.states["pos"].cpu().numpy(), want, rtol=1e-5, atol=1e-6)
+    far = (coll.states["pos"] ** 2).sum(1) > 0.5
+    assert torch.equal(coll.states["tag"].bool(), far)
+
+
+def test_facade_model_run(J, capsys):
+    """The AgentPy-style Model: setup, agent step (no key), env-phase step with
+    record, env override, Results."""
+    from jaxabm_amd import numpy as jnp
+
+    class Walker(J.Agent):
+        def setup(self):
+            return {"x": 0.0, "energy": 10}
+
+        def step(self, model_state):
+            return {"x": self._state["x"] + model_state["env"]["speed"],
+                    "energy": self._state["energy"] - 1}
+
+    class World(J.Model):
+        def setup(self):
+            self.walkers = self.add_agents(5, Walker, name="walkers")
+            self.env.speed = 0.5
+            self.env.add_state("fixed", 3)
+
+        def step(self):
+            self.record("mean_x", float(jnp.mean(self.walkers.x)))
+            self.env.add_state("ticks", self.env.state.get("ticks", 0) + 1)
+
+        def compute_metrics(self, env_state, agent_states, params):
+            return {"energy": jnp.sum(agent_states["walkers"]["energy"])}
+
+    res = World({"steps": 4, "seed": 1}).run()
+    assert "Simulation executed in" in capsys.readouterr().out
+    assert res["step"] == [1, 2, 3, 4]
+    np.testing.assert_allclose(res["mean_x"], [0.5, 1.0, 1.5, 2.0])
+    assert [int(e) for e in res["energy"]] == [45, 40, 35, 30]
+    lst = World({"steps": 2}).run()
+    assert len(lst["step"]) == 2
+
+
+def test_grid_position_agents(J):
+    from jaxabm_amd import random as jr
+
+    class Cell(J.Agent):
+        def setup(self):
+            return {"v": 1}
+
+    class G(J.Model):
+        def setup(self):
+            self.agents = self.add_agents(8, Cell)
+            self.grid = J.Grid(self, (10, 7), periodic=True)
+
+    m = G({"seed": 5})
+    m.setup()
+    assert m.env.grid_shape == (10, 7) and m.env.grid_periodic is True
+    m.grid.position_agents(m.agents)  # no-op before the collection is initialised
+    assert m.agents.collection.states is None
+    m.agents.collection.init(jr.PRNGKey(0), J.ModelConfig())
+    m.grid.position_agents(m.agents)
+    pos = m.agents.collection.states["position"].cpu().numpy()
+    k = np.array([0, 5], np.uint32)
+    np.testing.assert_array_equal(pos[:, 0], O.randint(k, 8, 0, 10, PART))
+    np.testing.assert_array_equal(pos[:, 1], O.randint(O.split(k, 2, PART)[1], 8, 0, 7, PART))
+
+
+def test_network_semantics_and_kernels(J):
+    import torch
+
+    class N(J.Model):
+        pass
+
+    m = N()
+    net = J.Network(m)
+    edges = [(0, 1), (1, 2), (2, 0), (3, 3), (1, 0), (4, 1)]
+    for a, b in edges:
+        net.add_edge(a, b)
+    want = []
+    for a, b in edges:
+        want.append((a, b))
+        if a != b:
+            want.append((b, a))
+    assert net.edges().cpu().tolist() == [list(e) for e in want]
+    assert m.env.network_edges.shape == (len(want), 2)
+    assert net.get_neighbors(1).tolist() == [0, 2, 4]
+    assert net.get_neighbors(3).tolist() == [3]
+    d = J.Network(N(), directed=True)
+    for a, b in [(0, 2), (0, 1), (0, 2), (1, 0)]:
+        d.add_edge(a, b)
+    assert d.get_neighbors(0).tolist() == [2, 1, 2]
+    # bulk BA network: neighbour reductions against numpy over the CSR
+    ba = J.Network.barabasi_albert(N(), 3000, 4, seed=2)
+    g = ba.csr()
+    rp, col = g.host_row_ptr, g.host_col
+    assert (np.diff(rp) > 32).any()  # hub rows take the wavefront path
+    x = torch.rand(3000, device="cuda")
+    mask = torch.rand(3000, device="cuda") < 0.3
+    xs, ms = x.cpu().numpy().astype(np.float64), mask.cpu().numpy()
+    want_sum = np.array([xs[col[rp[v]:rp[v + 1]]].sum() for v in range(3000)])
+    want_cnt = np.array([ms[col[rp[v]:rp[v + 1]]].sum() for v in range(3000)])
+    np.testing.assert_allclose(ba.neighbor_sum(x).cpu().numpy(), want_sum, rtol=1e-6)
+    np.testing.assert_array_equal(ba.neighbor_count(mask).cpu().numpy(), want_cnt)
+    from jaxabm_amd.network import scatter_add
+
+    idx = torch.randint(0, 50, (5000,), device="cuda")
+    out = scatter_add(torch.zeros(50, dtype=torch.int32, device="cuda"), idx,
+                      torch.ones(5000, dtype=torch.int32, device="cuda"))
+    np.testing.assert_array_equal(out.cpu().numpy(), np.bincount(idx.cpu().numpy(), minlength=50))
+    with pytest.raises(IndexError):
+        scatter_add(torch.zeros(4, device="cuda"), torch.tensor([4], device="cuda"),
+                    torch.ones(1, device="cuda"))
+
+
+def test_library_models_match_oracle(J):
+    from jaxabm_amd.programs import library as L
+
+    r = L.WealthModel({"n": 500, "steps": 30}, seed=3).run()
+    x, y, w, rng = O.wealth_init(500, 3, PART)
+    mean, gini = O.wealth_run(x, y, w, rng, 30, PART)
+    np.testing.assert_array_equal(np.float32(r["gini"]), gini)
+    np.testing.assert_array_equal(np.float32(r["mean_wealth"]), mean)
+    r = L.NetworkSIRModel({"n": 3000, "steps": 40, "beta": 0.08}, seed=1).run()
+    rp, col = O.ba_csr(3000, 4, 0)
+    st, rng = O.sir_init(3000, 100, 1, PART)
+    c = O.sir_run(rp, col, st, rng, np.float32(0.08), np.float32(0.1), 40, PART)
+    np.testing.assert_array_equal(np.array([r["S"], r["I"], r["R"]]).T, c)
+    r = L.SchellingModel({"height": 40, "width": 30, "steps": 12}, seed=2).run()
+    cells, pos, typ, rng = O.schelling_init(40, 30, 2, PART)
+    hf, mv = O.schelling_run(40, 30, cells, pos, typ, rng, 12, PART)
+    np.testing.assert_array_equal(np.float32(r["fraction_happy"]), hf)
+    np.testing.assert_array_equal(r["moves"], mv)
+
+
+def test_deffuant_loss_gradient_matches_oracle(J):
+    from jaxabm_amd.programs import library as L
+
+    n, steps = 2048, 30
+    target = {"opinion_mean": 0.45, "opinion_var": 0.01, "hist_10": 0.05}
+    weights = {"opinion_mean": 1.0, "opinion_var": 2.0, "hist_10": 0.5}
+    m = L.DeffuantOpinionModel({"n": n, "eps": 0.15, "mu": 0.25, "steps": steps,
+                                "ckpt_every": 7}, seed=0)
+    g = m.loss_gradient(target, weights, steps=steps)
+    # the oracle's adjoint weighs every statistic by 1: compare the pieces by
+    # linearity (dL/dtheta = sum_j w_j dL_j/dtheta) using three single-stat runs
+    x0, rng0 = O.deffuant_init(n, 1, 0, PART)
+    tot_e = tot_m = 0.0
+    for name, w in weights.items():
+        j = L.DEFFUANT_METRICS.index(name)
+        only = L.DeffuantOpinionModel({"n": n, "eps": 0.15, "mu": 0.25, "steps": steps}, seed=0)
+        gj = only.loss_gradient({name: target[name]}, {name: 1.0}, steps=steps)
+        tot_e += w * gj["eps"]
+        tot_m += w * gj["mu"]
+        if name == "opinion_mean":
+            # single-stat gradient against the oracle: target = the run's own
+            # stats except this one, so the other terms vanish
+            st = O.deffuant_stats(_oracle_final(x0, rng0, steps))[0]
+            tv = st.copy()
+            tv[j] = target[name]
+            _, oge, ogm = O.deffuant_grad(x0, np.array([np.float32(0.15)], np.float64),
+                                          np.array([np.float32(0.25)], np.float64), rng0, steps,
+                                          tv, PART)
+            np.testing.assert_allclose([gj["eps"], gj["mu"]], [oge[0], ogm[0]], rtol=2e-3,
+                                       atol=1e-7)
+    # the weighted loss's gradient is the weighted sum of the single-statistic
+    # gradients, up to float32 rounding of the adjoint sweep (lam is float32)
+    np.testing.assert_allclose([g["eps"], g["mu"]], [tot_e, tot_m], rtol=1e-6, atol=1e-12)
+
+
+def _oracle_final(x0, rng0, steps):
+    x = x0.copy()
+    O.deffuant_forward(x, np.float32([0.15]), np.float32([0.25]), rng0.copy(), steps, PART)
+    return x
+
+
+def test_analysis_clients(J):
+    from jaxabm_amd.analysis import ModelCalibrator, SensitivityAnalysis
+    from jaxabm_amd.programs import library as L
+
+    def factory(params=None, config=None):
+        return L.WealthModel({"n": int(params["n"]), "steps": 20}, config=config)
+
+    sa = SensitivityAnalysis(factory, {"n": (100.0, 2000.0)}, ["gini", "mean_wealth"],
+                             num_samples=4, seed=0)
+    assert sa.samples.shape == (4, 1)
+    lo = np.sort(sa.samples[:, 0])
+    assert ((lo - 100.0) / 1900.0 * 4).astype(int).tolist() == [0, 1, 2, 3]  # one per stratum
+    res = sa.run(verbose=False)
+    assert np.allclose(res["mean_wealth"], 1.0)
+    idx = sa.sobol_indices()
+    assert set(idx) == {"gini", "mean_wealth"} and 0.0 <= idx["gini"]["n"] <= 1.0
+
+    def dfactory(params=None, config=None):
+        return L.DeffuantOpinionModel({"n": 1024, "eps": params["eps"], "mu": params["mu"],
+                                       "steps": 20, "ckpt_every": 10}, config=config)
+
+    cal = ModelCalibrator(dfactory, {"eps": 0.3, "mu": 0.3}, {"opinion_var": 0.02},
+                          param_bounds={"eps": (0.05, 0.5), "mu": (0.05, 0.5)},
+                          learning_rate=0.05, max_iterations=6, method="adam",
+                          evaluation_steps=20)
+    best = cal.calibrate(verbose=False)
+    assert cal.loss_history[0] >= min(cal.loss_history)
+    assert set(best) == {"eps", "mu"}
+    with pytest.raises(NotImplementedError):
+        ModelCalibrator(dfactory, {"eps": 0.3}, {"opinion_var": 0.02}, method="dqn")
