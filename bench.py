@@ -86,6 +86,195 @@ def cpu_baseline(row_ptr, col, runs: int) -> dict:
     }
 
 
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "k_sir_step_traffic.json")
+
+
+def measured_traffic():
+    """HBM bytes per k_sir_step launch from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the same
+    counters' reading for a copy of known size), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            d = json.load(f)
+        return {"bytes_per_launch": d["bytes_per_launch"], "source": "profiles/" +
+                os.path.basename(TRAFFIC_FILE)}
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def pmc_probe() -> int:
+    """A 256 MiB device-to-device copy (3 times: the counters' calibration),
+    then one 365-step SIR simulation launched step by step."""
+    import torch
+
+    from jaxabm_amd.network import ba_csr
+    from jaxabm_amd.programs.sir import SIRNetworkEngine
+
+    nbytes = 256 << 20
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    row_ptr, col = ba_csr(N_AGENTS, BA_M, GRAPH_SEED)
+    eng = SIRNetworkEngine(N_AGENTS, csr=(row_ptr, col), beta=BETA, gamma=GAMMA,
+                           n_seed_infected=N_SEED, seed=SEED)
+    counts = eng.run(SIM_STEPS)
+    print(json.dumps({"probe": "copy 256 MiB x3 (read 256 MiB + write 256 MiB each) + one "
+                      "365-step SIR simulation", "copy_bytes": nbytes,
+                      "final_counts": counts[-1].tolist()}))
+    return 0
+
+
+def _timed(stream, fn, runs: int) -> float:
+    """Seconds per call of fn(), HIP events on ``stream`` around ``runs`` calls."""
+    import torch
+
+    from jaxabm_amd._lib import check, lib
+
+    sh = C.c_void_p(stream.cuda_stream)
+    torch.cuda.synchronize()
+    check(lib().abm_event_timer(sh, 0, None), "timer")
+    for _ in range(runs):
+        fn()
+    ms = C.c_float()
+    check(lib().abm_event_timer(sh, 1, C.byref(ms)), "timer")
+    torch.cuda.synchronize()
+    return ms.value / 1e3 / runs
+
+
+def bench_deffuant(args) -> dict:
+    """Differentiable Deffuant: 65,536 agents x 256 (eps, mu) instances x 200
+    steps, checkpoints every 20; one bench step = one forward + adjoint
+    gradient evaluation of the whole batch."""
+    import numpy as np
+    import torch
+
+    from jaxabm_amd.programs.deffuant import DeffuantModel, target_stats
+
+    n, B, S, K = 65536, 256, 200, 20
+    target = target_stats(n, S, seed=0)
+    g = np.linspace(0.05, 0.5, 16, dtype=np.float32)
+    eps, mu = np.repeat(g, 16), np.tile(g, 16)
+    m = DeffuantModel(n, B, S, seed=0, ckpt_every=K)
+    stream = torch.cuda.current_stream()
+
+    def fwd():
+        m.forward(eps, mu, target)
+
+    def both():
+        m.forward(eps, mu, target)
+        m.backward()
+
+    for _ in range(args.warmup):
+        both()
+    t_fwd = _timed(stream, fwd, args.steps)
+    t_both = _timed(stream, both, args.steps)
+    fwd_bytes = n * B * 8 + 4 * n  # x read + write, the pairing
+    step_s = t_fwd / S
+    achieved = fwd_bytes / step_s / 1e9
+    return {
+        "metric": "agent-instance-steps/s (Deffuant gradient evaluation, forward + adjoint)",
+        "value": n * B * S / t_both, "unit": "agent-instance-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_both * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded opinions, 16 x 16 grid of (eps, mu) in [0.05, 0.5]^2)",
+        "config": {"workload": "deffuant_65536x256_200steps_ckpt20", "n_agents": n,
+                   "instances": B, "sim_steps": S, "ckpt_every": K,
+                   "bench_step": "one forward + adjoint of the batch"},
+        "forward_ms": t_fwd * 1e3, "adjoint_ms": (t_both - t_fwd) * 1e3,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_deffuant_step (forward average incl. final statistics)",
+                     "algorithmic_bytes_per_launch": fwd_bytes, "avg_kernel_us": step_s * 1e6},
+    }
+
+
+def bench_wealth(args) -> dict:
+    """Wealth exchange, 1,000 agents, 100 steps, seed 0: one bench step = one
+    whole simulation (state init + one single-workgroup launch)."""
+    import numpy as np
+    import torch
+
+    from jaxabm_amd._lib import check, lib, ptr, stream_handle
+    from jaxabm_amd.programs.wealth import WealthEngine
+
+    n, S = 1000, 100
+    eng = WealthEngine(n, seed=0)
+    keys = eng.key_table(S)
+    mean = torch.empty(S, dtype=torch.float32, device="cuda")
+    gini = torch.empty(S, dtype=torch.float32, device="cuda")
+    desc = torch.from_numpy(eng.descriptor(keys, S, mean, gini).view(np.uint8)).to("cuda")
+    from jaxabm_amd import random as jr
+
+    ik = torch.from_numpy(jr.host_split(jr.host_prng_key(0), 2, eng.mode)[1].view(np.int32)
+                          .copy()).to("cuda")
+    stream = torch.cuda.current_stream()
+
+    def run():
+        check(lib().abm_wealth_init(ptr(ik), n, ptr(eng.x), ptr(eng.y), ptr(eng.w), eng.mode,
+                                    stream_handle()), "init")
+        check(lib().abm_wealth_run_batch(ptr(desc), 1, eng.mode, stream_handle()), "run")
+
+    for _ in range(args.warmup):
+        run()
+    t = _timed(stream, run, args.steps)
+    return {
+        "metric": "steps/s (wealth exchange, 1,000 agents)", "value": S / t, "unit": "steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "i32",
+        "data": "synthetic (seeded)",
+        "config": {"workload": "wealth_1000agents_100steps", "n_agents": n, "sim_steps": S,
+                   "bench_step": "one full simulation"},
+        "run_latency_us": t * 1e6, "agent_steps_per_s": n * S / t,
+        "roofline": {"bound": "latency", "achieved": None, "peak": None, "unit": None,
+                     "frac": None, "traffic": None,
+                     "note": "one workgroup holds the 32 KB/step working set; launch- and "
+                             "barrier-bound, a bandwidth roofline does not apply"},
+    }
+
+
+def bench_schelling(args) -> dict:
+    """Schelling, 512 x 512 torus, 200 steps: one bench step = one whole
+    simulation from the seeded grid."""
+    import torch
+
+    from jaxabm_amd.programs.schelling import SchellingEngine
+
+    H = W = 512
+    S = 200
+    eng = SchellingEngine(H, W, seed=0)
+    keys = eng.key_table(S)
+    counters = torch.empty((S, 2), dtype=torch.int32, device="cuda")
+    happy = torch.empty(S, dtype=torch.float32, device="cuda")
+    moves = torch.empty(S, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def run():
+        eng.reset(0)
+        eng.launch(keys, S, counters, happy, moves)
+
+    for _ in range(args.warmup):
+        run()
+    t = _timed(stream, run, args.steps)
+    cell_bytes = 10.8
+    achieved = cell_bytes * H * W * S / t / 1e9
+    return {
+        "metric": "cell-steps/s (Schelling 512 x 512)", "value": H * W * S / t,
+        "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "i8", "data": "synthetic (seeded grid)",
+        "config": {"workload": "schelling_512x512_200steps", "cells": H * W, "sim_steps": S,
+                   "bench_step": "one full simulation incl. the seeded reset"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "note": "10.8 algorithmic B per cell-step over the whole step (4 launches); "
+                             "the 2.8 MB/step working set is cache-resident, so this is "
+                             "launch/barrier-bound"},
+    }
+
+
 def serving_summary(n_requests: int) -> dict:
     from jaxabm_amd import serving
 
@@ -120,12 +309,30 @@ def main() -> int:
     ap.add_argument("--no-graph", action="store_true", help="launch kernels directly instead of a hipGraph")
     ap.add_argument("--serving-requests", type=int, default=400,
                     help="requests in the serving trace (0 = skip the serving measurement)")
+    ap.add_argument("--workload", default="sir", choices=["sir", "deffuant", "wealth", "schelling"],
+                    help="sir (default): the headline line; the others measure the other "
+                         "programs at their configured sizes")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="run a 256 MiB device copy and one un-captured SIR simulation (for "
+                         "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
               "multi-GPU sharding is a design note, not a measured path.", file=sys.stderr)
         return 2
+    if args.pmc_probe:
+        return pmc_probe()
+    if args.workload != "sir":
+        import torch
+
+        if not torch.cuda.is_available():
+            print("bench.py: no ROCm GPU visible", file=sys.stderr)
+            return 2
+        fn = {"deffuant": bench_deffuant, "wealth": bench_wealth,
+              "schelling": bench_schelling}[args.workload]
+        print(json.dumps(fn(args)))
+        return 0
 
     import numpy as np
     import torch
@@ -267,6 +474,7 @@ def main() -> int:
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
             "kernel": "k_sir_step",
+            "traffic_source": None,
             "algorithmic_bytes_per_launch": algo,
             "avg_kernel_us": avg_kernel_s * 1e6,
             "data_dependent_bytes_per_launch": dd_avg,
@@ -274,6 +482,10 @@ def main() -> int:
         },
         "final_counts": ref_counts[-1].tolist(),
     }
+    traffic = measured_traffic()
+    if traffic is not None:
+        out["roofline"]["traffic"] = traffic["bytes_per_launch"]
+        out["roofline"]["traffic_source"] = traffic["source"]
     if args.cpu_runs > 0:
         out["cpu_baseline"] = cpu_baseline(row_ptr, col, args.cpu_runs)
     if args.serving_requests > 0:
