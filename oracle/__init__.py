@@ -80,6 +80,8 @@ def _declare(L):
         "orc_gini_i32": (C.c_float, [_i32p, _i64]),
         "orc_mean_i32": (C.c_float, [_i32p, _i64]),
         "orc_sir_init": (None, [_i64, _i32, _i64, C.c_int, _u8p, _u32p]),
+        "orc_sir_run_threads": (None, [_i64, _i32p, _i32p, _u8p, _u32p, C.c_float, C.c_float,
+                                       C.c_int32, C.c_int, _i32p, C.c_int]),
         "orc_sir_run": (None, [_i64, _i32p, _i32p, _u8p, _u32p, C.c_float,
                                C.c_float, _i32, C.c_int, _i32p]),
         "orc_wealth_init": (None, [_i64, _i64, C.c_int, _f32p, _f32p, _i32p, _u32p]),
@@ -215,6 +217,16 @@ def sir_run(row_ptr, col, state, rng, beta: float, gamma: float, steps: int,
     counts = np.zeros((max(steps, 1), 3), np.int32)
     lib().orc_sir_run(len(state), row_ptr, np.ascontiguousarray(col, np.int32) if len(col) else np.zeros(1, np.int32),
                       state, rng, beta, gamma, steps, mode, counts.reshape(-1))
+    return counts[:steps]
+
+
+def sir_run_threads(row_ptr, col, state, rng, beta: float, gamma: float, steps: int,
+                    threads: int, mode: int = MODE_PARTITIONABLE):
+    """sir_run with each step's agents on ``threads`` OpenMP threads."""
+    counts = np.zeros((max(steps, 1), 3), np.int32)
+    lib().orc_sir_run_threads(len(state), row_ptr, np.ascontiguousarray(col, np.int32),
+                              state, rng, beta, gamma, steps, mode, counts.reshape(-1),
+                              int(threads))
     return counts[:steps]
 
 

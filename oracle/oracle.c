@@ -16,8 +16,17 @@
  *   programs            -- invariants (conservation, monotone R, one agent
  *                          per cell, sum of opinions) + brute-force step
  *                          re-evaluation in tests/test_oracle_programs.py
- *   Deffuant adjoint    -- central finite differences (h=1e-3) of a
- *                          float64 forward of the same program
+ *   Deffuant adjoint    -- central finite differences of the float64
+ *                          forward of the same program: agreement to 1e-5
+ *                          at h = 1e-6 and monotone convergence from
+ *                          h = 1e-3 (the gate's curvature separates the
+ *                          secant from the tangent at h = 1e-3)
+ *
+ * Parity unpinned (no JAX in this environment to produce vectors): XLA:CPU's
+ * rounding of log1p / sqrt inside erf_inv and of exp in the sigmoid gate
+ * (tolerance-level: float32 results compared at 1e-5 relative), and
+ * end-to-end trajectories under an installed JAX version beyond the
+ * documented key and draw values above.
  */
 #include "oracle.h"
 
@@ -426,9 +435,47 @@ void orc_sir_init(int64_t n, int32_t n_seed_infected, int64_t seed, int mode,
   free(perm);
 }
 
-void orc_sir_run(int64_t n, const int32_t *row_ptr, const int32_t *col,
-                 uint8_t *state, uint32_t rng[2], float beta, float gamma,
-                 int32_t steps, int mode, int32_t *counts) {
+static uint8_t sir_next(int64_t v, int64_t n, int64_t nnz, const int32_t *row_ptr,
+                        const int32_t *col, const uint8_t *old, const uint32_t kinf[2],
+                        const uint32_t krec[2], float beta, float gamma, int mode) {
+  uint8_t s = old[v];
+  if (s == 0) {
+    for (int64_t e = row_ptr[v]; e < row_ptr[v + 1]; ++e) {
+      if (old[col[e]] != 1) continue;
+      uint32_t b;
+      if (mode == ORC_MODE_PARTITIONABLE) {
+        uint32_t o[2];
+        orc_threefry2x32(kinf[0], kinf[1], (uint32_t)((uint64_t)e >> 32),
+                         (uint32_t)e, o);
+        b = o[0] ^ o[1];
+      } else {
+        b = orig_element(kinf, nnz, e);
+      }
+      if (bits_to_unit(b) < beta) return 1;
+    }
+    return 0;
+  }
+  if (s == 1) {
+    uint32_t b;
+    if (mode == ORC_MODE_PARTITIONABLE) {
+      uint32_t o[2];
+      orc_threefry2x32(krec[0], krec[1], (uint32_t)((uint64_t)v >> 32),
+                       (uint32_t)v, o);
+      b = o[0] ^ o[1];
+    } else {
+      b = orig_element(krec, n, v);
+    }
+    if (bits_to_unit(b) < gamma) return 2;
+  }
+  return s;
+}
+
+/* The same run with the agents of each step split over `threads` OpenMP
+ * threads: every agent's next state depends only on the pre-step copy, so
+ * the result does not depend on the thread count. */
+void orc_sir_run_threads(int64_t n, const int32_t *row_ptr, const int32_t *col,
+                         uint8_t *state, uint32_t rng[2], float beta, float gamma,
+                         int32_t steps, int mode, int32_t *counts, int threads) {
   int64_t nnz = row_ptr[n];
   uint8_t *old = (uint8_t *)malloc((size_t)n);
   for (int32_t t = 0; t < steps; ++t) {
@@ -436,46 +483,26 @@ void orc_sir_run(int64_t n, const int32_t *row_ptr, const int32_t *col,
     step_keys(rng, ck, uk, mode);
     split2(uk, kinf, krec, mode);
     memcpy(old, state, (size_t)n);
-    int32_t c[3] = {0, 0, 0};
+    int32_t c0 = 0, c1 = 0, c2 = 0;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 4096) reduction(+ : c0, c1, c2)
     for (int64_t v = 0; v < n; ++v) {
-      uint8_t s = old[v];
-      if (s == 0) {
-        for (int64_t e = row_ptr[v]; e < row_ptr[v + 1]; ++e) {
-          if (old[col[e]] != 1) continue;
-          uint32_t b;
-          if (mode == ORC_MODE_PARTITIONABLE) {
-            uint32_t o[2];
-            orc_threefry2x32(kinf[0], kinf[1], (uint32_t)((uint64_t)e >> 32),
-                             (uint32_t)e, o);
-            b = o[0] ^ o[1];
-          } else {
-            b = orig_element(kinf, nnz, e);
-          }
-          if (bits_to_unit(b) < beta) {
-            s = 1;
-            break;
-          }
-        }
-      } else if (s == 1) {
-        uint32_t b;
-        if (mode == ORC_MODE_PARTITIONABLE) {
-          uint32_t o[2];
-          orc_threefry2x32(krec[0], krec[1], (uint32_t)((uint64_t)v >> 32),
-                           (uint32_t)v, o);
-          b = o[0] ^ o[1];
-        } else {
-          b = orig_element(krec, n, v);
-        }
-        if (bits_to_unit(b) < gamma) s = 2;
-      }
+      const uint8_t s = sir_next(v, n, nnz, row_ptr, col, old, kinf, krec, beta, gamma, mode);
       state[v] = s;
-      c[s]++;
+      c0 += s == 0;
+      c1 += s == 1;
+      c2 += s == 2;
     }
-    counts[3 * t + 0] = c[0];
-    counts[3 * t + 1] = c[1];
-    counts[3 * t + 2] = c[2];
+    counts[3 * t + 0] = c0;
+    counts[3 * t + 1] = c1;
+    counts[3 * t + 2] = c2;
   }
   free(old);
+}
+
+void orc_sir_run(int64_t n, const int32_t *row_ptr, const int32_t *col,
+                 uint8_t *state, uint32_t rng[2], float beta, float gamma,
+                 int32_t steps, int mode, int32_t *counts) {
+  orc_sir_run_threads(n, row_ptr, col, state, rng, beta, gamma, steps, mode, counts, 1);
 }
 
 /* ------------------------------------------------------------------------ */

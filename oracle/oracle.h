@@ -65,6 +65,9 @@ void orc_sir_init(int64_t n, int32_t n_seed_infected, int64_t seed, int mode,
 void orc_sir_run(int64_t n, const int32_t *row_ptr, const int32_t *col,
                  uint8_t *state, uint32_t rng[2], float beta, float gamma,
                  int32_t steps, int mode, int32_t *counts);
+void orc_sir_run_threads(int64_t n, const int32_t *row_ptr, const int32_t *col,
+                         uint8_t *state, uint32_t rng[2], float beta, float gamma,
+                         int32_t steps, int mode, int32_t *counts, int threads);
 
 void orc_wealth_init(int64_t n, int64_t seed, int mode, float *x, float *y,
                      int32_t *wealth, uint32_t rng[2]);

@@ -275,3 +275,17 @@ def test_deffuant_adjoint_matches_finite_differences():
     l0 = O.deffuant_grad(x0, np.array([0.2]), np.array([0.3]), rng0, steps, target,
                          O.MODE_PARTITIONABLE)[0]
     assert l0[0] < 1e-10
+
+
+def test_sir_threaded_run_is_identical():
+    """The all-cores CPU baseline computes the same simulation: each agent's
+    next state depends only on the pre-step copy."""
+    rp, col = O.ba_csr(20000, 4, 3)
+    st1, r1 = O.sir_init(20000, 50, 2, O.MODE_PARTITIONABLE)
+    st2, r2 = st1.copy(), r1.copy()
+    c1 = O.sir_run(rp, col, st1, r1, np.float32(0.07), np.float32(0.1), 60, O.MODE_PARTITIONABLE)
+    c2 = O.sir_run_threads(rp, col, st2, r2, np.float32(0.07), np.float32(0.1), 60, 4,
+                           O.MODE_PARTITIONABLE)
+    np.testing.assert_array_equal(c1, c2)
+    np.testing.assert_array_equal(st1, st2)
+    np.testing.assert_array_equal(r1, r2)
