@@ -144,7 +144,8 @@ int abm_sir_chunk_params(int32_t *align, int64_t *node_cost, int32_t *heavy_deg,
                          int32_t *max_nodes);
 /* Words of one infected-set bitmap (a multiple of 4). */
 int64_t abm_sir_bitmap_words(int64_t n);
-/* Count slots per step (one step's `slots` block holds slots*3 ints). */
+/* Slots per step: one step's `slots` block holds slots*3 ints (count slots
+ * plus one flag slot; callers zero it and never read it directly). */
 int abm_sir_slots(void);
 /* Initial state at step t0: all S with heavy-row flags, agents perm[0..n0)
  * infected, marks and bitmap (buffer 0) built from it. */

@@ -66,7 +66,13 @@ constexpr int64_t kChunkNodeCost = 4;
 // round trips per row whatever its degree
 constexpr int64_t kChunkHeavyCost = 128;
 constexpr int kChunksPerBlock = 1;
-constexpr int kSlots = 64;
+constexpr int kSlots = 64;                 // count slots per step
+constexpr int kStepSlots = kSlots + 1;     // + one flag slot
+constexpr int kStride = kStepSlots * 3;    // ints per step in the slots array
+// slots[t][kSlots][1] = 1 when step t left an infected agent (plain stores
+// of the same value by any block that has one): the next step reads this one
+// word to decide whether it is the identity
+constexpr int kLiveFlag = kSlots * 3 + 1;
 constexpr uint8_t kHeavyBit = 4;
 constexpr uint32_t kMarkHorizon = 8;
 // a tile stages the bitmap in LDS when it has at least this many rows to scan
@@ -252,16 +258,10 @@ __global__ void __launch_bounds__(kSirThreads) k_sir_step(SirStep a) {
   __shared__ int32_t cnt[3];
   const int tid = threadIdx.x, lane = tid & 63;
   if (tid < 3) cnt[tid] = 0;
-  // loads of the first chunk bounds and of the previous step's infected count
-  // are independent; issue both before the block-wide decision
+  // the first chunk's bounds and the previous step's live flag (one
+  // uniform word: no block-wide reduction or barrier) load together
   int64_t c0 = a.chunk_ptr[blockIdx.x], c1 = a.chunk_ptr[blockIdx.x + 1];
-  int32_t prev_i = 1;
-  if (a.prev_slots) {
-    prev_i = tid < kSlots ? a.prev_slots[tid * 3 + 1] : 0;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) prev_i += __shfl_xor(prev_i, off);
-  }
-  const bool any_inf = __syncthreads_or(prev_i != 0);
+  const bool any_inf = a.prev_slots ? a.prev_slots[kLiveFlag] != 0 : true;
   if (!any_inf) {
     // identity step: clear this block's spans of the next bitmap, copy counts
     for (int32_t c = blockIdx.x; c < a.n_chunks; c += gridDim.x) {
@@ -269,6 +269,7 @@ __global__ void __launch_bounds__(kSirThreads) k_sir_step(SirStep a) {
       const int64_t w1 = c + 1 == a.n_chunks ? a.words : ((int64_t)a.chunk_ptr[c + 1] >> 5);
       for (int64_t w = w0 + tid; w < w1; w += kSirThreads) a.bm_out[w] = 0u;
     }
+    // counts carry over; the live flag stays 0
     if (blockIdx.x == 0 && tid < kSlots * 3) a.slots[tid] = a.prev_slots[tid];
     return;
   }
@@ -419,6 +420,7 @@ __global__ void __launch_bounds__(kSirThreads) k_sir_step(SirStep a) {
   }
   lds_barrier();
   if (tid < 3 && cnt[tid]) atomicAdd(&a.slots[(blockIdx.x % kSlots) * 3 + tid], cnt[tid]);
+  if (tid == 1 && cnt[1]) a.slots[kLiveFlag] = 1;
 }
 
 // counts[t][k] = sum over slots of slots[t][s][k]
@@ -427,7 +429,7 @@ __global__ void k_sir_reduce_slots(const int32_t *__restrict__ slots,
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= steps * 3) return;
   const int64_t t = i / 3, k = i - t * 3;
-  const int32_t *p = slots + t * kSlots * 3 + k;
+  const int32_t *p = slots + t * kStride + k;
   int32_t s = 0;
   for (int j = 0; j < kSlots; ++j) s += p[j * 3];
   counts[i] = s;
@@ -573,7 +575,7 @@ int abm_fill_i32(int32_t *p, int64_t n, int32_t v, void *stream) {
 
 uint32_t abm_unit_threshold(float p) { return unit_threshold(p); }
 
-int abm_sir_slots(void) { return kSlots; }
+int abm_sir_slots(void) { return kStepSlots; }
 
 int64_t abm_sir_bitmap_words(int64_t n) { return n > 0 ? bitmap_words(n) : 0; }
 
@@ -630,15 +632,15 @@ int abm_sir_run(const abm_sir_graph *g, uint8_t *state, uint16_t *mark,
   hipStream_t s = (hipStream_t)stream;
   const uint32_t tb = unit_threshold(beta), tg = unit_threshold(gamma);
   const int64_t words = bitmap_words(g->n);
-  const int64_t ns = (int64_t)steps * kSlots * 3;
+  const int64_t ns = (int64_t)steps * kStride;
   hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(ns, 256)), dim3(256), 0, s,
                      slots, ns, 0);
   uint32_t *bm[2] = {bitmaps, bitmaps + words};
   for (int32_t t = 0; t < steps; ++t)
     if (int rc = launch_step(g, state, mark, bm[t & 1], bm[(t + 1) & 1],
                              t0 + (uint32_t)t, keys + 4 * (int64_t)t, tb, tg, mode,
-                             t ? slots + (int64_t)(t - 1) * kSlots * 3 : nullptr,
-                             slots + (int64_t)t * kSlots * 3, s))
+                             t ? slots + (int64_t)(t - 1) * kStride : nullptr,
+                             slots + (int64_t)t * kStride, s))
       return rc;
   hipLaunchKernelGGL(k_sir_reduce_slots, dim3(blocks_for(3 * (int64_t)steps, 256)),
                      dim3(256), 0, s, slots, (int64_t)steps, counts);
