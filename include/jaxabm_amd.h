@@ -164,7 +164,11 @@ int abm_sir_run(const abm_sir_graph *g, uint8_t *state, uint16_t *mark, uint32_t
 int abm_sir_step(const abm_sir_graph *g, uint8_t *state, uint16_t *mark, const uint32_t *bm_in,
                  uint32_t *bm_out, uint32_t t, const uint32_t *keys4, float beta, float gamma,
                  int mode, const int32_t *prev_slots, int32_t *slots, void *stream);
-int abm_sir_reduce_counts(const int32_t *slots, int32_t steps, int32_t *counts, void *stream);
+/* counts[t] = sum of the count slots of step t, for the slots of consecutive
+ * steps of one run whose first step was computed (prev_slots NULL or live).
+ * A step that found no infected agent leaves its slots zero and gets the
+ * previous step's counts.  Writes a mark word into each step's flag slot. */
+int abm_sir_reduce_counts(int32_t *slots, int32_t steps, int32_t *counts, void *stream);
 /* out[i] = public S/I/R code of state[i]. */
 int abm_sir_states(const uint8_t *state, int64_t n, uint8_t *out, void *stream);
 /* ceil(p * 2^23): integer form of `uniform < p`. */

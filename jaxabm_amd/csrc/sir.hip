@@ -91,6 +91,7 @@ struct SirStep {
   int64_t words;
   uint32_t t;
   const int32_t *prev_slots;  // previous step's count slots, or null
+  const int32_t *prev2_slots; // the step before that, or null
   const uint32_t *keys;
   uint32_t thr_beta, thr_gamma;
   int mode;
@@ -263,14 +264,17 @@ __global__ void __launch_bounds__(kSirThreads) k_sir_step(SirStep a) {
   int64_t c0 = a.chunk_ptr[blockIdx.x], c1 = a.chunk_ptr[blockIdx.x + 1];
   const bool any_inf = a.prev_slots ? a.prev_slots[kLiveFlag] != 0 : true;
   if (!any_inf) {
-    // identity step: clear this block's spans of the next bitmap, copy counts
-    for (int32_t c = blockIdx.x; c < a.n_chunks; c += gridDim.x) {
-      const int64_t w0 = (int64_t)a.chunk_ptr[c] >> 5;
-      const int64_t w1 = c + 1 == a.n_chunks ? a.words : ((int64_t)a.chunk_ptr[c + 1] >> 5);
-      for (int64_t w = w0 + tid; w < w1; w += kSirThreads) a.bm_out[w] = 0u;
+    // identity step.  Its slots stay zero (the count reduction carries the
+    // previous step's counts forward).  The next bitmap B[(t+1)&1] is the
+    // infected set after step t-2: empty when step t-2 left no infected agent,
+    // so only the first identity step clears its spans.
+    if (!(a.prev2_slots && a.prev2_slots[kLiveFlag] == 0)) {
+      for (int32_t c = blockIdx.x; c < a.n_chunks; c += gridDim.x) {
+        const int64_t w0 = (int64_t)a.chunk_ptr[c] >> 5;
+        const int64_t w1 = c + 1 == a.n_chunks ? a.words : ((int64_t)a.chunk_ptr[c + 1] >> 5);
+        for (int64_t w = w0 + tid; w < w1; w += kSirThreads) a.bm_out[w] = 0u;
+      }
     }
-    // counts carry over; the live flag stays 0
-    if (blockIdx.x == 0 && tid < kSlots * 3) a.slots[tid] = a.prev_slots[tid];
     return;
   }
   bool staged = false;
@@ -423,9 +427,13 @@ __global__ void __launch_bounds__(kSirThreads) k_sir_step(SirStep a) {
   if (tid == 1 && cnt[1]) a.slots[kLiveFlag] = 1;
 }
 
-// counts[t][k] = sum over slots of slots[t][s][k]
-__global__ void k_sir_reduce_slots(const int32_t *__restrict__ slots,
-                                   int64_t steps, int32_t *__restrict__ counts) {
+// counts[t][k] = sum over the count slots of slots[t][s][k]; an identity
+// step leaves its slots zero (S + I + R = n > 0 for any computed step), which
+// this pass records in the flag slot's spare word for the carry pass
+constexpr int kComputedMark = kSlots * 3;
+
+__global__ void k_sir_reduce_slots(int32_t *__restrict__ slots, int64_t steps,
+                                   int32_t *__restrict__ counts) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= steps * 3) return;
   const int64_t t = i / 3, k = i - t * 3;
@@ -433,6 +441,31 @@ __global__ void k_sir_reduce_slots(const int32_t *__restrict__ slots,
   int32_t s = 0;
   for (int j = 0; j < kSlots; ++j) s += p[j * 3];
   counts[i] = s;
+  if (k == 0) {
+    int32_t tot = 0;
+    for (int j = 0; j < kSlots * 3; ++j) tot |= slots[t * kStride + j];
+    slots[t * kStride + kComputedMark] = tot != 0;
+  }
+}
+
+// Identity steps form a suffix of a run (no infected agent, no new
+// infection): counts[t] = counts[t_a - 1] for t >= t_a, the first identity
+// step, found by binary search over the computed marks.
+__global__ void k_sir_carry_counts(const int32_t *__restrict__ slots, int64_t steps,
+                                   int32_t *__restrict__ counts) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= steps * 3) return;
+  const int64_t t = i / 3, k = i - t * 3;
+  if (slots[t * kStride + kComputedMark]) return;
+  int64_t lo = 0, hi = t;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (slots[mid * kStride + kComputedMark])
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  counts[i] = lo > 0 ? counts[(lo - 1) * 3 + k] : 0;
 }
 
 __global__ void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v) {
@@ -521,7 +554,8 @@ int check_buffers(const uint8_t *state, const uint16_t *mark, const uint32_t *b0
 int launch_step(const abm_sir_graph *g, uint8_t *state, uint16_t *mark,
                 const uint32_t *bm_in, uint32_t *bm_out, uint32_t t,
                 const uint32_t *keys4, uint32_t tb, uint32_t tg, int mode,
-                const int32_t *prev_slots, int32_t *slots, hipStream_t s) {
+                const int32_t *prev_slots, const int32_t *prev2_slots, int32_t *slots,
+                hipStream_t s) {
   static bool lds_attr_set = false;
   SirStep a;
   a.n = g->n;
@@ -537,6 +571,7 @@ int launch_step(const abm_sir_graph *g, uint8_t *state, uint16_t *mark,
   a.words = bitmap_words(g->n);
   a.t = t;
   a.prev_slots = prev_slots;
+  a.prev2_slots = prev2_slots;
   a.keys = keys4;
   a.thr_beta = tb;
   a.thr_gamma = tg;
@@ -640,10 +675,13 @@ int abm_sir_run(const abm_sir_graph *g, uint8_t *state, uint16_t *mark,
     if (int rc = launch_step(g, state, mark, bm[t & 1], bm[(t + 1) & 1],
                              t0 + (uint32_t)t, keys + 4 * (int64_t)t, tb, tg, mode,
                              t ? slots + (int64_t)(t - 1) * kStride : nullptr,
+                             t > 1 ? slots + (int64_t)(t - 2) * kStride : nullptr,
                              slots + (int64_t)t * kStride, s))
       return rc;
   hipLaunchKernelGGL(k_sir_reduce_slots, dim3(blocks_for(3 * (int64_t)steps, 256)),
                      dim3(256), 0, s, slots, (int64_t)steps, counts);
+  hipLaunchKernelGGL(k_sir_carry_counts, dim3(blocks_for(3 * (int64_t)steps, 256)),
+                     dim3(256), 0, s, (const int32_t *)slots, (int64_t)steps, counts);
   if (steps & 1)
     ABM_CHECK_HIP(hipMemcpyAsync(bm[0], bm[1], (size_t)words * 4,
                                  hipMemcpyDeviceToDevice, s));
@@ -661,20 +699,22 @@ int abm_sir_step(const abm_sir_graph *g, uint8_t *state, uint16_t *mark,
               "abm_sir_step: bad argument");
   if (int rc = check_buffers(state, mark, bm_in, bm_out, "abm_sir_step")) return rc;
   if (int rc = launch_step(g, state, mark, bm_in, bm_out, t, keys4, unit_threshold(beta),
-                           unit_threshold(gamma), mode, prev_slots, slots,
+                           unit_threshold(gamma), mode, prev_slots, nullptr, slots,
                            (hipStream_t)stream))
     return rc;
   ABM_LAUNCH_CHECK("abm_sir_step");
   return kOk;
 }
 
-int abm_sir_reduce_counts(const int32_t *slots, int32_t steps, int32_t *counts,
-                          void *stream) {
+int abm_sir_reduce_counts(int32_t *slots, int32_t steps, int32_t *counts, void *stream) {
   ABM_REQUIRE(slots && counts && steps >= 0, "abm_sir_reduce_counts: bad args");
   if (steps == 0) return kOk;
   hipLaunchKernelGGL(k_sir_reduce_slots, dim3(blocks_for(3 * (int64_t)steps, 256)),
                      dim3(256), 0, (hipStream_t)stream, slots, (int64_t)steps,
                      counts);
+  hipLaunchKernelGGL(k_sir_carry_counts, dim3(blocks_for(3 * (int64_t)steps, 256)),
+                     dim3(256), 0, (hipStream_t)stream, (const int32_t *)slots,
+                     (int64_t)steps, counts);
   ABM_LAUNCH_CHECK("abm_sir_reduce_counts");
   return kOk;
 }
