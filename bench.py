@@ -46,6 +46,7 @@ SEED = 0
 GRAPH_SEED = 0
 HBM_PEAK_GBS = 8000.0
 ALGO_BYTES_PER_AGENT_STEP = 46.0
+MARK_HORIZON = 16  # the step kernel's mark horizon (steps a mark stays valid)
 
 
 def cpu_model() -> str:
@@ -438,14 +439,14 @@ def main() -> int:
         for t in range(SIM_STEPS):
             old = eng.state & 3
             s_mask = old == 0
-            risk = s_mask & ((((eng.mark.to(torch.int32) - t) & 0xFFFF)) <= 8)
+            risk = s_mask & ((((eng.mark.to(torch.int32) - t) & 0xFFFF)) <= MARK_HORIZON)
             check(lib().abm_sir_step(C.byref(eng._g), ptr(eng.state), ptr(eng.mark),
                                      ptr(eng.bitmaps[t & 1]), ptr(eng.bitmaps[(t + 1) & 1]), t,
                                      ptr(keys[t]), eng.beta, eng.gamma, eng.mode,
                                      ptr(sl1[t - 1]) if t else None, ptr(sl1[t]), sh),
                   "sir_step")
             new = eng.state & 3
-            marking = (new == 1) & ((old == 0) | (((t + idx) % 8) == 0))
+            marking = (new == 1) & ((old == 0) | (((t + idx) % MARK_HORIZON) == 0))
             # bytes the kernel touches (upper bound): state read, changed-state
             # writes, marks of S agents, the bitmap staged once per XCD-resident
             # copy and written once, row_ptr + col of scanned rows, and row_ptr

@@ -33,7 +33,7 @@ int abm_graph_end(void *stream, abm_graph **out) {
   hipGraphExec_t ex = nullptr;
   hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
   if (e != hipSuccess) {
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);  // the instantiation error is the one reported
     return hip_fail(e, "hipGraphInstantiate");
   }
   abm_graph *h = new abm_graph{g, ex};
@@ -49,9 +49,14 @@ int abm_graph_launch(abm_graph *g, void *stream) {
 
 int abm_graph_destroy(abm_graph *g) {
   if (!g) return kOk;
-  if (g->exec) hipGraphExecDestroy(g->exec);
-  if (g->graph) hipGraphDestroy(g->graph);
+  hipError_t e = hipSuccess;
+  if (g->exec) e = hipGraphExecDestroy(g->exec);
+  if (g->graph) {
+    const hipError_t e2 = hipGraphDestroy(g->graph);
+    if (e == hipSuccess) e = e2;
+  }
   delete g;
+  if (e != hipSuccess) return hip_fail(e, "abm_graph_destroy");
   return kOk;
 }
 

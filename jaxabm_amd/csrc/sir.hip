@@ -30,8 +30,8 @@
 //    at the start of step t, u wrote a mark in some step s in [t-H, t-1] (its
 //    infection step or its refresh phase) with value s + H in [t, t+H-1], and
 //    later writes only store values in [t, t+H], so v is never skipped; stale
-//    marks only cost a scan.  H = 8 with per-agent refresh phases cuts the
-//    mark stores of long-lived infections by 8x without step-to-step spikes.
+//    marks only cost a scan.  H = 16 with per-agent refresh phases cuts the
+//    mark stores of long-lived infections by 16x without step-to-step spikes.
 //  * Only its owner reads an agent's state byte, so state is updated in place
 //    (an agent octet is written only when it changes); the bitmap is double-buffered (read
 //    B[t&1], write B[(t+1)&1]) and each bitmap word is written whole by the
@@ -55,7 +55,6 @@
 namespace abm {
 
 constexpr int kSirThreads = 1024;
-constexpr int kSirWaves = kSirThreads / 64;
 constexpr int kPerThread = 8;
 constexpr int kTile = kSirThreads * kPerThread;
 constexpr int kSirBlocks = 256;
@@ -74,7 +73,7 @@ constexpr int kStride = kStepSlots * 3;    // ints per step in the slots array
 // word to decide whether it is the identity
 constexpr int kLiveFlag = kSlots * 3 + 1;
 constexpr uint8_t kHeavyBit = 4;
-constexpr uint32_t kMarkHorizon = 8;
+constexpr uint32_t kMarkHorizon = 16;
 // a tile stages the bitmap in LDS when it has at least this many rows to scan
 constexpr int32_t kStageItems = 512;
 // LDS bitmap budget: 32768 words (1,048,576 agents, 128 KB)
