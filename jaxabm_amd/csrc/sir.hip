@@ -125,11 +125,15 @@ __device__ __forceinline__ bool scan_light(const int32_t *__restrict__ col,
     int32_t c[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) c[k] = (e + k < hi) ? col[e + k] : -1;
+    uint32_t cm = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-      if (c[k] >= 0 && infected(bm, c[k]) &&
-          (bits_at(kinf, nnz, e + k, mode) >> 9) < thr)
-        return true;
+      if (c[k] >= 0 && infected(bm, c[k])) cm |= 1u << k;
+    while (cm) {
+      const int k = __builtin_ctz(cm);
+      cm &= cm - 1;
+      if ((bits_at(kinf, nnz, e + k, mode) >> 9) < thr) return true;
+    }
   }
   return false;
 }
@@ -148,11 +152,16 @@ __device__ __forceinline__ bool scan_heavy(const int32_t *__restrict__ col,
       const int64_t e = b + k * 64 + lane;
       c[k] = e < hi ? col[e] : -1;
     }
-    bool hit = false;
+    uint32_t cm = 0;
 #pragma unroll
     for (int k = 0; k < 8; ++k)
-      if (c[k] >= 0 && infected(bm, c[k]))
-        hit |= (bits_at(kinf, nnz, b + k * 64 + lane, mode) >> 9) < thr;
+      if (c[k] >= 0 && infected(bm, c[k])) cm |= 1u << k;
+    bool hit = false;
+    while (cm && !hit) {
+      const int k = __builtin_ctz(cm);
+      cm &= cm - 1;
+      hit = (bits_at(kinf, nnz, b + k * 64 + lane, mode) >> 9) < thr;
+    }
     if (__ballot(hit)) return true;
   }
   return false;
@@ -224,6 +233,7 @@ __device__ __forceinline__ void push_items(uint32_t lmask, uint32_t hmask, uint1
 
 // Phase-2 scans of a tile's worklist: heavy rows (back of the list) by
 // wavefronts, light rows (front) by lanes; a hit marks the agent infected.
+template <int kMode>
 __device__ __forceinline__ void scan_items(const SirStep &a, const uint32_t *__restrict__ bm,
                                            int64_t sub, int32_t nl, int32_t nh,
                                            const uint16_t *items, uint8_t *outb, Key kinf,
@@ -237,19 +247,19 @@ __device__ __forceinline__ void scan_items(const SirStep &a, const uint32_t *__r
     const int loc = items[kTile - 1 - k];
     const int64_t u = sub + loc;
     const bool hit = scan_heavy(a.col, bm, a.row_ptr[u], a.row_ptr[u + 1], kinf, a.nnz,
-                                a.thr_beta, a.mode);
+                                a.thr_beta, kMode);
     if (lane == 0 && hit) outb[loc] = 1;
   }
   for (int32_t k = tid; k < nl; k += kSirThreads) {
     const int loc = items[k];
     const int64_t u = sub + loc;
     if (scan_light(a.col, bm, a.row_ptr[u], a.row_ptr[u + 1], kinf, a.nnz, a.thr_beta,
-                   a.mode))
+                   kMode))
       outb[loc] = 1;
   }
 }
 
-template <bool kLds>
+template <bool kLds, int kMode>
 __global__ void __launch_bounds__(kSirThreads) k_sir_step(SirStep a) {
   extern __shared__ uint4 lds_bm4[];  // kLds: the bitmap words
   __shared__ __align__(16) uint8_t outb[kTile];
@@ -314,18 +324,22 @@ __global__ void __launch_bounds__(kSirThreads) k_sir_step(SirStep a) {
           mk[j] = j < nv ? a.mark[v0 + j] : (uint16_t)(a.t + 0x8000u);
         }
       }
-      uint32_t lmask = 0, hmask = 0;
+      uint32_t lmask = 0, hmask = 0, imask = 0;
       uint32_t ob[2] = {0u, 0u};
 #pragma unroll
       for (int j = 0; j < kPerThread; ++j) {
         const uint8_t s = b[j] & 3;
-        uint8_t o = s;
-        if (s == 1 && (bits_at(krec, a.n, v0 + j, a.mode) >> 9) < a.thr_gamma) o = 2;
-        ob[j >> 2] |= (uint32_t)o << (8 * (j & 3));
+        ob[j >> 2] |= (uint32_t)s << (8 * (j & 3));
+        if (s == 1) imask |= 1u << j;
         if (s == 0 && at_risk(mk[j], a.t)) {
           if (b[j] & kHeavyBit) hmask |= 1u << j;
           else lmask |= 1u << j;
         }
+      }
+      for (uint32_t m = imask; m; m &= m - 1) {
+        const int j = __builtin_ctz(m);
+        if ((bits_at(krec, a.n, v0 + j, kMode) >> 9) < a.thr_gamma)
+          ob[j >> 2] ^= 3u << (8 * (j & 3));  // I (1) -> R (2)
       }
       *reinterpret_cast<uint2 *>(outb + tid * kPerThread) = make_uint2(ob[0], ob[1]);
       lds_barrier();  // counters reset before any push
@@ -344,10 +358,10 @@ __global__ void __launch_bounds__(kSirThreads) k_sir_step(SirStep a) {
           }
         }
         if (kLds && staged)
-          scan_items(a, reinterpret_cast<const uint32_t *>(lds_bm4), sub, nl, nh, items,
-                     outb, kinf, &next_heavy);
+          scan_items<kMode>(a, reinterpret_cast<const uint32_t *>(lds_bm4), sub, nl, nh,
+                            items, outb, kinf, &next_heavy);
         else
-          scan_items(a, a.bm_in, sub, nl, nh, items, outb, kinf, &next_heavy);
+          scan_items<kMode>(a, a.bm_in, sub, nl, nh, items, outb, kinf, &next_heavy);
       }
       lds_barrier();
       // phase 3: outputs (one 8-byte state store when anything changed), the
@@ -577,17 +591,23 @@ int launch_step(const abm_sir_graph *g, uint8_t *state, uint16_t *mark,
   a.mode = mode;
   a.slots = slots;
   const unsigned blocks = (unsigned)(g->n_chunks < kSirBlocks ? g->n_chunks : kSirBlocks);
+  // one instantiation per stream layout: each carries one copy of the draw
+  const bool part = a.mode == kModePartitionable;
   if (a.words <= kLdsBitmapWords) {
+    auto *kp = k_sir_step<true, kModePartitionable>;
+    auto *ko = k_sir_step<true, kModeOriginal>;
     if (!lds_attr_set) {
-      ABM_CHECK_HIP(hipFuncSetAttribute((const void *)k_sir_step<true>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)(kLdsBitmapWords * 4)));
+      for (auto *k : {kp, ko})
+        ABM_CHECK_HIP(hipFuncSetAttribute((const void *)k,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)(kLdsBitmapWords * 4)));
       lds_attr_set = true;
     }
-    hipLaunchKernelGGL(k_sir_step<true>, dim3(blocks), dim3(kSirThreads),
-                       (size_t)a.words * 4, s, a);
+    hipLaunchKernelGGL(part ? kp : ko, dim3(blocks), dim3(kSirThreads), (size_t)a.words * 4,
+                       s, a);
   } else {
-    hipLaunchKernelGGL(k_sir_step<false>, dim3(blocks), dim3(kSirThreads), 0, s, a);
+    auto *kg = part ? k_sir_step<false, kModePartitionable> : k_sir_step<false, kModeOriginal>;
+    hipLaunchKernelGGL(kg, dim3(blocks), dim3(kSirThreads), 0, s, a);
   }
   return kOk;
 }

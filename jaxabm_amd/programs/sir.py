@@ -33,7 +33,10 @@ class SIRNetworkEngine:
     def __init__(self, n: int, m: int = 4, graph_seed: int = 0, beta: float = 0.05,
                  gamma: float = 0.1, n_seed_infected: int = 100, seed: int = 0,
                  mode: int | None = None, csr: tuple[np.ndarray, np.ndarray] | None = None,
-                 device: str = "cuda"):
+                 device: str = "cuda", chunk_cost: int | None = None):
+        """``chunk_cost`` overrides the cost target of the agent chunks (one
+        per block by default); a large value gives chunks of the maximum tile
+        size, whatever the graph size."""
         require_gpu()
         self.n = int(n)
         self.beta = float(np.float32(beta))
@@ -53,6 +56,8 @@ class SIRNetworkEngine:
         total = (node_cost.value * self.n + int(row_ptr[-1])
                  + heavy_cost.value * int((deg > heavy_deg.value).sum()))
         target = max(1, -(-total // (blocks.value * per_block.value)))
+        if chunk_cost is not None:
+            target = max(1, int(chunk_cost))
         cp = node_chunks(row_ptr, align.value, node_cost.value, target, max_nodes.value,
                          heavy_deg.value, heavy_cost.value)
         self.chunks = torch.from_numpy(cp).to(self.device)

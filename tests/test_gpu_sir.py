@@ -45,6 +45,31 @@ def test_sir_matches_oracle(sir, mode, n, m, steps, beta, gamma, seed):
 
 
 @pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("n,m,steps,beta,gamma,seed", [
+    # chunks of 8192 agents (the maximum tile): tiles with many rows to scan
+    # take the CSR sweep (one and two rounds), tiles of more than 65535 CSR
+    # entries take worklists with row_ptr from global memory, and worklist
+    # overflow in those makes each thread scan its own rows
+    (100_000, 4, 40, 0.3, 0.1, 11),
+    (40_000, 2, 40, 0.5, 0.05, 12),
+    (20_000, 4, 60, 0.1, 0.1, 14),
+    (3000, 80, 20, 0.02, 0.2, 13),   # one tile of 467k entries, all rows heavy
+])
+def test_sir_tile_paths_match_oracle(sir, mode, n, m, steps, beta, gamma, seed):
+    row_ptr, col = O.ba_csr(n, m, graph_seed=seed + 100)
+    n0 = max(10, n // 50)
+    eng = sir.SIRNetworkEngine(n, csr=(row_ptr, col), beta=beta, gamma=gamma,
+                               n_seed_infected=n0, seed=seed, mode=mode, chunk_cost=1 << 40)
+    assert eng.n_chunks == -(-n // 8192)
+    st, rng = O.sir_init(n, n0, seed, mode)
+    want = O.sir_run(row_ptr, col, st, rng, beta, gamma, steps, mode)
+    got = eng.run(steps)
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(eng.state_host(), st)
+    assert want[:, 1].max() > n // 4   # the epidemic reaches most tiles
+
+
+@pytest.mark.parametrize("mode", MODES)
 def test_sir_two_runs_continue(sir, mode):
     """A second run() continues from the current state (jaxabm/model.py:227-235)."""
     n = 3000
