@@ -230,7 +230,7 @@ def bench_wealth(args) -> dict:
     def run():
         check(lib().abm_wealth_init(ptr(ik), n, ptr(eng.x), ptr(eng.y), ptr(eng.w), eng.mode,
                                     stream_handle()), "init")
-        check(lib().abm_wealth_run_batch(ptr(desc), 1, eng.mode, stream_handle()), "run")
+        check(lib().abm_wealth_run_batch(ptr(desc), 1, eng.n, eng.mode, stream_handle()), "run")
 
     for _ in range(args.warmup):
         run()

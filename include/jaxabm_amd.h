@@ -192,8 +192,12 @@ typedef struct abm_wealth_inst {
 int abm_wealth_init(const uint32_t *init_key, int64_t n, float *x, float *y, int32_t *w,
                     int mode, void *stream);
 /* Runs every instance of the device array insts[count] for its steps, one
- * workgroup per instance. */
-int abm_wealth_run_batch(const abm_wealth_inst *insts, int32_t count, int mode, void *stream);
+ * workgroup per instance.  max_n: the largest n in the table (0 if unknown).
+ * With 0 < max_n <= 8000 every instance runs out of LDS (20 bytes per agent
+ * of dynamic LDS per workgroup); otherwise, and for an instance above max_n,
+ * on its global buffers.  Same results either way. */
+int abm_wealth_run_batch(const abm_wealth_inst *insts, int32_t count, int64_t max_n, int mode,
+                         void *stream);
 
 /* ---- Schelling program on a periodic Grid (jaxabm/agentpy.py:465-527
  *      rebuilt with a Moore-8 step) ---- */

@@ -17,7 +17,11 @@
 // so S = sum_w w * ranksum_w (mod 2^32, the int32 wrap of the reference).
 // Whole run in one launch: every phase boundary is a workgroup barrier.
 // Scatter-adds are integer atomics (exact, order-free); the histogram and
-// increments are cleared by the kernel after use.
+// increments are cleared by the kernel after use.  An instance of up to
+// kWealthLdsAgents agents runs out of LDS (x, y, wealth, increments and
+// histogram, 20 bytes per agent, loaded at the start and x, y, wealth written
+// back at the end), so a step costs LDS latencies instead of global memory
+// round trips; larger instances run on their global buffers.
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
@@ -26,6 +30,9 @@
 namespace abm {
 
 constexpr int kWealthThreads = 1024;
+// LDS per resident agent: x, y, wealth, increment, histogram bin
+constexpr int64_t kWealthLdsBytesPerAgent = 20;
+constexpr int64_t kWealthLdsAgents = 8000;  // 160,020 bytes with the extra bin
 
 __device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t *red) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -43,13 +50,13 @@ __device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t *red) {
   return s;  // valid in wave 0
 }
 
-__global__ void __launch_bounds__(kWealthThreads)
-    k_wealth_run(const abm_wealth_inst *__restrict__ insts, int mode) {
-  const abm_wealth_inst in = insts[blockIdx.x];
+// The whole run of one instance on the arrays x, y, w (state), inc and hist
+// (zero on entry, left zero) -- global buffers or their LDS copies.
+__device__ __forceinline__ void wealth_steps(const abm_wealth_inst &in, float *gx, float *gy,
+                                             int32_t *gw, int32_t *ginc, int32_t *ghist,
+                                             uint32_t *red, uint32_t &carry, int mode) {
   const int64_t n = in.n;
   const int tid = threadIdx.x;
-  __shared__ uint32_t red[16];
-  __shared__ uint32_t carry;
   const uint32_t span = n > 1 ? (uint32_t)(n - 1) : 1u;
   for (int32_t t = 0; t < in.steps; ++t) {
     const Key ck{in.keys[4 * t + 0], in.keys[4 * t + 1]};
@@ -62,25 +69,25 @@ __global__ void __launch_bounds__(kWealthThreads)
       split2(split_at(ck, n, i, mode), k1, k2, mode);
       const float dx = normal_from_bits(bits_scalar(k1, mode));
       const float dy = normal_from_bits(bits_scalar(k2, mode));
-      const float sx = in.x[i] + 0.01f * dx;
-      const float sy = in.y[i] + 0.01f * dy;
-      in.x[i] = mod1(sx);
-      in.y[i] = mod1(sy);
-      if (in.w[i] > 0) {
+      const float sx = gx[i] + 0.01f * dx;
+      const float sy = gy[i] + 0.01f * dy;
+      gx[i] = mod1(sx);
+      gy[i] = mod1(sy);
+      if (gw[i] > 0) {
         uint32_t ti = randint_offset(bits_at(kk1, n, i, mode), bits_at(kk2, n, i, mode), span);
         ti += ti >= (uint32_t)i ? 1u : 0u;
-        atomicAdd(&in.inc[ti], 1);
+        atomicAdd(&ginc[ti], 1);
       }
     }
     __syncthreads();
     // apply the exchange, histogram of wealth values
     uint32_t total = 0;
     for (int64_t i = tid; i < n; i += kWealthThreads) {
-      const int32_t w0 = in.w[i];
-      const int32_t wi = w0 - (w0 > 0 ? 1 : 0) + in.inc[i];
-      in.w[i] = wi;
-      in.inc[i] = 0;
-      atomicAdd(&in.hist[wi], 1);  // 0 <= wi <= n: wealth is conserved
+      const int32_t w0 = gw[i];
+      const int32_t wi = w0 - (w0 > 0 ? 1 : 0) + ginc[i];
+      gw[i] = wi;
+      ginc[i] = 0;
+      atomicAdd(&ghist[wi], 1);  // 0 <= wi <= n: wealth is conserved
       total += (uint32_t)wi;
     }
     const uint32_t T = block_sum_u32(total, red);
@@ -92,7 +99,7 @@ __global__ void __launch_bounds__(kWealthThreads)
     uint32_t S_part = 0;
     for (int64_t base = 0; base <= n; base += kWealthThreads) {
       const int64_t w = base + tid;
-      const uint32_t c = w <= n ? (uint32_t)in.hist[w] : 0u;
+      const uint32_t c = w <= n ? (uint32_t)ghist[w] : 0u;
       // inclusive wave scan of c, then across waves
       uint32_t inc = c;
       const int lane = tid & 63, wave = tid >> 6;
@@ -109,7 +116,7 @@ __global__ void __launch_bounds__(kWealthThreads)
       if (c) {
         const uint64_t ranksum = (uint64_t)c * before + (uint64_t)c * (c + 1) / 2;
         S_part += (uint32_t)w * (uint32_t)ranksum;
-        in.hist[w] = 0;
+        ghist[w] = 0;
       }
       __syncthreads();
       if (tid == kWealthThreads - 1) carry = before + c;
@@ -130,6 +137,43 @@ __global__ void __launch_bounds__(kWealthThreads)
       in.gini_out[t] = g;
     }
     __syncthreads();
+  }
+}
+
+// kLds: the launch carries dynamic LDS for max_lds agents; instances with
+// n <= max_lds run out of it, others on their global buffers.
+template <bool kLds>
+__global__ void __launch_bounds__(kWealthThreads)
+    k_wealth_run(const abm_wealth_inst *__restrict__ insts, int64_t max_lds, int mode) {
+  extern __shared__ int32_t lds_agents[];
+  __shared__ uint32_t red[16];
+  __shared__ uint32_t carry;
+  const abm_wealth_inst in = insts[blockIdx.x];
+  const int64_t n = in.n;
+  const int tid = threadIdx.x;
+  if (kLds && n <= max_lds) {
+    float *lx = reinterpret_cast<float *>(lds_agents);
+    float *ly = lx + n;
+    int32_t *lw = reinterpret_cast<int32_t *>(ly + n);
+    int32_t *linc = lw + n;
+    int32_t *lhist = linc + n;
+    for (int64_t i = tid; i < n; i += kWealthThreads) {
+      lx[i] = in.x[i];
+      ly[i] = in.y[i];
+      lw[i] = in.w[i];
+      linc[i] = 0;
+      lhist[i] = 0;
+    }
+    if (tid == 0) lhist[n] = 0;
+    __syncthreads();
+    wealth_steps(in, lx, ly, lw, linc, lhist, red, carry, mode);
+    for (int64_t i = tid; i < n; i += kWealthThreads) {
+      in.x[i] = lx[i];
+      in.y[i] = ly[i];
+      in.w[i] = lw[i];
+    }
+  } else {
+    wealth_steps(in, in.x, in.y, in.w, in.inc, in.hist, red, carry, mode);
   }
 }
 
@@ -162,13 +206,26 @@ int abm_wealth_init(const uint32_t *init_key, int64_t n, float *x, float *y, int
   return kOk;
 }
 
-int abm_wealth_run_batch(const abm_wealth_inst *insts, int32_t count, int mode,
+int abm_wealth_run_batch(const abm_wealth_inst *insts, int32_t count, int64_t max_n, int mode,
                          void *stream) {
-  ABM_REQUIRE(insts && count >= 0 && (mode == 0 || mode == 1),
+  ABM_REQUIRE(insts && count >= 0 && max_n >= 0 && (mode == 0 || mode == 1),
               "abm_wealth_run_batch: bad arguments");
   if (count == 0) return kOk;
-  hipLaunchKernelGGL(k_wealth_run, dim3((unsigned)count), dim3(kWealthThreads), 0,
-                     (hipStream_t)stream, insts, mode);
+  if (max_n > 0 && max_n <= kWealthLdsAgents) {
+    static bool attr_set = false;
+    if (!attr_set) {
+      ABM_CHECK_HIP(hipFuncSetAttribute((const void *)k_wealth_run<true>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)(kWealthLdsAgents * kWealthLdsBytesPerAgent + 4)));
+      attr_set = true;
+    }
+    hipLaunchKernelGGL(k_wealth_run<true>, dim3((unsigned)count), dim3(kWealthThreads),
+                       (size_t)(max_n * kWealthLdsBytesPerAgent + 4), (hipStream_t)stream,
+                       insts, max_n, mode);
+  } else {
+    hipLaunchKernelGGL(k_wealth_run<false>, dim3((unsigned)count), dim3(kWealthThreads), 0,
+                       (hipStream_t)stream, insts, (int64_t)0, mode);
+  }
   ABM_LAUNCH_CHECK("abm_wealth_run_batch");
   return kOk;
 }

@@ -69,7 +69,8 @@ class WealthEngine:
         mean = torch.empty(max(steps, 1), dtype=torch.float32, device=self.device)
         gini = torch.empty(max(steps, 1), dtype=torch.float32, device=self.device)
         desc = torch.from_numpy(self.descriptor(keys, steps, mean, gini).view(np.uint8)).to(self.device)
-        check(lib().abm_wealth_run_batch(ptr(desc), 1, self.mode, stream_handle()), "wealth_run")
+        check(lib().abm_wealth_run_batch(ptr(desc), 1, self.n, self.mode, stream_handle()),
+              "wealth_run")
         return mean[:steps].cpu().numpy(), gini[:steps].cpu().numpy()
 
 
@@ -91,6 +92,7 @@ def run_batch(engines: list[WealthEngine], steps: list[int], stream=None):
         descs.append(e.descriptor(keys, s, mean, gini))
         outs.append((mean, gini, s))
     table = torch.from_numpy(np.concatenate(descs).view(np.uint8)).to(dev)
-    check(lib().abm_wealth_run_batch(ptr(table), len(engines), mode, stream_handle(stream)),
+    max_n = max(e.n for e in engines)
+    check(lib().abm_wealth_run_batch(ptr(table), len(engines), max_n, mode, stream_handle(stream)),
           "wealth_run_batch")
     return [(m[:s].cpu().numpy(), g[:s].cpu().numpy()) for m, g, s in outs]

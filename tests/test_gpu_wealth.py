@@ -56,3 +56,43 @@ def test_wealth_batch_equals_alone(wealth):
     for (m1, g1), (m2, g2) in zip(solo, mixed):
         np.testing.assert_array_equal(m1, m2)
         np.testing.assert_array_equal(g1, g2)
+
+
+def _run_raw(wealth, eng, steps, max_n):
+    """One instance through abm_wealth_run_batch with an explicit max_n."""
+    import torch
+    from jaxabm_amd._lib import check, lib, ptr, stream_handle
+
+    keys = eng.key_table(steps)
+    mean = torch.empty(steps, dtype=torch.float32, device=eng.device)
+    gini = torch.empty(steps, dtype=torch.float32, device=eng.device)
+    desc = torch.from_numpy(eng.descriptor(keys, steps, mean, gini).view(np.uint8)).to(eng.device)
+    check(lib().abm_wealth_run_batch(ptr(desc), 1, max_n, eng.mode, stream_handle()), "run")
+    return mean.cpu().numpy(), gini.cpu().numpy()
+
+
+@pytest.mark.parametrize("n,max_n", [(1000, 0), (1000, 500), (5000, 8000), (8000, 8000)])
+def test_wealth_lds_and_global_paths_agree(wealth, n, max_n):
+    """LDS-resident runs (n <= max_n <= 8000) and global-buffer runs (max_n = 0,
+    or an instance above max_n) give identical records and state."""
+    steps = 40
+    a = wealth.WealthEngine(n, seed=3)
+    b = wealth.WealthEngine(n, seed=3)
+    ma, ga = a.run(steps)                     # max_n = n: resident when n <= 8000
+    mb, gb = _run_raw(wealth, b, steps, max_n)
+    np.testing.assert_array_equal(ma, mb)
+    np.testing.assert_array_equal(ga, gb)
+    for t in ("w", "x", "y"):
+        np.testing.assert_array_equal(getattr(a, t).cpu().numpy(), getattr(b, t).cpu().numpy())
+
+
+def test_wealth_global_path_matches_oracle(wealth):
+    """An instance above the LDS capacity runs on its global buffers."""
+    n, steps = 9000, 8
+    eng = wealth.WealthEngine(n, seed=11)
+    x, y, w, rng = O.wealth_init(n, 11, eng.mode)
+    mean, gini = eng.run(steps)
+    om, og = O.wealth_run(x, y, w, rng, steps, eng.mode)
+    np.testing.assert_array_equal(eng.w.cpu().numpy(), w)
+    np.testing.assert_array_equal(mean, om)
+    np.testing.assert_array_equal(gini, og)
