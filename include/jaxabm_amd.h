@@ -210,7 +210,7 @@ typedef struct abm_schelling_grid {
   int32_t *pos;          /* device [H*W]: cell of agent k (first n_agents) */
   int8_t *type;          /* device [H*W]: type of agent k */
   int32_t *target;       /* device [H*W]: scratch */
-  int32_t *empty;        /* device [H*W]: scratch (empty-cell list) */
+  int32_t *empty;        /* device [H*W], 16-byte aligned: scratch (empty-cell bitmap) */
   int32_t *block_counts; /* device [abm_schelling_block_count(H*W)] */
 } abm_schelling_grid;
 int64_t abm_schelling_block_count(int64_t n_cells);
@@ -219,7 +219,8 @@ int64_t abm_schelling_block_count(int64_t n_cells);
 int abm_schelling_init(abm_schelling_grid *g, const uint32_t *key, int mode, int32_t *n_agents_out,
                        void *stream);
 /* Runs `steps` steps; keys: device [steps][2] collection keys ck.  counters:
- * device scratch [steps][2]; happy_frac, moves: device [steps]. */
+ * device scratch [steps][2]; happy_frac, moves: device [steps].  Grids of up
+ * to 16,776,192 cells. */
 int abm_schelling_run(const abm_schelling_grid *g, const uint32_t *keys, int32_t steps, int mode,
                       int32_t *counters, float *happy_frac, int32_t *moves, void *stream);
 

@@ -13,16 +13,21 @@
 //         move (their old cell is not reusable until the next step)
 //  record fraction happy (agent phase) and number of moves
 //
-// Kernels per step (captured with the rest of the run):
-//   k_cells_count / k_cells_scatter  row-major compaction of the empty cells
-//                                    (per-1024-cell block counts, then block
-//                                    offsets and an in-block scan)
-//   k_schelling_agents               32x32 cell tiles + 1-cell torus halo in
-//                                    LDS; happiness, draw, atomicMin claim
-//   k_schelling_moves                winners move; claims reset for the next step
+// The empty cells are kept as a bitmap (bit c set when cell c is empty); the
+// moves kernel keeps it current with atomic bit flips (a cell changes state at
+// most once per step), so the row-major empty list is never materialised:
+// every agents block counts the empty cells of each 1024-cell block by
+// popcounts of the (L2-resident, 32 KB at 512 x 512) bitmap, scans the counts
+// in LDS, and empty[j] is then a search of that prefix and a popcount scan of
+// one block's 32 words.  Two kernels per step (captured with the rest of the
+// run):
+//   k_schelling_agents  32x32 cell tiles + 1-cell torus halo in LDS;
+//                       happiness, draw, empty[j], atomicMin claim
+//   k_schelling_moves   winners move; bitmap and claims updated
 // Layout (HBM): cells int8[H*W], id int32[H*W] (agent on a cell or -1),
 // claim int32[H*W] (INT32_MAX when unclaimed), pos int32[n_agents],
-// type int8[n_agents], target int32[n_agents], empty int32[H*W].
+// type int8[n_agents], target int32[n_agents], empty-cell bitmap
+// uint32[ceil(H*W/32)] (in the grid's `empty` scratch).
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -34,6 +39,9 @@ namespace abm {
 
 constexpr int kTileDim = 32;
 constexpr int kCellsPerBlock = 1024;
+constexpr int kWordsPerBlock = kCellsPerBlock / 32;
+// block prefix of the empty counts in LDS: at most 64 KB
+constexpr int64_t kMaxEmptyBlocks = 16383;
 
 // number of cells with (cells[c] == 0) == want_empty per 1024-cell block
 __global__ void __launch_bounds__(kCellsPerBlock)
@@ -94,11 +102,115 @@ __global__ void __launch_bounds__(kCellsPerBlock)
   }
 }
 
+// Empty-cell bitmap of the current grid: one ballot per 64 cells.
+__global__ void __launch_bounds__(kCellsPerBlock)
+    k_empty_bits(const int8_t *__restrict__ cells, int64_t nc, int64_t nwords,
+                 uint32_t *__restrict__ ebits) {
+  const int64_t c = (int64_t)blockIdx.x * kCellsPerBlock + threadIdx.x;
+  const bool e = c < nc && cells[c] == 0;
+  const uint64_t m = __ballot(e);
+  if ((threadIdx.x & 63) == 0) {
+    const int64_t w = c >> 5;  // c is the wave's first cell (a multiple of 64)
+    if (w < nwords) ebits[w] = (uint32_t)m;
+    if (w + 1 < nwords) ebits[w + 1] = (uint32_t)(m >> 32);
+  }
+}
+
+// Number of empty cells of 1024-cell block b: popcount of its 32 bitmap words
+// (eight 16-byte loads issued together).
+__device__ __forceinline__ int32_t block_empty(const uint32_t *__restrict__ ebits,
+                                               int64_t nwords, int nb, int b) {
+  if (b >= nb) return 0;
+  const int64_t w0 = (int64_t)b * kWordsPerBlock;
+  int32_t s = 0;
+  if (w0 + kWordsPerBlock <= nwords) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(ebits + w0);
+    uint4 v[kWordsPerBlock / 4];
+#pragma unroll
+    for (int q = 0; q < kWordsPerBlock / 4; ++q) v[q] = p[q];
+#pragma unroll
+    for (int q = 0; q < kWordsPerBlock / 4; ++q)
+      s += __popc(v[q].x) + __popc(v[q].y) + __popc(v[q].z) + __popc(v[q].w);
+  } else {
+    for (int q = 0; q < kWordsPerBlock; ++q)
+      if (w0 + q < nwords) s += __popc(ebits[w0 + q]);
+  }
+  return s;
+}
+
+// P[0] = 0, P[b + 1] = empty cells of blocks 0 .. b, for b < nb, in LDS (wave
+// scans, chunks of blockDim.x blocks).  first: this thread's count for the
+// first chunk, loaded by the caller ahead of its other loads.
+__device__ void empty_prefix(const uint32_t *__restrict__ ebits, int64_t nwords, int nb,
+                             int32_t first, int32_t *P, int32_t *wsum, int32_t *carry) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) {
+    P[0] = 0;
+    *carry = 0;
+  }
+  __syncthreads();
+  for (int base = 0; base < nb; base += (int)blockDim.x) {
+    const int b = base + tid;
+    int32_t v = base == 0 ? first : block_empty(ebits, nwords, nb, b);
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int32_t y = __shfl_up(v, off);
+      if (lane >= off) v += y;
+    }
+    if (lane == 63) wsum[wave] = v;
+    __syncthreads();
+    int32_t before = *carry;
+    for (int q = 0; q < wave; ++q) before += wsum[q];
+    if (b < nb) P[b + 1] = before + v;
+    __syncthreads();
+    if (tid == (int)blockDim.x - 1) *carry = before + v;
+    __syncthreads();
+  }
+}
+
+// Cell of the j-th empty cell in row-major order (j < P[nb]): the block b with
+// P[b] <= j < P[b + 1], then the (j - P[b])-th set bit of its 32 bitmap words.
+__device__ __forceinline__ int32_t jth_empty(uint32_t j, const int32_t *P, int nb,
+                                             const uint32_t *__restrict__ ebits,
+                                             int64_t nwords) {
+  int lo = 0, hi = nb;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if ((uint32_t)P[mid] <= j)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  uint32_t r = j - (uint32_t)P[lo];
+  const int64_t w0 = (int64_t)lo * kWordsPerBlock;
+  uint32_t wv[kWordsPerBlock];
+#pragma unroll
+  for (int q = 0; q < kWordsPerBlock; ++q) wv[q] = w0 + q < nwords ? ebits[w0 + q] : 0u;
+  int32_t cell = -1;
+#pragma unroll
+  for (int q = 0; q < kWordsPerBlock; ++q) {
+    const uint32_t pc = (uint32_t)__popc(wv[q]);
+    if (cell < 0) {
+      if (r < pc) {
+        uint32_t v = wv[q];
+        for (uint32_t k = 0; k < r; ++k) v &= v - 1;
+        cell = (int32_t)((w0 + q) * 32 + __builtin_ctz(v));
+      } else {
+        r -= pc;
+      }
+    }
+  }
+  return cell;
+}
+
 struct SchellingStep {
   int32_t H, W;
   int64_t n_agents, n_empty;
   const int8_t *cells;
-  const int32_t *id, *empty;
+  const int32_t *id;
+  const uint32_t *ebits;
+  int64_t nwords;
+  int32_t nb;
   int32_t *claim, *target;
   const uint32_t *ck;
   int mode;
@@ -106,22 +218,41 @@ struct SchellingStep {
 };
 
 __global__ void __launch_bounds__(kTileDim * kTileDim) k_schelling_agents(SchellingStep a) {
+  extern __shared__ int32_t empty_prefix_lds[];  // [nb + 1]
   __shared__ int8_t tile[kTileDim + 2][kTileDim + 2];
   __shared__ int32_t wc[16];
-  const int tx = threadIdx.x % kTileDim, ty = threadIdx.x / kTileDim;
+  __shared__ int32_t carry;
+  constexpr int kHalo = (kTileDim + 2) * (kTileDim + 2);  // 34 x 34 cells, <= 2 per thread
+  const int tid = threadIdx.x;
+  const int tx = tid % kTileDim, ty = tid / kTileDim;
   const int r0 = blockIdx.y * kTileDim, c0 = blockIdx.x * kTileDim;
-  // stage the tile and its torus halo: 34 x 34 cells
-  for (int q = threadIdx.x; q < (kTileDim + 2) * (kTileDim + 2); q += blockDim.x) {
-    const int lr = q / (kTileDim + 2), lc = q % (kTileDim + 2);
-    int gr = r0 + lr - 1, gc = c0 + lc - 1;
-    gr = (gr % a.H + a.H) % a.H;
-    gc = (gc % a.W + a.W) % a.W;
-    tile[lr][lc] = a.cells[(int64_t)gr * a.W + gc];
-  }
-  __syncthreads();
   const int r = r0 + ty, c = c0 + tx;
+  const bool inside = r < a.H && c < a.W;
+  const int64_t cell = (int64_t)r * a.W + c;
+  // one memory round trip: the tile and its torus halo, the bitmap words of
+  // this thread's block for the prefix, the agent on this thread's cell
+  int8_t tv[2] = {0, 0};
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int q = tid + i * (int)blockDim.x;
+    if (q < kHalo) {
+      const int lr = q / (kTileDim + 2), lc = q % (kTileDim + 2);
+      int gr = r0 + lr - 1, gc = c0 + lc - 1;
+      gr = (gr % a.H + a.H) % a.H;
+      gc = (gc % a.W + a.W) % a.W;
+      tv[i] = a.cells[(int64_t)gr * a.W + gc];
+    }
+  }
+  const int32_t first = block_empty(a.ebits, a.nwords, a.nb, tid);
+  const int32_t k = inside ? a.id[cell] : -1;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int q = tid + i * (int)blockDim.x;
+    if (q < kHalo) tile[q / (kTileDim + 2)][q % (kTileDim + 2)] = tv[i];
+  }
+  empty_prefix(a.ebits, a.nwords, a.nb, first, empty_prefix_lds, wc, &carry);
   bool happy = false;
-  if (r < a.H && c < a.W) {
+  if (inside) {
     const int8_t me = tile[ty + 1][tx + 1];
     if (me) {
       int occ = 0, same = 0;
@@ -135,24 +266,22 @@ __global__ void __launch_bounds__(kTileDim * kTileDim) k_schelling_agents(Schell
           same += v == me;
         }
       happy = 8 * same >= 3 * occ;
-      const int64_t cell = (int64_t)r * a.W + c;
-      const int32_t k = a.id[cell];
       int32_t tg = -1;
       if (!happy && a.n_empty > 0) {
         Key k1, k2;
         split2(split_at(Key{a.ck[0], a.ck[1]}, a.n_agents, k, a.mode), k1, k2, a.mode);
         const uint32_t j = randint_offset(bits_scalar(k1, a.mode), bits_scalar(k2, a.mode),
                                           (uint32_t)a.n_empty);
-        tg = a.empty[j];
-        atomicMin(&a.claim[tg], k);
+        tg = jth_empty(j, empty_prefix_lds, a.nb, a.ebits, a.nwords);
+        if (tg >= 0) atomicMin(&a.claim[tg], k);
       }
       a.target[k] = tg;
     }
   }
   const uint64_t m = __ballot(happy);
-  if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = (int32_t)__popcll(m);
+  if ((tid & 63) == 0) wc[tid >> 6] = (int32_t)__popcll(m);
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
     int32_t s = 0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += wc[w];
     if (s) atomicAdd(a.happy_count, s);
@@ -162,7 +291,7 @@ __global__ void __launch_bounds__(kTileDim * kTileDim) k_schelling_agents(Schell
 __global__ void k_schelling_moves(int64_t n_agents, int8_t *__restrict__ cells,
                                   int32_t *__restrict__ id, int32_t *__restrict__ claim,
                                   int32_t *__restrict__ pos, const int8_t *__restrict__ type,
-                                  const int32_t *__restrict__ target,
+                                  const int32_t *__restrict__ target, uint32_t *__restrict__ ebits,
                                   int32_t *__restrict__ moves) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool moved = false;
@@ -176,6 +305,10 @@ __global__ void k_schelling_moves(int64_t n_agents, int8_t *__restrict__ cells,
       id[tg] = (int32_t)k;
       pos[k] = tg;
       claim[tg] = INT_MAX;
+      // the old cell becomes empty, the target occupied (distinct cells: a
+      // target was empty before the step, an old cell occupied)
+      atomicOr(&ebits[old >> 5], 1u << (old & 31));
+      atomicAnd(&ebits[tg >> 5], ~(1u << (tg & 31)));
       moved = true;
     }
   }
@@ -286,15 +419,20 @@ int abm_schelling_run(const abm_schelling_grid *g, const uint32_t *keys, int32_t
   const int64_t nc = (int64_t)g->height * g->width;
   const int64_t n_empty = nc - g->n_agents;
   const unsigned nb = nblk(nc, kCellsPerBlock);
+  ABM_REQUIRE(nb <= kMaxEmptyBlocks,
+              "abm_schelling_run: grids of more than 16,776,192 cells are not supported");
+  const int64_t nwords = (nc + 31) / 32;
+  uint32_t *ebits = reinterpret_cast<uint32_t *>(g->empty);  // nwords <= H*W words
+  ABM_REQUIRE(((uintptr_t)ebits & 15u) == 0,
+              "abm_schelling_run: the empty scratch must be 16-byte aligned");
   hipLaunchKernelGGL(k_fill_i32_s, dim3(nblk(2 * (int64_t)steps, 256)), dim3(256), 0, s,
                      counters, 2 * (int64_t)steps, 0);
+  hipLaunchKernelGGL(k_empty_bits, dim3(nb), dim3(kCellsPerBlock), 0, s, g->cells, nc, nwords,
+                     ebits);
   const dim3 tiles((unsigned)((g->width + kTileDim - 1) / kTileDim),
                    (unsigned)((g->height + kTileDim - 1) / kTileDim));
+  const size_t prefix_lds = (size_t)(nb + 1) * sizeof(int32_t);
   for (int32_t t = 0; t < steps; ++t) {
-    hipLaunchKernelGGL(k_cells_count, dim3(nb), dim3(kCellsPerBlock), 0, s, g->cells, nc, 1,
-                       g->block_counts);
-    hipLaunchKernelGGL(k_cells_scatter, dim3(nb), dim3(kCellsPerBlock), 0, s, g->cells, nc, 1,
-                       g->block_counts, g->empty, (int8_t *)nullptr, (int32_t *)nullptr);
     SchellingStep a;
     a.H = g->height;
     a.W = g->width;
@@ -302,17 +440,19 @@ int abm_schelling_run(const abm_schelling_grid *g, const uint32_t *keys, int32_t
     a.n_empty = n_empty;
     a.cells = g->cells;
     a.id = g->id;
-    a.empty = g->empty;
+    a.ebits = ebits;
+    a.nwords = nwords;
+    a.nb = (int32_t)nb;
     a.claim = g->claim;
     a.target = g->target;
     a.ck = keys + 2 * (int64_t)t;
     a.mode = mode;
     a.happy_count = counters + 2 * (int64_t)t;
-    hipLaunchKernelGGL(k_schelling_agents, tiles, dim3(kTileDim * kTileDim), 0, s, a);
+    hipLaunchKernelGGL(k_schelling_agents, tiles, dim3(kTileDim * kTileDim), prefix_lds, s, a);
     if (g->n_agents > 0)
       hipLaunchKernelGGL(k_schelling_moves, dim3(nblk(g->n_agents, 256)), dim3(256), 0, s,
                          g->n_agents, g->cells, g->id, g->claim, g->pos, g->type, g->target,
-                         counters + 2 * (int64_t)t + 1);
+                         ebits, counters + 2 * (int64_t)t + 1);
   }
   hipLaunchKernelGGL(k_schelling_record, dim3(nblk(steps, 256)), dim3(256), 0, s, counters,
                      steps, g->n_agents > 0 ? g->n_agents : 1, happy_frac, moves);
