@@ -234,7 +234,7 @@ typedef struct abm_deffuant_problem {
   int32_t ckpt_every;      /* checkpoint interval k */
   int32_t pair_blocks;     /* workgroup rows of the step grids (<= 65535) */
   int32_t stat_chunks;     /* row chunks of the statistics pass (<= 65535) */
-  int32_t reserved;
+  int32_t flags;           /* ABM_DEFFUANT_* bits below */
   const int32_t *perms;    /* device [steps][n]: pairing of each step */
   const float *eps, *mu;   /* device [B] */
   float *work;             /* device [n][B]: final state after the forward */
@@ -251,6 +251,12 @@ typedef struct abm_deffuant_problem {
   double *grads;           /* device [2][B]: dL/deps, dL/dmu (backward) */
   const double *weights;   /* device [34] per-statistic loss weights, or NULL (all 1) */
 } abm_deffuant_problem;
+/* flags bit: in abm_deffuant_forward, also keep the last checkpoint segment's
+ * intermediate states in traj (same bytes: those steps write out of place);
+ * in abm_deffuant_backward, traj holds them from the forward just before on
+ * this descriptor, so that segment is not recomputed.  A second backward of
+ * the same forward must clear it (the first one overwrites traj). */
+#define ABM_DEFFUANT_TAPE_LAST 1
 /* x[i][b] = uniform(split(init_key, n)[i]) for every instance b. */
 int abm_deffuant_init(const uint32_t *init_key, int64_t n, int32_t batch, float *x, int mode,
                       void *stream);
@@ -258,8 +264,9 @@ int abm_deffuant_init(const uint32_t *init_key, int64_t n, int32_t batch, float 
 int abm_deffuant_step(const float *src, float *dst, int64_t n, int32_t batch,
                       const int32_t *perm, const float *eps, const float *mu,
                       int32_t pair_blocks, void *stream);
-/* Steps from ckpt[0], keeping the checkpoints; then stats, loss and the
- * dL/dx coefficients of the final state. */
+/* Steps from ckpt[0], keeping the checkpoints (and, with
+ * ABM_DEFFUANT_TAPE_LAST and traj, the last segment's states); then stats,
+ * loss and the dL/dx coefficients of the final state. */
 int abm_deffuant_forward(const abm_deffuant_problem *p, void *stream);
 /* Reverse sweep after abm_deffuant_forward on the same descriptor:
  * recomputes each checkpoint segment, then grads = dL/d(eps, mu). */

@@ -44,11 +44,14 @@ class SchellingGrid(C.Structure):
     ]
 
 
+DEFFUANT_TAPE_LAST = 1  # abm_deffuant_problem.flags bit
+
+
 class DeffuantProblem(C.Structure):
     _fields_ = [
         ("n", C.c_int64), ("batch", C.c_int32), ("steps", C.c_int32),
         ("ckpt_every", C.c_int32), ("pair_blocks", C.c_int32), ("stat_chunks", C.c_int32),
-        ("reserved", C.c_int32),
+        ("flags", C.c_int32),
         ("perms", C.c_void_p), ("eps", C.c_void_p), ("mu", C.c_void_p), ("work", C.c_void_p),
         ("ckpt", C.c_void_p), ("traj", C.c_void_p), ("lam", C.c_void_p), ("acc", C.c_void_p),
         ("partial", C.c_void_p), ("sums", C.c_void_p), ("coef", C.c_void_p),

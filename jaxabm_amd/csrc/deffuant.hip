@@ -74,6 +74,10 @@ __device__ __forceinline__ float gate(float eps, float d) {
   return 1.0f / den;
 }
 
+// Pairs each wavefront holds in flight per loop iteration of the step kernel:
+// eight 1 KB row loads outstanding per wave (measured: 4 beats 2 and 8).
+constexpr int kDefStepPairs = 4;
+
 // One step for the whole batch.  src == dst updates in place (pairs are
 // disjoint); otherwise every row of dst is written (the odd agent is copied).
 template <int VEC>
@@ -94,33 +98,41 @@ __global__ void __launch_bounds__(kDefThreads)
   }
   const int64_t npairs = n >> 1;
   const int64_t stride = (int64_t)gridDim.y * kDefWaves;
-  for (int64_t p0 = (int64_t)blockIdx.y * kDefWaves + wave; p0 < npairs; p0 += 2 * stride) {
-    const int64_t p1 = p0 + stride;
-    const bool two = p1 < npairs;
-    const int64_t a0 = perm[2 * p0], c0 = perm[2 * p0 + 1];
-    const int64_t a1 = two ? perm[2 * p1] : a0, c1 = two ? perm[2 * p1 + 1] : c0;
-    float xa0[VEC], xc0[VEC], xa1[VEC], xc1[VEC];
-    load_cols<VEC>(src + a0 * B + col, xa0);
-    load_cols<VEC>(src + c0 * B + col, xc0);
-    load_cols<VEC>(src + a1 * B + col, xa1);
-    load_cols<VEC>(src + c1 * B + col, xc1);
+  for (int64_t p0 = (int64_t)blockIdx.y * kDefWaves + wave; p0 < npairs;
+       p0 += kDefStepPairs * stride) {
+    // pairs p0, p0 + stride, ...; one past the end re-reads pair p0 and
+    // stores nothing
+    int64_t a[kDefStepPairs], c[kDefStepPairs];
+    bool live[kDefStepPairs];
 #pragma unroll
-    for (int v = 0; v < VEC; ++v) {
-      float d = xc0[v] - xa0[v];
-      float delta = (m[v] * gate(e[v], d)) * d;
-      xa0[v] = xa0[v] + delta;
-      xc0[v] = xc0[v] - delta;
-      d = xc1[v] - xa1[v];
-      delta = (m[v] * gate(e[v], d)) * d;
-      xa1[v] = xa1[v] + delta;
-      xc1[v] = xc1[v] - delta;
+    for (int i = 0; i < kDefStepPairs; ++i) {
+      const int64_t p = p0 + i * stride;
+      live[i] = p < npairs;
+      const int64_t pp = live[i] ? p : p0;
+      a[i] = perm[2 * pp];
+      c[i] = perm[2 * pp + 1];
     }
-    store_cols<VEC>(dst + a0 * B + col, xa0);
-    store_cols<VEC>(dst + c0 * B + col, xc0);
-    if (two) {
-      store_cols<VEC>(dst + a1 * B + col, xa1);
-      store_cols<VEC>(dst + c1 * B + col, xc1);
+    float xa[kDefStepPairs][VEC], xc[kDefStepPairs][VEC];
+#pragma unroll
+    for (int i = 0; i < kDefStepPairs; ++i) {
+      load_cols<VEC>(src + a[i] * B + col, xa[i]);
+      load_cols<VEC>(src + c[i] * B + col, xc[i]);
     }
+#pragma unroll
+    for (int i = 0; i < kDefStepPairs; ++i)
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        const float d = xc[i][v] - xa[i][v];
+        const float delta = (m[v] * gate(e[v], d)) * d;
+        xa[i][v] = xa[i][v] + delta;
+        xc[i][v] = xc[i][v] - delta;
+      }
+#pragma unroll
+    for (int i = 0; i < kDefStepPairs; ++i)
+      if (live[i]) {
+        store_cols<VEC>(dst + a[i] * B + col, xa[i]);
+        store_cols<VEC>(dst + c[i] * B + col, xc[i]);
+      }
   }
   if ((n & 1) && src != dst && blockIdx.y == 0 && wave == 0) {
     const int64_t r = perm[n - 1];
@@ -431,10 +443,19 @@ int abm_deffuant_forward(const abm_deffuant_problem *p, void *stream) {
   if (int rc = check_problem(p, "abm_deffuant_forward", false)) return rc;
   hipStream_t s = (hipStream_t)stream;
   const int64_t nB = p->n * (int64_t)p->batch;
+  const int32_t k = p->ckpt_every;
+  // first step of the last segment; with the tape flag its steps write traj
+  // out of place (the backward then skips recomputing it)
+  const int32_t t_last = p->steps > 0 ? (p->steps - 1) / k * k : 0;
+  const bool tape = (p->flags & ABM_DEFFUANT_TAPE_LAST) && p->traj && k > 1;
   const float *src = p->ckpt;
   for (int32_t t = 0; t < p->steps; ++t) {
-    const bool to_ckpt = ((t + 1) % p->ckpt_every) == 0 && t + 1 < p->steps;
-    float *dst = to_ckpt ? p->ckpt + (int64_t)((t + 1) / p->ckpt_every) * nB : p->work;
+    const bool to_ckpt = ((t + 1) % k) == 0 && t + 1 < p->steps;
+    float *dst = p->work;
+    if (to_ckpt)
+      dst = p->ckpt + (int64_t)((t + 1) / k) * nB;
+    else if (tape && t >= t_last && t + 1 < p->steps)
+      dst = p->traj + (int64_t)(t - t_last) * nB;
     launch_step(src, dst, p->n, p->batch, p->perms + (int64_t)t * p->n, p->eps, p->mu,
                 p->pair_blocks, s);
     src = dst;
@@ -460,7 +481,8 @@ int abm_deffuant_backward(const abm_deffuant_problem *p, void *stream) {
     const int32_t t0 = j * k, t1 = t0 + k < p->steps ? t0 + k : p->steps;
     const float *ck = p->ckpt + (int64_t)j * nB;
     const float *src = ck;
-    for (int32_t u = 0; u + 1 < t1 - t0; ++u) {
+    const bool taped = j == nseg - 1 && (p->flags & ABM_DEFFUANT_TAPE_LAST);
+    for (int32_t u = 0; !taped && u + 1 < t1 - t0; ++u) {
       float *dst = p->traj + (int64_t)u * nB;
       launch_step(src, dst, n, p->batch, p->perms + (int64_t)(t0 + u) * n, p->eps, p->mu,
                   p->pair_blocks, s);

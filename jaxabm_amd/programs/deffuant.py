@@ -27,7 +27,8 @@ import numpy as np
 import torch
 
 from .. import random as jr
-from .._lib import DeffuantProblem, check, lib, ptr, require_gpu, stream_handle
+from .._lib import (DEFFUANT_TAPE_LAST, DeffuantProblem, check, lib, ptr, require_gpu,
+                    stream_handle)
 
 STATS_LEN = 34
 HIST_BINS = 32
@@ -95,6 +96,7 @@ class DeffuantModel:
         self.mu = torch.zeros((B,), dtype=torch.float32, device=dev)
         self.version = 0
         self._backward_ready = False
+        self._tape_fresh = False   # traj holds the last segment from the forward
 
     @property
     def x0(self) -> torch.Tensor:
@@ -103,9 +105,9 @@ class DeffuantModel:
     def final_state(self) -> torch.Tensor:
         return self.work if self.steps else self.ckpt[0]
 
-    def _problem(self) -> DeffuantProblem:
+    def _problem(self, flags: int = 0) -> DeffuantProblem:
         p = DeffuantProblem()
-        p.n, p.batch, p.steps = self.n, self.batch, self.steps
+        p.n, p.batch, p.steps, p.flags = self.n, self.batch, self.steps, flags
         p.ckpt_every, p.pair_blocks, p.stat_chunks = (self.ckpt_every, self.pair_blocks,
                                                       self.stat_chunks)
         for name in ("perms", "eps", "mu", "work", "ckpt", "traj", "lam", "acc", "partial",
@@ -126,17 +128,20 @@ class DeffuantModel:
         self.target.copy_(torch.as_tensor(np.asarray(target, np.float64).reshape(STATS_LEN)))
         w = np.ones(STATS_LEN) if weights is None else np.asarray(weights, np.float64)
         self.weights.copy_(torch.as_tensor(w.reshape(STATS_LEN)))
-        check(lib().abm_deffuant_forward(C.byref(self._problem()), stream_handle()),
-              "deffuant_forward")
+        check(lib().abm_deffuant_forward(C.byref(self._problem(DEFFUANT_TAPE_LAST)),
+                                         stream_handle()), "deffuant_forward")
         self.version += 1
         self._backward_ready = True
+        self._tape_fresh = True
         return self.loss.clone()
 
     def backward(self) -> tuple[torch.Tensor, torch.Tensor]:
         """dL_b/deps_b and dL_b/dmu_b (float64, device) of the last forward."""
         if not self._backward_ready:
             raise RuntimeError("backward needs a forward on this model first")
-        check(lib().abm_deffuant_backward(C.byref(self._problem()), stream_handle()),
+        flags = DEFFUANT_TAPE_LAST if self._tape_fresh else 0
+        self._tape_fresh = False   # this sweep overwrites traj
+        check(lib().abm_deffuant_backward(C.byref(self._problem(flags)), stream_handle()),
               "deffuant_backward")
         return self.grads[0].clone(), self.grads[1].clone()
 

@@ -113,16 +113,21 @@ def test_adjoint_matches_restated_jax_grad(dm):
 
 def test_checkpoint_interval_does_not_change_gradients(dm):
     """Recomputation replays the forward kernels bit for bit, so every
-    checkpoint interval gives identical gradients."""
+    checkpoint interval gives identical gradients; so does the forward's tape
+    of the last segment (first backward) against recomputing it (a second
+    backward of the same forward).  Last segments of 23 % k steps: 3, 2, 1
+    (k = 2, 11), whole (k = 1, 23) and the whole run (k = 50)."""
     n, B, steps = 1500, 4, 23
     target = oracle_target(n, steps, 4)
     eps = np.float32([0.1, 0.2, 0.3, 0.15])
     mu = np.float32([0.3, 0.2, 0.1, 0.45])
     ref = None
-    for k in (1, 4, 7, 23, 50):
+    for k in (1, 2, 4, 7, 11, 23, 50):
         m = dm.DeffuantModel(n, B, steps, seed=4, ckpt_every=k)
         loss = m.forward(eps, mu, target).cpu().numpy()
         g = np.stack([x.cpu().numpy() for x in m.backward()])
+        g2 = np.stack([x.cpu().numpy() for x in m.backward()])
+        np.testing.assert_array_equal(g2, g)
         if ref is None:
             ref = (loss, g)
         else:
