@@ -101,30 +101,29 @@ def cpu_baseline(row_ptr, col, runs: int) -> dict:
     }
 
 
-TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                            "k_sir_step_traffic.json")
+PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
 
 
-def measured_traffic():
-    """HBM bytes per k_sir_step launch from the committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the same
-    counters' reading for a copy of known size), or None."""
+def measured_traffic(kernel: str = "k_sir_step"):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 PMC
+    summary profiles/<kernel>_traffic.json (FETCH_SIZE + WRITE_SIZE per
+    dispatch, scaled by the same counters' reading for a copy of known size),
+    or None."""
+    path = os.path.join(PROFILES, f"{kernel}_traffic.json")
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             d = json.load(f)
         return {"bytes_per_launch": d["bytes_per_launch"], "source": "profiles/" +
-                os.path.basename(TRAFFIC_FILE)}
+                os.path.basename(path)}
     except (OSError, ValueError, KeyError):
         return None
 
 
-def pmc_probe() -> int:
+def pmc_probe(workload: str = "sir") -> int:
     """A 256 MiB device-to-device copy (3 times: the counters' calibration),
-    then one 365-step SIR simulation launched step by step."""
+    then one 365-step SIR simulation launched step by step, or (deffuant) one
+    forward + adjoint gradient evaluation of the 65,536 x 256 x 200 batch."""
     import torch
-
-    from jaxabm_amd.network import ba_csr
-    from jaxabm_amd.programs.sir import SIRNetworkEngine
 
     nbytes = 256 << 20
     src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
@@ -132,6 +131,27 @@ def pmc_probe() -> int:
     for _ in range(3):
         dst.copy_(src)
     torch.cuda.synchronize()
+    del src, dst
+    if workload == "deffuant":
+        import numpy as np
+
+        from jaxabm_amd.programs.deffuant import DeffuantModel, target_stats
+
+        n, B, S, K = 65536, 256, 200, 20
+        target = target_stats(n, S, seed=0)
+        g = np.linspace(0.05, 0.5, 16, dtype=np.float32)
+        m = DeffuantModel(n, B, S, seed=0, ckpt_every=K)
+        loss = m.forward(np.repeat(g, 16), np.tile(g, 16), target)
+        ge, gm = m.backward()
+        torch.cuda.synchronize()
+        print(json.dumps({"probe": "copy 256 MiB x3 (read 256 MiB + write 256 MiB each) + one "
+                          "Deffuant forward + adjoint (65,536 x 256 x 200, checkpoints every 20)",
+                          "copy_bytes": nbytes, "loss_sum": float(loss.sum()),
+                          "grad_abs_sum": float(ge.abs().sum() + gm.abs().sum())}))
+        return 0
+    from jaxabm_amd.network import ba_csr
+    from jaxabm_amd.programs.sir import SIRNetworkEngine
+
     row_ptr, col = ba_csr(N_AGENTS, BA_M, GRAPH_SEED)
     eng = SIRNetworkEngine(N_AGENTS, csr=(row_ptr, col), beta=BETA, gamma=GAMMA,
                            n_seed_infected=N_SEED, seed=SEED)
@@ -157,6 +177,22 @@ def _timed(stream, fn, runs: int) -> float:
     check(lib().abm_event_timer(sh, 1, C.byref(ms)), "timer")
     torch.cuda.synchronize()
     return ms.value / 1e3 / runs
+
+
+def copy_reference_gbs(stream) -> dict:
+    """Device-to-device copy rate of 1 GiB (read + write bytes / time), the
+    plain streaming ceiling the kernels' HBM rates compare with; a copy this
+    size does not fit the 256 MB Infinity Cache."""
+    import torch
+
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    t = _timed(stream, lambda: dst.copy_(src), 20)
+    del src, dst
+    return {"bytes": 2 << 30, "GBs": (2 << 30) / t / 1e9, "frac": (2 << 30) / t / 1e9 / HBM_PEAK_GBS,
+            "what": "torch copy_ of 1 GiB, read + write bytes per second"}
 
 
 def bench_deffuant(args) -> dict:
@@ -189,6 +225,7 @@ def bench_deffuant(args) -> dict:
     fwd_bytes = n * B * 8 + 4 * n  # x read + write, the pairing
     step_s = t_fwd / S
     achieved = fwd_bytes / step_s / 1e9
+    traffic = measured_traffic("k_deffuant_step")
     return {
         "metric": "agent-instance-steps/s (Deffuant gradient evaluation, forward + adjoint)",
         "value": n * B * S / t_both, "unit": "agent-instance-steps/s", "n_gpus": 1,
@@ -200,9 +237,12 @@ def bench_deffuant(args) -> dict:
                    "bench_step": "one forward + adjoint of the batch"},
         "forward_ms": t_fwd * 1e3, "adjoint_ms": (t_both - t_fwd) * 1e3,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
                      "kernel": "k_deffuant_step (forward average incl. final statistics)",
-                     "algorithmic_bytes_per_launch": fwd_bytes, "avg_kernel_us": step_s * 1e6},
+                     "algorithmic_bytes_per_launch": fwd_bytes, "avg_kernel_us": step_s * 1e6,
+                     "copy_reference": copy_reference_gbs(stream)},
     }
 
 
@@ -358,8 +398,9 @@ def main() -> int:
                     help="sir (default): the headline line; the others measure the other "
                          "programs at their configured sizes")
     ap.add_argument("--pmc-probe", action="store_true",
-                    help="run a 256 MiB device copy and one un-captured SIR simulation (for "
-                         "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) and exit")
+                    help="run a 256 MiB device copy and one un-captured SIR simulation, or with "
+                         "--workload deffuant one gradient evaluation (for rocprofv3 --pmc "
+                         "FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -367,7 +408,7 @@ def main() -> int:
               "multi-GPU sharding is a design note, not a measured path.", file=sys.stderr)
         return 2
     if args.pmc_probe:
-        return pmc_probe()
+        return pmc_probe(args.workload)
     if args.workload != "sir":
         import torch
 
@@ -531,6 +572,7 @@ def main() -> int:
     if traffic is not None:
         out["roofline"]["traffic"] = traffic["bytes_per_launch"]
         out["roofline"]["traffic_source"] = traffic["source"]
+    out["roofline"]["copy_reference"] = copy_reference_gbs(torch.cuda.current_stream())
     if args.cpu_runs > 0:
         out["cpu_baseline"] = cpu_baseline(row_ptr, col, args.cpu_runs)
     if args.serving_requests > 0:
