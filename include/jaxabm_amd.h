@@ -338,6 +338,13 @@ int abm_graph_begin(void *stream);
 int abm_graph_end(void *stream, abm_graph **out);
 int abm_graph_launch(abm_graph *g, void *stream);
 int abm_graph_destroy(abm_graph *g);
+/* Prepares a captured, not yet instantiated hipGraph_t (a generic model step
+ * of torch ops; replaces the per-step dispatch of jaxabm/model.py:146-216)
+ * for replay: every memset node becomes a fill-kernel node with the same
+ * edges; a graph holding host, event, allocation, symbol-copy or child-graph
+ * nodes is refused (kErrInvalid) and left unchanged.  counts[4]: memsets
+ * replaced, kernel nodes, memcpy nodes, empty nodes. */
+int abm_graph_sanitize(void *graph, int32_t *counts);
 /* Stream-ordered device-to-device copy (capturable). */
 int abm_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
 

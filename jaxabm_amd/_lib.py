@@ -178,6 +178,7 @@ SIGNATURES = {
     "abm_graph_end": (_int, [_p, C.POINTER(_p)]),
     "abm_graph_launch": (_int, [_p, _p]),
     "abm_graph_destroy": (_int, [_p]),
+    "abm_graph_sanitize": (_int, [_p, _p]),
     "abm_memcpy_d2d": (_int, [_p, _p, _sz, _p]),
     "abm_event_timer": (_int, [_p, _int, C.POINTER(_f32)]),
 }

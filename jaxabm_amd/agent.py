@@ -36,12 +36,14 @@ def as_state_value(v) -> torch.Tensor:
     dtypes (bool, int32, float32), float64 arrays become float32."""
     if isinstance(v, torch.Tensor):
         return v
+    # scalars by a fill kernel, not a host copy: no host synchronisation, so a
+    # step that returns constants can be captured as a graph
     if isinstance(v, bool):
-        return torch.tensor(v, dtype=torch.bool, device=jr._device())
+        return torch.full((), v, dtype=torch.bool, device=jr._device())
     if isinstance(v, int):
-        return torch.tensor(v, dtype=torch.int32, device=jr._device())
+        return torch.full((), v, dtype=torch.int32, device=jr._device())
     if isinstance(v, float):
-        return torch.tensor(v, dtype=torch.float32, device=jr._device())
+        return torch.full((), v, dtype=torch.float32, device=jr._device())
     a = np.asarray(v)
     if a.dtype == np.float64:
         a = a.astype(np.float32)

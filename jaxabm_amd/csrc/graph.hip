@@ -5,8 +5,13 @@
 // Python iteration and a dozen un-jitted dispatches per step) by one graph
 // launch for the whole run.  Captured code records kernels and device copies
 // only: buffers that must start at zero are cleared by fill kernels, never by
-// memset nodes.
+// memset nodes.  abm_graph_sanitize enforces the same on a graph captured
+// from other code (a generic model step of torch ops): memset nodes become
+// fill-kernel nodes, and graphs with host, event or allocation nodes are
+// refused.
 #include <hip/hip_runtime.h>
+
+#include <vector>
 
 #include "common.hpp"
 #include "jaxabm_amd.h"
@@ -15,6 +20,81 @@ struct abm_graph {
   hipGraph_t graph;
   hipGraphExec_t exec;
 };
+
+namespace abm {
+
+// Stands in for a captured memset node: `height` rows of `width` elements of
+// `elem` bytes (1, 2 or 4) at dst + row * pitch, each set to `value`.
+__global__ void __launch_bounds__(256)
+    k_graph_fill(char *dst, size_t pitch, size_t width, size_t height, unsigned elem,
+                 unsigned value) {
+  const size_t total = width * height;
+  const size_t step = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += step) {
+    const size_t r = i / width, c = i - r * width;
+    char *p = dst + r * pitch + c * elem;
+    if (elem == 4)
+      *(uint32_t *)p = value;
+    else if (elem == 2)
+      *(uint16_t *)p = (uint16_t)value;
+    else
+      *(uint8_t *)p = (uint8_t)value;
+  }
+}
+
+const char *node_type_name(hipGraphNodeType t) {
+  switch (t) {
+    case hipGraphNodeTypeHost: return "host";
+    case hipGraphNodeTypeGraph: return "child graph";
+    case hipGraphNodeTypeWaitEvent: return "event wait";
+    case hipGraphNodeTypeEventRecord: return "event record";
+    case hipGraphNodeTypeMemAlloc: return "memory alloc";
+    case hipGraphNodeTypeMemFree: return "memory free";
+    case hipGraphNodeTypeMemcpyFromSymbol: return "memcpy from symbol";
+    case hipGraphNodeTypeMemcpyToSymbol: return "memcpy to symbol";
+    default: return "unsupported";
+  }
+}
+
+// Replaces one memset node by a k_graph_fill kernel node with the same
+// dependencies and dependents.
+int replace_memset(hipGraph_t g, hipGraphNode_t node) {
+  hipMemsetParams mp{};
+  ABM_CHECK_HIP(hipGraphMemsetNodeGetParams(node, &mp));
+  ABM_REQUIRE(mp.dst && (mp.elementSize == 1 || mp.elementSize == 2 || mp.elementSize == 4),
+              "abm_graph_sanitize: memset node with unsupported parameters");
+  const size_t height = mp.height ? mp.height : 1;
+  ABM_REQUIRE(height == 1 || mp.pitch >= mp.width * mp.elementSize,
+              "abm_graph_sanitize: memset node with pitch below its row width");
+  size_t ndep = 0, nout = 0;
+  ABM_CHECK_HIP(hipGraphNodeGetDependencies(node, nullptr, &ndep));
+  ABM_CHECK_HIP(hipGraphNodeGetDependentNodes(node, nullptr, &nout));
+  std::vector<hipGraphNode_t> deps(ndep), outs(nout);
+  if (ndep) ABM_CHECK_HIP(hipGraphNodeGetDependencies(node, deps.data(), &ndep));
+  if (nout) ABM_CHECK_HIP(hipGraphNodeGetDependentNodes(node, outs.data(), &nout));
+  char *dst = (char *)mp.dst;
+  size_t pitch = mp.pitch, width = mp.width, h = height;
+  unsigned elem = mp.elementSize, value = mp.value;
+  void *args[] = {&dst, &pitch, &width, &h, &elem, &value};
+  const size_t total = width * h;
+  size_t blocks = (total + 255) / 256;
+  if (blocks > 1024) blocks = 1024;
+  if (blocks == 0) blocks = 1;
+  hipKernelNodeParams kp{};
+  kp.blockDim = dim3(256);
+  kp.gridDim = dim3((unsigned)blocks);
+  kp.func = (void *)k_graph_fill;
+  kp.kernelParams = args;
+  kp.sharedMemBytes = 0;
+  kp.extra = nullptr;
+  hipGraphNode_t fill = nullptr;
+  ABM_CHECK_HIP(hipGraphAddKernelNode(&fill, g, ndep ? deps.data() : nullptr, ndep, &kp));
+  for (hipGraphNode_t o : outs) ABM_CHECK_HIP(hipGraphAddDependencies(g, &fill, &o, 1));
+  ABM_CHECK_HIP(hipGraphDestroyNode(node));
+  return kOk;
+}
+
+}  // namespace abm
 
 using namespace abm;
 
@@ -57,6 +137,40 @@ int abm_graph_destroy(abm_graph *g) {
   }
   delete g;
   if (e != hipSuccess) return hip_fail(e, "abm_graph_destroy");
+  return kOk;
+}
+
+int abm_graph_sanitize(void *graph, int32_t *counts) {
+  ABM_REQUIRE(graph && counts, "abm_graph_sanitize: null argument");
+  hipGraph_t g = (hipGraph_t)graph;
+  size_t n = 0;
+  ABM_CHECK_HIP(hipGraphGetNodes(g, nullptr, &n));
+  std::vector<hipGraphNode_t> nodes(n);
+  if (n) ABM_CHECK_HIP(hipGraphGetNodes(g, nodes.data(), &n));
+  int32_t c[4] = {0, 0, 0, 0};
+  std::vector<hipGraphNode_t> memsets;
+  for (hipGraphNode_t node : nodes) {
+    hipGraphNodeType t;
+    ABM_CHECK_HIP(hipGraphNodeGetType(node, &t));
+    if (t == hipGraphNodeTypeKernel) {
+      ++c[1];
+    } else if (t == hipGraphNodeTypeMemcpy) {
+      ++c[2];
+    } else if (t == hipGraphNodeTypeMemset) {
+      memsets.push_back(node);
+    } else if (t == hipGraphNodeTypeEmpty) {
+      ++c[3];
+    } else {
+      set_error(std::string("abm_graph_sanitize: the graph holds a ") + node_type_name(t) +
+                " node; only kernels, device copies and memsets are replayed");
+      return kErrInvalid;
+    }
+  }
+  for (hipGraphNode_t node : memsets) {
+    if (int rc = replace_memset(g, node)) return rc;
+    ++c[0];
+  }
+  for (int i = 0; i < 4; ++i) counts[i] = c[i];
   return kOk;
 }
 

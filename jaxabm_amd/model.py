@@ -46,6 +46,7 @@ class Model:
         self._time_step = 0
         self._history: List[Dict[str, Any]] = []
         self._is_initialized = False
+        self.last_run_info: Dict[str, Any] = {}
 
     def add_agent_collection(self, name: str, agent_collection: AgentCollection) -> None:
         if self._is_initialized:
@@ -100,17 +101,29 @@ class Model:
             self._history.append({"time_step": self._time_step, "metrics": metrics})
         return metrics
 
-    def run(self, steps: Optional[int] = None) -> Dict[str, List[Any]]:
+    def run(self, steps: Optional[int] = None, graph: bool = False) -> Dict[str, List[Any]]:
         """Initialise if needed, step, and return the history as columns
         ({'step': [...], metric: [...]}); a second run continues the state and
-        resets only the history (jaxabm/model.py:218-262)."""
+        resets only the history (jaxabm/model.py:218-262).
+
+        ``graph=True`` runs steps 2..n as replays of one captured hipGraph of
+        the step (same results; ``last_run_info`` says whether it was
+        replayed and why not); a step that reads device values on the host
+        runs eagerly as usual."""
         if not self._is_initialized:
             self.initialize()
         n = steps if steps is not None else self.config.steps
         if self.config.track_history:
             self._history = []
         t0 = time.time()
-        for _ in range(n):
+        done = 0
+        self.last_run_info = {"mode": "eager", "steps": n, "steps_done": 0}
+        if graph and n > 1:
+            from .graphrun import run_graph
+
+            self.last_run_info = run_graph(self, n)
+            done = self.last_run_info["steps_done"]
+        for _ in range(n - done):
             self.step()
         elapsed = time.time() - t0
         print(f"Ran {n} steps in {elapsed:.2f}s ({n / max(elapsed, 1e-12):.1f} steps/sec)")

@@ -63,6 +63,9 @@ def asarray(a, dtype=None) -> torch.Tensor:
         return t.to(device=_dev(), dtype=dtype or t.dtype)
     if isinstance(a, (list, tuple)) and a and any(isinstance(x, torch.Tensor) for x in a):
         return torch.stack([asarray(x, dtype) for x in a])
+    if isinstance(a, (bool, int, float)):
+        # a fill kernel, not a host copy (no host synchronisation)
+        return torch.full((), a, dtype=dtype or _default_dtype(a), device=_dev())
     if dtype is None:
         dtype = _default_dtype(a)
         if dtype is None:

@@ -253,8 +253,12 @@ def uniform(key, shape=(), dtype=torch.float32, minval=0.0, maxval=1.0, mode=Non
     k = _key_arg(key)
     if isinstance(minval, torch.Tensor) or isinstance(maxval, torch.Tensor):
         f = torch.ops.jaxabm.draw(k, list(_shape(shape)), 1, 0.0, 1.0, _mode(mode))
-        lo = torch.as_tensor(minval, dtype=torch.float32, device=f.device)
-        hi = torch.as_tensor(maxval, dtype=torch.float32, device=f.device)
+        def bound(v):
+            if isinstance(v, torch.Tensor):
+                return v.to(device=f.device, dtype=torch.float32)
+            return torch.full((), float(v), dtype=torch.float32, device=f.device)
+
+        lo, hi = bound(minval), bound(maxval)
         return torch.maximum(lo, f * (hi - lo) + lo)
     return torch.ops.jaxabm.draw(k, list(_shape(shape)), 1, float(minval), float(maxval),
                                  _mode(mode))

@@ -1,0 +1,229 @@
+"""GPU: generic model steps replayed as a hipGraph (Model.run(graph=True))
+against the eager loop of the same model (jaxabm/model.py:146-262).
+
+Every replay is checked: the history holds each step's metrics, and the
+replayed run must equal the eager run bit for bit at every step, with eager
+work and synchronisations between replays in one case.  The large model's
+full reductions span many workgroups, whose semaphores torch clears with a
+memset: the capture holds memset nodes that abm_graph_sanitize replaces by
+fill kernels."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def J():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import jaxabm_amd
+
+    return jaxabm_amd
+
+
+class Dummy:
+    def init_state(self, cfg, key):
+        from jaxabm_amd import random as jr
+
+        return {"value": jr.uniform(key, minval=0.0, maxval=10.0)}
+
+    def update(self, state, model_state, cfg, key):
+        return {"value": state["value"] + model_state["env"].get("increment", 1.0)}
+
+
+def counter_fn(env, agents, params, key):
+    new = dict(env)
+    new["counter"] = env.get("counter", 0) + 1
+    return new
+
+
+def core_metrics(env, agents, params):
+    from jaxabm_amd import numpy as jnp
+
+    return {"total_value": jnp.sum(agents["c"]["value"]), "step_counter": env.get("counter", 0)}
+
+
+class Trader:
+    """Per-agent position walk and a coin-flip wealth change, with a constant
+    output and the time step in the arithmetic."""
+
+    def init_state(self, cfg, key):
+        from jaxabm_amd import random as jr
+
+        k1, k2 = jr.split(key)
+        return {"x": jr.uniform(k1), "w": jr.randint(k2, (), 0, 6), "flag": 1}
+
+    def update(self, state, model_state, cfg, key):
+        from jaxabm_amd import numpy as jnp
+        from jaxabm_amd import random as jr
+
+        k1, k2 = jr.split(key)
+        x = jnp.mod(state["x"] + jr.normal(k1) * 0.01, 1.0)
+        give = jr.uniform(k2) < 0.5
+        w = jnp.where(give & (state["w"] > 0), state["w"] - 1, state["w"] + 1)
+        w = w + (model_state["time_step"] % 2)
+        return {"x": x, "w": w, "flag": 1}
+
+
+def trader_env(env, agents, params, key):
+    from jaxabm_amd import numpy as jnp
+    from jaxabm_amd import random as jr
+
+    new = dict(env)
+    new["mean_x"] = jnp.mean(agents["t"]["x"])
+    new["shock"] = jr.uniform(key) * params["scale"]
+    new["t"] = env.get("t", 0) + 1
+    return new
+
+
+def trader_metrics(env, agents, params):
+    from jaxabm_amd import numpy as jnp
+
+    a = agents["t"]
+    return {"mean_x": env["mean_x"], "total_w": jnp.sum(a["w"]), "max_x": jnp.max(a["x"]),
+            "rich": jnp.sum(a["w"] > 3), "shock": env["shock"], "t": env["t"]}
+
+
+def make_core(J, seed=42):
+    m = J.JaxModel(params={}, config=J.ModelConfig(steps=20, seed=seed),
+                   update_state_fn=counter_fn, metrics_fn=core_metrics)
+    m.add_agent_collection("c", J.AgentCollection(Dummy(), 10))
+    m.add_env_state("increment", 1.0)
+    return m
+
+
+def make_trader(J, n=262144, seed=7, interval=1, metrics=trader_metrics, env_fn=trader_env):
+    m = J.JaxModel(params={"scale": 0.5},
+                   config=J.ModelConfig(steps=30, seed=seed, collect_interval=interval),
+                   update_state_fn=env_fn, metrics_fn=metrics)
+    m.add_agent_collection("t", J.AgentCollection(Trader(), n))
+    return m
+
+
+def _host(v):
+    import torch
+
+    if isinstance(v, torch.Tensor):
+        return v.detach().cpu().numpy()
+    return np.asarray(v)
+
+
+def assert_same_run(a, b, ra, rb):
+    assert list(ra) == list(rb)
+    for k in ra:
+        assert len(ra[k]) == len(rb[k]), k
+        for i, (x, y) in enumerate(zip(ra[k], rb[k])):
+            assert type(x) is type(y) or (hasattr(x, "shape") and hasattr(y, "shape")), (k, i)
+            np.testing.assert_array_equal(_host(x), _host(y), err_msg=f"{k} step {i}")
+    for name in a.agent_collections:
+        sa, sb = a.agent_collections[name].states, b.agent_collections[name].states
+        assert set(sa) == set(sb)
+        for k in sa:
+            assert sa[k].dtype == sb[k].dtype
+            np.testing.assert_array_equal(_host(sa[k]), _host(sb[k]), err_msg=f"{name}.{k}")
+    assert set(a._env_state) == set(b._env_state)
+    for k in a._env_state:
+        x, y = a._env_state[k], b._env_state[k]
+        assert type(x) is type(y) or (hasattr(x, "shape") and hasattr(y, "shape")), k
+        np.testing.assert_array_equal(_host(x), _host(y), err_msg=f"env {k}")
+    np.testing.assert_array_equal(a._rng, b._rng)
+    assert a._time_step == b._time_step
+
+
+def test_core_model_graph_matches_eager(J, capsys):
+    a, b = make_core(J), make_core(J)
+    ra = a.run(20)
+    rb = b.run(20, graph=True)
+    info = b.last_run_info
+    print(info)
+    assert info["mode"] == "graph" and info["steps_done"] == 20
+    assert_same_run(a, b, ra, rb)
+    assert b._env_state["counter"] == 20 and isinstance(b._env_state["counter"], int)
+    assert rb["step_counter"] == list(range(1, 21))
+    # a second run continues from the replayed state
+    assert_same_run(a, b, a.run(7), b.run(7, graph=True))
+
+
+def test_large_model_graph_matches_eager_every_replay(J):
+    """262,144 agents: draws under vmap, multi-workgroup reductions (memset
+    nodes replaced), env callback with its own key; eager work and a
+    synchronisation between replays."""
+    import torch
+
+    from jaxabm_amd.graphrun import run_graph
+
+    a, b = make_trader(J), make_trader(J)
+    ra = a.run(30)
+    b.initialize()
+    b._history = []
+    scratch = torch.rand(1 << 22, device="cuda")
+
+    def between():
+        scratch.mul_(1.0001).sum()
+        torch.cuda.synchronize()
+
+    info = run_graph(b, 30, between=between)
+    print(info)
+    assert info["mode"] == "graph" and info["steps_done"] == 30
+    assert info["kernel_nodes"] > 0
+    assert info["memsets_replaced"] > 0, "the reductions should have recorded memset nodes"
+    out = {"step": [h["time_step"] for h in b._history]}
+    for k in b._history[0]["metrics"]:
+        out[k] = [h["metrics"][k] for h in b._history]
+    ra2 = {k: v for k, v in ra.items()}
+    assert_same_run(a, b, ra2, out)
+    assert out["t"] == list(range(1, 31))
+
+
+def test_collect_interval_and_continuation(J):
+    a, b = make_trader(J, n=5000, interval=3), make_trader(J, n=5000, interval=3)
+    assert_same_run(a, b, a.run(10), b.run(10, graph=True))
+    assert b.last_run_info["mode"] == "graph"
+    assert_same_run(a, b, a.run(8), b.run(8, graph=True))
+
+
+def test_host_read_in_step_runs_eagerly(J):
+    """A metric read on the host (float()) cannot be replayed: step 2 (run
+    with synchronising calls raising) stops the replay and the run continues
+    eagerly, with the same results."""
+    from jaxabm_amd import numpy as jnp
+
+    def host_metrics(env, agents, params):
+        return {"mean_x": float(jnp.mean(agents["t"]["x"]))}
+
+    a = make_trader(J, n=4096, metrics=host_metrics)
+    b = make_trader(J, n=4096, metrics=host_metrics)
+    ra, rb = a.run(6), b.run(6, graph=True)
+    info = b.last_run_info
+    assert info["mode"] == "eager" and info["steps_done"] == 1 and "step 2" in info["reason"]
+    assert_same_run(a, b, ra, rb)
+
+
+def test_python_value_changing_between_steps_runs_eagerly_after_step_1(J):
+    """An env value computed from Python-side state changes every step: it
+    would be a constant of the graph, so step 2's result is discarded and
+    steps 2..n run eagerly.  Step 2's Python code ran once more (the counter
+    below is one ahead of the eager run's); the device state is that of the
+    eager run."""
+
+    def make():
+        calls = {"n": 0}
+
+        def env_fn(env, agents, params, key):
+            calls["n"] += 1
+            new = trader_env(env, agents, params, key)
+            new["label"] = "odd" if calls["n"] % 2 else "even"
+            return new
+
+        return make_trader(J, n=4096, env_fn=env_fn), calls
+
+    (a, ca), (b, cb) = make(), make()
+    ra, rb = a.run(5), b.run(5, graph=True)
+    info = b.last_run_info
+    assert info["mode"] == "eager" and info["steps_done"] == 1 and "step 2" in info["reason"]
+    assert (ca["n"], cb["n"]) == (5, 6)
+    assert (a._env_state.pop("label"), b._env_state.pop("label")) == ("odd", "even")
+    assert_same_run(a, b, ra, rb)
