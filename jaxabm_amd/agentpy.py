@@ -598,12 +598,17 @@ class Model:
         """Every parameter combination, ``repetitions`` runs with seeds
         seed + rep (jaxabm/agentpy.py:1116-1152)."""
         names = list(parameter_ranges.keys())
-        results = {}
-        for values in itertools.product(*parameter_ranges.values()):
-            params = {**self.p, **dict(zip(names, values))}
-            results[tuple(values)] = [type(self)(params, seed=self.seed + rep).run()
-                                      for rep in range(repetitions)]
-        return results
+        combos = list(itertools.product(*parameter_ranges.values()))
+        models = [type(self)({**self.p, **dict(zip(names, values))}, seed=self.seed + rep)
+                  for values in combos for rep in range(repetitions)]
+        from .programs.library import run_models, servable
+
+        if len(models) > 1 and all(servable(m) for m in models):
+            runs = run_models(models)   # one continuous batch, same results
+        else:
+            runs = [m.run() for m in models]
+        return {tuple(values): runs[i * repetitions:(i + 1) * repetitions]
+                for i, values in enumerate(combos)}
 
 
 class Parameter:

@@ -41,6 +41,18 @@ def _last(results, metric):
     return float(v)
 
 
+def _servable(model) -> bool:
+    from .programs.library import servable
+
+    return servable(model)
+
+
+def _run_models(models, steps=None):
+    from .programs.library import run_models
+
+    return run_models(models, steps)
+
+
 def _pyplot():
     try:
         import matplotlib.pyplot as plt
@@ -76,17 +88,33 @@ class SensitivityAnalysis:
         return samples
 
     def run(self, verbose: bool = True) -> Dict[str, np.ndarray]:
+        """One run per sample with seed 1000 + i.  Library models with a
+        server request (wealth, network SIR) run as one continuous batch,
+        with the results of running them one after another."""
         if verbose:
             print(f"Running sensitivity analysis with {self.num_samples} samples...")
         names = list(self.param_ranges.keys())
         out = {m: np.zeros(self.num_samples, np.float32) for m in self.metrics_of_interest}
+        sample_params = [{p: float(self.samples[i, j]) for j, p in enumerate(names)}
+                         for i in range(self.num_samples)]
+        batch = None
+        if self.num_samples > 1:
+            first = self.model_factory(params=sample_params[0], config=ModelConfig(seed=1000))
+            if _servable(first):
+                models = [first] + [self.model_factory(params=sample_params[i],
+                                                       config=ModelConfig(seed=i + 1000))
+                                    for i in range(1, self.num_samples)]
+                batch = _run_models(models)
         for i in range(self.num_samples):
-            params = {p: float(self.samples[i, j]) for j, p in enumerate(names)}
+            params = sample_params[i]
             if verbose:
                 print(f"\nSample {i+1}/{self.num_samples}")
                 print("Parameters: " + ", ".join(f"{k}={v:.4f}" for k, v in params.items()))
-            model = self.model_factory(params=params, config=ModelConfig(seed=i + 1000))
-            results = model.run()
+            if batch is not None:
+                results = batch[i]
+            else:
+                model = self.model_factory(params=params, config=ModelConfig(seed=i + 1000))
+                results = model.run()
             for m in self.metrics_of_interest:
                 v = _last(results, m)
                 if v is not None:
@@ -211,24 +239,36 @@ class ModelCalibrator:
                 total += (abs(metrics[m] - target) / (abs(target) + 1e-8)) ** 2
         return float(total)
 
-    def _metrics(self, params: Dict[str, float], seed: int) -> Dict[str, float]:
-        model = self.model_factory(params=params, config=ModelConfig(seed=seed))
-        results = model.run(steps=self.evaluation_steps)
-        out = {}
-        for m in self.target_metrics:
-            v = _last(results, m)
-            out[m] = 0.0 if v is None else v
+    def _metrics_many(self, runs: List[Tuple[Dict[str, float], int]]) -> List[Dict[str, float]]:
+        """Target metrics of each (params, seed) run after evaluation_steps
+        steps.  Library models with a server request run as one continuous
+        batch (same results as one after another)."""
+        models = [self.model_factory(params=p, config=ModelConfig(seed=sd)) for p, sd in runs]
+        if len(models) > 1 and all(_servable(m) for m in models):
+            results = _run_models(models, self.evaluation_steps)
+        else:
+            results = [m.run(steps=self.evaluation_steps) for m in models]
+        out = []
+        for res in results:
+            d = {}
+            for m in self.target_metrics:
+                v = _last(res, m)
+                d[m] = 0.0 if v is None else v
+            out.append(d)
         return out
 
-    def _evaluate_params_robust(self, params):
-        """Mean metrics over num_evaluation_runs seeds drawn as
-        randint(sub, (), 0, 1_000_000); normalized loss and 95% intervals."""
-        runs = {m: [] for m in self.target_metrics}
+    def _metrics(self, params: Dict[str, float], seed: int) -> Dict[str, float]:
+        return self._metrics_many([(params, seed)])[0]
+
+    def _robust_seeds(self) -> List[int]:
+        seeds = []
         for _ in range(self.num_evaluation_runs):
             self.key, sub = jr.host_split(self.key, 2)
-            seed = int(jr.randint(sub, (), 0, 1_000_000).item())
-            for m, v in self._metrics(params, seed).items():
-                runs[m].append(v)
+            seeds.append(int(jr.randint(sub, (), 0, 1_000_000).item()))
+        return seeds
+
+    def _robust_summary(self, metric_runs: List[Dict[str, float]]):
+        runs = {m: [r[m] for r in metric_runs] for m in self.target_metrics}
         mean, ci = {}, {}
         for m, vals in runs.items():
             a = np.asarray(vals, np.float64)
@@ -236,6 +276,12 @@ class ModelCalibrator:
             half = 1.96 * float(a.std()) / np.sqrt(len(a))
             ci[m] = (mean[m] - half, mean[m] + half)
         return self._compute_normalized_loss(mean), ci
+
+    def _evaluate_params_robust(self, params):
+        """Mean metrics over num_evaluation_runs seeds drawn as
+        randint(sub, (), 0, 1_000_000); normalized loss and 95% intervals."""
+        seeds = self._robust_seeds()
+        return self._robust_summary(self._metrics_many([(params, sd) for sd in seeds]))
 
     def _clip(self, params: Dict[str, float]) -> Dict[str, float]:
         return {k: float(np.clip(v, *self.param_bounds.get(k, (-np.inf, np.inf))))
@@ -254,11 +300,14 @@ class ModelCalibrator:
                                     steps=self.evaluation_steps)
             return np.array([g[k] for k in self._names], np.float64)
         g = np.zeros(len(self._names))
-        for i, k in enumerate(self._names):
+        hs, runs = [], []
+        for k in self._names:
             h = self.fd_step * max(1.0, abs(params[k]))
-            lp = self.loss_fn({**params, k: params[k] + h})
-            lm = self.loss_fn({**params, k: params[k] - h})
-            g[i] = (lp - lm) / (2 * h)
+            hs.append(h)
+            runs += [({**params, k: params[k] + h}, 42), ({**params, k: params[k] - h}, 42)]
+        losses = [self._compute_loss(m) for m in self._metrics_many(runs)]
+        for i, h in enumerate(hs):
+            g[i] = (losses[2 * i] - losses[2 * i + 1]) / (2 * h)
         return g
 
     # -- optimisers
@@ -304,8 +353,15 @@ class ModelCalibrator:
         return jr.uniform(sub, shape).double().cpu().numpy()
 
     def _population_loss(self, pop: np.ndarray) -> np.ndarray:
-        return np.array([self._evaluate_params_robust(self._clip(dict(zip(self._names, x))))[0]
-                         for x in pop])
+        """Robust loss of every member; the seeds are drawn member by member
+        as the serial loop draws them, and all member x seed runs go out as
+        one batch."""
+        members = [self._clip(dict(zip(self._names, x))) for x in pop]
+        seeds = [self._robust_seeds() for _ in members]
+        flat = self._metrics_many([(p, sd) for p, ss in zip(members, seeds) for sd in ss])
+        k = self.num_evaluation_runs
+        return np.array([self._robust_summary(flat[i * k:(i + 1) * k])[0]
+                         for i in range(len(members))])
 
     def _bounds_arrays(self):
         lo = np.array([self.param_bounds.get(k, (-np.inf, np.inf))[0] for k in self._names])

@@ -45,6 +45,16 @@ class WealthModel(_ProgramModel):
 
     defaults = {"n": 1000, "steps": 100}
 
+    def serve_request(self, steps: Optional[int] = None):
+        """The same run as a request to the batching server."""
+        from ..serving import SimRequest
+
+        steps = int(steps if steps is not None else self.p.get("steps", 100))
+        return SimRequest("wealth", n=int(self.p["n"]), steps=steps, seed=int(self.seed))
+
+    def results_from_records(self, steps: int, rec: Dict[str, Any], t0: float) -> Results:
+        return self._results(steps, {"mean_wealth": rec["mean"], "gini": rec["gini"]}, t0)
+
     def run(self, steps: Optional[int] = None) -> Results:
         from .wealth import WealthEngine
 
@@ -79,6 +89,19 @@ class NetworkSIRModel(_ProgramModel):
 
     defaults = {"n": 1_000_000, "m": 4, "beta": 0.05, "gamma": 0.1, "n_infected": 100,
                 "graph_seed": 0, "steps": 365}
+
+    def serve_request(self, steps: Optional[int] = None):
+        """The same run as a request to the batching server."""
+        from ..serving import SimRequest
+
+        steps = int(steps if steps is not None else self.p.get("steps", 365))
+        return SimRequest("sir", n=int(self.p["n"]), steps=steps, seed=int(self.seed),
+                          graph_seed=int(self.p["graph_seed"]), m=int(self.p["m"]),
+                          n_infected=int(self.p["n_infected"]), beta=float(self.p["beta"]),
+                          gamma=float(self.p["gamma"]))
+
+    def results_from_records(self, steps: int, rec: Dict[str, Any], t0: float) -> Results:
+        return self._results(steps, {"S": rec["S"], "I": rec["I"], "R": rec["R"]}, t0)
 
     def run(self, steps: Optional[int] = None) -> Results:
         from ..network import ba_csr
@@ -138,3 +161,40 @@ class DeffuantOpinionModel(_ProgramModel):
         eng.forward([self.p["eps"]], [self.p["mu"]], target, weights)
         ge, gm = eng.backward()
         return {"eps": float(ge[0]), "mu": float(gm[0])}
+
+
+def servable(model) -> bool:
+    """Whether ``model`` runs as a request of the batching server."""
+    return callable(getattr(model, "serve_request", None)) and \
+        callable(getattr(model, "results_from_records", None))
+
+
+def run_models(models, steps: Optional[int] = None, config: Optional[Dict[str, Any]] = None):
+    """``[m.run(steps) for m in models]`` with the same results, where the
+    models that have a server request (wealth, network SIR) are submitted
+    together to one continuous-batching server instead of run one after
+    another (the serial loops of jaxabm/analysis.py:97-165, 434-476,
+    jaxabm/utils.py:128-176 and jaxabm/agentpy.py:1116-1152).  A request the
+    server fails (e.g. larger than its memory budget) runs on its own."""
+    from .. import random as jr
+    from ..serving import FAILED, SimulationServer, tuned_config
+
+    out: list = [None] * len(models)
+    reqs = {i: m.serve_request(steps) for i, m in enumerate(models) if servable(m)}
+    if len(reqs) >= 2:
+        cfg = tuned_config("batched")
+        cfg["mode"] = jr.default_mode()
+        cfg.update(config or {})
+        t0 = time.time()
+        with SimulationServer(cfg) as server:
+            tickets = {i: server.submit(r) for i, r in reqs.items()}
+            server.drain()
+            for i, t in tickets.items():
+                if server.info(t).status != FAILED:
+                    out[i] = models[i].results_from_records(reqs[i].steps, server.result(t), t0)
+        print(f"{sum(o is not None for o in out)} simulations executed together in "
+              f"{format_time(time.time() - t0)}")
+    for i, m in enumerate(models):
+        if out[i] is None:
+            out[i] = m.run(steps)
+    return out

@@ -60,10 +60,23 @@ def standardize_metrics(metrics: Dict[str, Any]) -> Dict[str, float]:
 def run_parallel_simulations(model_factory: Callable[..., Any], param_sets: List[Dict[str, Any]],
                              num_runs: int = 1, seed_offset: int = 0) -> List[Dict[str, Any]]:
     """Every parameter set num_runs times with seed seed_offset + i * num_runs
-    + j (jaxabm/utils.py:128-176).  Library-program requests go through the
-    batching server instead (``jaxabm_amd.serving``)."""
+    + j (jaxabm/utils.py:128-176).  Library models with a server request
+    (wealth, network SIR) run as one continuous batch on the batching server
+    (``programs.library.run_models``), with the same results."""
     from .core import ModelConfig
+    from .programs.library import run_models, servable
 
+    jobs = [(i, j, params, seed_offset + i * num_runs + j)
+            for i, params in enumerate(param_sets) for j in range(num_runs)]
+    if len(jobs) > 1:
+        models = [model_factory(params=p, config=ModelConfig(seed=sd)) for _, _, p, sd in jobs]
+        if all(servable(m) for m in models):
+            out = []
+            for (_, _, params, seed), results in zip(jobs, run_models(models)):
+                results["params"] = params
+                results["seed"] = seed
+                out.append(results)
+            return out
     all_results = []
     for i, params in enumerate(param_sets):
         for j in range(num_runs):
