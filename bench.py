@@ -328,9 +328,11 @@ def _trader_metrics(env, agents, params):
 
 def bench_generic(args) -> dict:
     """A generic model (no library kernel): 10,000 agents, 200 steps, per-agent
-    torch ops under vmap, an env callback and three record-time reductions.
-    One bench step = one 200-step run, eager loop against graph replays
-    (Model.run(graph=True)); results compared bit for bit."""
+    torch ops, an env callback and three record-time reductions.  One bench
+    step = one 200-step run.  Four ways, same results (compared bit for bit):
+    eager loop or graph replays (Model.run(graph=True)), each with the agent
+    update vmapped (one kernel per primitive) or traced into one generated
+    kernel (jaxabm_amd.fuse, the default); ``value`` is graph + fused."""
     import contextlib
     import io
 
@@ -338,6 +340,7 @@ def bench_generic(args) -> dict:
     import torch
 
     import jaxabm_amd as J
+    from jaxabm_amd import fuse
 
     n, S = 10_000, 200
 
@@ -349,28 +352,39 @@ def bench_generic(args) -> dict:
         return m
 
     stream = torch.cuda.current_stream()
-    eager, graph = make(), make()
+    runs = max(1, min(args.steps, 5))
+    out, models = {}, {}
     quiet = io.StringIO()
-    with contextlib.redirect_stdout(quiet):
-        for _ in range(args.warmup):
-            eager.run(S)
-            graph.run(S, graph=True)
-        runs = max(1, min(args.steps, 5))
-        t_eager = _timed(stream, lambda: eager.run(S), runs)
-        t_graph = _timed(stream, lambda: graph.run(S, graph=True), runs)
-    info = graph.last_run_info
-    same = all(torch.equal(eager.agent_collections["t"].states[k],
-                           graph.agent_collections["t"].states[k]) for k in ("x", "w"))
-    same = same and bool(np.array_equal(eager._rng, graph._rng))
+    for fused in (False, True):
+        fuse.enable(fused)
+        try:
+            for graph in (False, True):
+                m = make()
+                with contextlib.redirect_stdout(quiet):
+                    for _ in range(args.warmup):
+                        m.run(S, graph=graph)
+                    t = _timed(stream, lambda: m.run(S, graph=graph), runs)
+                name = ("graph" if graph else "eager") + ("_fused" if fused else "_vmap")
+                out[name] = S / t
+                models[name] = m
+        finally:
+            fuse.enable(True)
+    ref = models["eager_vmap"]
+    same = all(
+        all(torch.equal(ref.agent_collections["t"].states[k], m.agent_collections["t"].states[k])
+            for k in ("x", "w")) and bool(np.array_equal(ref._rng, m._rng))
+        for m in models.values())
+    t_best = S / out["graph_fused"]
     return {
-        "metric": "steps/s (generic model, 10,000 agents, graph replays)", "value": S / t_graph,
-        "unit": "steps/s", "n_gpus": 1, "steps": runs, "warmup": args.warmup,
-        "ms_per_step": t_graph * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32+i32", "data": "synthetic (seeded)",
+        "metric": "steps/s (generic model, 10,000 agents, graph replays of the fused update)",
+        "value": out["graph_fused"], "unit": "steps/s", "n_gpus": 1, "steps": runs,
+        "warmup": args.warmup, "ms_per_step": t_best * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+i32", "data": "synthetic (seeded)",
         "config": {"workload": "generic_trader_10000agents_200steps", "n_agents": n,
                    "sim_steps": S, "bench_step": "one 200-step Model.run"},
-        "eager_steps_per_s": S / t_eager, "speedup_vs_eager": t_eager / t_graph,
-        "identical_results": same, "graph_info": info,
+        "steps_per_s": out, "speedup_vs_eager_vmap": out["graph_fused"] / out["eager_vmap"],
+        "identical_results": same, "graph_info": models["graph_fused"].last_run_info,
+        "fuse": fuse.stats(),
     }
 
 

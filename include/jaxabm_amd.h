@@ -348,6 +348,21 @@ int abm_graph_sanitize(void *graph, int32_t *counts);
 /* Stream-ordered device-to-device copy (capturable). */
 int abm_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
 
+/* ---- generated agent-update kernels (replace the jax.vmap of a per-agent
+ *      function in AgentCollection.update, jaxabm/agent.py:156-177, which XLA
+ *      fuses) ---- */
+/* Compiles kernel source text for `arch` (e.g. "gfx950") with hiprtc; *code
+ * (free with abm_rtc_code_free) receives the code object.  Needs no GPU. */
+int abm_rtc_compile(const char *src, const char *arch, void **code, size_t *size);
+int abm_rtc_code_free(void *code);
+/* Loads a code object and finds `kernel` in it. */
+int abm_rtc_load(const void *code, const char *kernel, void **handle);
+int abm_rtc_unload(void *handle);
+/* Launches a loaded kernel whose one parameter is a struct of args_bytes
+ * bytes (64-bit words) on `stream`. */
+int abm_rtc_launch(void *handle, uint32_t grid, uint32_t block, const void *args,
+                   size_t args_bytes, void *stream);
+
 /* ---- timing on the stream the kernels run on ---- */
 int abm_event_timer(void *stream, int phase, float *ms_out);
 

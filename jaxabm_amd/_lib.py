@@ -179,6 +179,11 @@ SIGNATURES = {
     "abm_graph_launch": (_int, [_p, _p]),
     "abm_graph_destroy": (_int, [_p]),
     "abm_graph_sanitize": (_int, [_p, _p]),
+    "abm_rtc_compile": (_int, [C.c_char_p, C.c_char_p, C.POINTER(_p), C.POINTER(C.c_size_t)]),
+    "abm_rtc_code_free": (_int, [_p]),
+    "abm_rtc_load": (_int, [_p, C.c_char_p, C.POINTER(_p)]),
+    "abm_rtc_unload": (_int, [_p]),
+    "abm_rtc_launch": (_int, [_p, _u32, _u32, _p, C.c_size_t, _p]),
     "abm_memcpy_d2d": (_int, [_p, _p, _sz, _p]),
     "abm_event_timer": (_int, [_p, _int, C.POINTER(_f32)]),
 }

@@ -2,10 +2,13 @@
 
 A collection holds its agents' state as a dict of GPU tensors of shape
 (num_agents, ...) (structure of arrays).  ``init`` and ``update`` map the
-agent type's per-agent functions over the agents with ``torch.func.vmap``,
-as the reference does with ``jax.vmap`` (jaxabm/agent.py:115-177), with the
-same key schedule: ``split(key, num_agents)`` gives agent i its key, and the
-draws of ``jaxabm_amd.random`` under vmap reproduce JAX's vmapped bits.
+agent type's per-agent functions over the agents, as the reference does with
+``jax.vmap`` (jaxabm/agent.py:115-177), with the same key schedule:
+``split(key, num_agents)`` gives agent i its key, and the draws of
+``jaxabm_amd.random`` reproduce JAX's vmapped bits.  A function of per-agent
+scalars, env scalars and draws runs as one generated kernel
+(``jaxabm_amd.fuse``); anything else runs under ``torch.func.vmap``, with
+the same results.
 """
 from __future__ import annotations
 
@@ -93,6 +96,13 @@ class AgentCollection:
         init_method = getattr(self.agent_type, "init_state", None)
         if not callable(init_method):
             raise AttributeError(f"Agent type {self._name()} must implement 'init_state'")
+        from . import fuse
+
+        out = fuse.run_agents(init_method, self.num_agents, key, jr.default_mode(), None, None,
+                              lambda s, ms, k: init_method(model_config, k))
+        if out is not None:
+            self._states = out
+            return
         agent_keys = jr.split(key, self.num_agents)
         self._states = vmap_agents(lambda k: init_method(model_config, k), agent_keys,
                                    n=self.num_agents)
@@ -107,6 +117,14 @@ class AgentCollection:
         update_method = getattr(self.agent_type, "update", None)
         if not callable(update_method):
             raise AttributeError(f"Agent type {self._name()} must implement 'update'")
+        from . import fuse
+
+        out = fuse.run_agents(update_method, self.num_agents, key, jr.default_mode(),
+                              self._states, model_state,
+                              lambda s, ms, k: update_method(s, ms, model_config, k))
+        if out is not None:
+            self._states = out
+            return
         agent_keys = jr.split(key, self.num_agents)
         self._states = vmap_agents(
             lambda s, k: update_method(s, model_state, model_config, k), self._states, agent_keys,

@@ -1,0 +1,954 @@
+"""Per-agent functions traced into one generated HIP kernel.
+
+The reference maps ``AgentType.init_state`` / ``update`` over the agents with
+``jax.vmap`` inside ``AgentCollection.init`` / ``update``
+(jaxabm/agent.py:92-177), and XLA fuses the mapped program.  Run through
+``torch.func.vmap`` instead, every primitive of the function is a separate
+kernel over all agents (a 6-op update with two draws is about 20 launches).
+Here the function is run once on symbolic per-agent values, the operations
+are recorded, and one kernel is generated in which thread i computes agent
+i's whole update.  That kernel is compiled at run time with hiprtc
+(``abm_rtc_compile``), cached by its source text, and launched on the
+current stream, so it is also captured by ``Model.run(graph=True)``.
+
+Semantics are those of the vmapped torch program, bit for bit except the
+sign bit of a NaN result: torch subtracts as ``a + (-1) * b`` with a run-time
+multiply, the kernel as ``a - b``, whose negate modifier flips a NaN
+operand's sign.
+
+* Result dtypes come from ``torch.result_type`` on stand-ins with the same
+  roles: a per-agent value is a 1-D tensor (it is batched under vmap), a 0-d
+  env tensor is 0-d, and a Python scalar (env entries, ``time_step``,
+  literals) is a weak Python scalar.  Arithmetic between Python scalars is
+  done as Python does it (float64, int64 with floored ``//`` and ``%``).
+* Every operation rounds where the torch op would: the kernel is compiled
+  without contraction into fused multiply-adds, literals keep their exact
+  float64 bits and are converted to float32 where torch converts them, and
+  ``remainder``, ``floor_divide``, ``clamp``, ``minimum``, ``maximum`` and
+  ``sign`` follow torch's element functions.  A tensor divided by a Python
+  scalar is multiplied by the scalar's reciprocal and a Python scalar
+  divided by a tensor is ``reciprocal(tensor) * scalar``, as torch computes
+  them.
+* Draws are the ``abm_device.hpp`` functions the vmapped draw kernels call:
+  agent i's key is ``split(key, n)[i]``; ``split``, ``bits``, ``uniform``,
+  ``normal``, ``bernoulli`` and ``randint`` on it give JAX's bits.
+
+Anything else is not traced.  That includes reading other agents' arrays,
+per-agent vectors, reductions, Python control flow on a traced value and
+host reads; ``Unsupported`` is raised, and the collection runs the vmapped
+path, which is remembered per function and input signature.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import hashlib
+import math
+import os
+import struct
+from typing import Any, Dict, List, Optional, Tuple
+
+import numpy as np
+import torch
+
+_ENABLED = True
+_ARCH = os.environ.get("JAXABM_AMD_ARCH", "gfx950")
+_PRELUDE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc",
+                             "abm_device.hpp")
+
+
+def enable(flag: bool = True) -> None:
+    """Turn the fused path on or off (on by default)."""
+    global _ENABLED
+    _ENABLED = bool(flag)
+
+
+def enabled() -> bool:
+    return _ENABLED
+
+
+class Unsupported(Exception):
+    """The function uses something the generated kernel does not cover."""
+
+
+# kinds: tensor dtypes, and the three Python scalar types (weak)
+_CT = {"f32": "float", "f64": "double", "i32": "int32_t", "i64": "int64_t", "u8": "uint8_t",
+       "bool": "bool", "pyf": "double", "pyi": "int64_t", "pyb": "bool"}
+_TORCH = {"f32": torch.float32, "f64": torch.float64, "i32": torch.int32, "i64": torch.int64,
+          "u8": torch.uint8, "bool": torch.bool}
+_KIND = {v: k for k, v in _TORCH.items()}
+_WEAK = ("pyf", "pyi", "pyb")
+_FLOATS = ("f32", "f64", "pyf")
+_DEFAULT = {"pyf": "f32", "pyi": "i32", "pyb": "bool"}   # JAX's defaults for Python scalars
+
+
+def _lit(v, kind: str) -> str:
+    """C expression of the Python value v converted to kind, as torch
+    converts a Python scalar operand (through float64 for floats)."""
+    ct = _CT[kind]
+    if isinstance(v, bool):
+        return f"(({ct}){'true' if v else 'false'})"
+    if isinstance(v, int):
+        if not -(1 << 63) <= v < (1 << 63):
+            raise Unsupported("integer literal out of int64 range")
+        return f"(({ct})({v}LL))" if v != -(1 << 63) else f"(({ct})(-9223372036854775807LL - 1))"
+    if isinstance(v, float):
+        if math.isnan(v):
+            return f"(({ct})__longlong_as_double(0x7ff8000000000000LL))"
+        if math.isinf(v):
+            return f"(({ct})__longlong_as_double({'0x7ff0000000000000LL' if v > 0 else '(long long)0xfff0000000000000ULL'}))"
+        return f"(({ct}){float(v).hex()})"
+    raise Unsupported(f"literal of type {type(v).__name__}")
+
+
+class Sym:
+    """A traced value: one scalar per agent (varying) or one for all agents
+    (uniform: an env value, a literal, a draw-free expression of those)."""
+
+    _abm_sym = True
+    __slots__ = ("tr", "name", "kind", "varying")
+
+    def __init__(self, tr: "_Trace", name: str, kind: str, varying: bool):
+        self.tr, self.name, self.kind, self.varying = tr, name, kind, varying
+
+    # -- what torch sees
+    @property
+    def weak(self) -> bool:
+        return self.kind in _WEAK
+
+    @property
+    def dtype(self):
+        if self.weak:
+            raise Unsupported("dtype of a Python scalar")
+        return _TORCH[self.kind]
+
+    @property
+    def shape(self):
+        return torch.Size(())
+
+    def dim(self):
+        return 0
+
+    def size(self, *a):
+        return torch.Size(())
+
+    def is_floating_point(self):
+        return self.kind in _FLOATS
+
+    def as_array(self) -> "Sym":
+        """jnp.asarray of a Python scalar: a 0-d tensor of JAX's default dtype."""
+        if not self.weak:
+            return self
+        return self.tr.cast(self, _DEFAULT[self.kind])
+
+    def to(self, *args, **kwargs):
+        dt = kwargs.get("dtype")
+        for a in args:
+            if isinstance(a, torch.dtype):
+                dt = a
+        if dt is None:
+            return self
+        if dt not in _KIND:
+            raise Unsupported(f"cast to {dt}")
+        return self.tr.cast(self, _KIND[dt])
+
+    def astype(self, dt):
+        return self.to(dt)
+
+    def float(self):
+        return self.to(torch.float32)
+
+    def double(self):
+        return self.to(torch.float64)
+
+    def int(self):
+        return self.to(torch.int32)
+
+    def long(self):
+        return self.to(torch.int64)
+
+    def bool(self):
+        return self.to(torch.bool)
+
+    def clone(self):
+        return self
+
+    def contiguous(self):
+        return self
+
+    def detach(self):
+        return self
+
+    def reshape(self, *shape):
+        if shape in ((), ((),)) or shape == ((1,),) or shape == (1,):
+            return self
+        raise Unsupported("reshape of a per-agent scalar")
+
+    def __bool__(self):
+        raise Unsupported("Python control flow on a traced value")
+
+    def __int__(self):
+        raise Unsupported("int() of a traced value")
+
+    def __float__(self):
+        raise Unsupported("float() of a traced value")
+
+    def __index__(self):
+        raise Unsupported("indexing with a traced value")
+
+    def item(self):
+        raise Unsupported("item() of a traced value")
+
+    def __len__(self):
+        raise Unsupported("len() of a per-agent scalar")
+
+    def __iter__(self):
+        raise Unsupported("iteration over a per-agent scalar")
+
+    def __getitem__(self, i):
+        raise Unsupported("indexing a per-agent scalar")
+
+    # -- operators
+    def __add__(self, o): return self.tr.binary("add", self, o)
+    def __radd__(self, o): return self.tr.binary("add", o, self)
+    def __sub__(self, o): return self.tr.binary("sub", self, o)
+    def __rsub__(self, o): return self.tr.binary("sub", o, self)
+    def __mul__(self, o): return self.tr.binary("mul", self, o)
+    def __rmul__(self, o): return self.tr.binary("mul", o, self)
+    def __truediv__(self, o): return self.tr.binary("div", self, o)
+    def __rtruediv__(self, o): return self.tr.binary("div", o, self)
+    def __floordiv__(self, o): return self.tr.binary("floordiv", self, o)
+    def __rfloordiv__(self, o): return self.tr.binary("floordiv", o, self)
+    def __mod__(self, o): return self.tr.binary("mod", self, o)
+    def __rmod__(self, o): return self.tr.binary("mod", o, self)
+    def __pow__(self, o): return self.tr.power(self, o)
+    def __lt__(self, o): return self.tr.binary("lt", self, o)
+    def __le__(self, o): return self.tr.binary("le", self, o)
+    def __gt__(self, o): return self.tr.binary("gt", self, o)
+    def __ge__(self, o): return self.tr.binary("ge", self, o)
+    def __eq__(self, o): return self.tr.binary("eq", self, o)  # noqa: PLE0307
+    def __ne__(self, o): return self.tr.binary("ne", self, o)
+    def __and__(self, o): return self.tr.binary("and", self, o)
+    def __rand__(self, o): return self.tr.binary("and", o, self)
+    def __or__(self, o): return self.tr.binary("or", self, o)
+    def __ror__(self, o): return self.tr.binary("or", o, self)
+    def __xor__(self, o): return self.tr.binary("xor", self, o)
+    def __rxor__(self, o): return self.tr.binary("xor", o, self)
+    def __neg__(self): return self.tr.unary("neg", self)
+    def __pos__(self): return self
+    def __abs__(self): return self.tr.unary("abs", self)
+    def __invert__(self): return self.tr.unary("invert", self)
+
+    __hash__ = object.__hash__
+
+    @classmethod
+    def __torch_function__(cls, func, types, args=(), kwargs=None):
+        kwargs = kwargs or {}
+        tr = None
+        for a in list(args) + list(kwargs.values()):
+            if isinstance(a, Sym):
+                tr = a.tr
+                break
+            if isinstance(a, (list, tuple)):
+                for b in a:
+                    if isinstance(b, Sym):
+                        tr = b.tr
+        if tr is None:
+            raise Unsupported(f"{getattr(func, '__name__', func)}")
+        return tr.torch_call(func, args, kwargs)
+
+
+class SymKey:
+    """A traced PRNG key (two uint32 words per agent)."""
+
+    _abm_symkey = True
+    __slots__ = ("tr", "name")
+
+    def __init__(self, tr: "_Trace", name: str):
+        self.tr, self.name = tr, name
+
+    @property
+    def shape(self):
+        return torch.Size((2,))
+
+    def _bits(self) -> str:
+        return self.tr.tmp("uint32_t", f"abm::bits_scalar({self.name}, {self.tr.mode})")
+
+    def split(self, num: int) -> "SymKeyArray":
+        num = int(num)
+        if num < 1:
+            raise Unsupported("split into fewer than one key")
+        tr = self.tr
+        if num == 2:
+            a, b = tr.fresh("k"), tr.fresh("k")
+            tr.lines.append(f"abm::Key {a}, {b}; abm::split2({self.name}, {a}, {b}, {tr.mode});")
+            return SymKeyArray([SymKey(tr, a), SymKey(tr, b)])
+        out = []
+        for j in range(num):
+            k = tr.fresh("k")
+            tr.lines.append(f"const abm::Key {k} = abm::split_at({self.name}, {num}LL, {j}LL, "
+                            f"{tr.mode});")
+            out.append(SymKey(tr, k))
+        return SymKeyArray(out)
+
+    def bits(self) -> Sym:
+        return Sym(self.tr, self.tr.tmp("int32_t", f"(int32_t){self._bits()}"), "i32", True)
+
+    def uniform(self, minval, maxval) -> Sym:
+        tr = self.tr
+        if isinstance(minval, (Sym, torch.Tensor)) or isinstance(maxval, (Sym, torch.Tensor)):
+            f = self.uniform(0.0, 1.0)
+            lo, hi = tr.bound(minval), tr.bound(maxval)
+            return tr.torch_call(torch.maximum, (lo, f * (hi - lo) + lo), {})
+        lo = np.float32(float(minval))
+        span = np.float32(np.float32(float(maxval)) - lo)
+        expr = (f"abm::scale_uniform(abm::unit_float({self._bits()}), "
+                f"{_lit(float(lo), 'f32')}, {_lit(float(span), 'f32')})")
+        return Sym(tr, tr.tmp("float", expr), "f32", True)
+
+    def normal(self) -> Sym:
+        return Sym(self.tr, self.tr.tmp("float", f"abm::normal_from_bits({self._bits()})"),
+                   "f32", True)
+
+    def randint(self, lo: int, hi: int) -> Sym:
+        tr = self.tr
+        span = 1 if hi <= lo else hi - lo
+        a, b = tr.fresh("k"), tr.fresh("k")
+        tr.lines.append(f"abm::Key {a}, {b}; abm::split2({self.name}, {a}, {b}, {tr.mode});")
+        hb = tr.tmp("uint32_t", f"abm::bits_at({a}, 1LL, 0LL, {tr.mode})")
+        lb = tr.tmp("uint32_t", f"abm::bits_at({b}, 1LL, 0LL, {tr.mode})")
+        expr = f"(int32_t)((uint32_t)({lo}) + abm::randint_offset({hb}, {lb}, {span}u))"
+        return Sym(tr, tr.tmp("int32_t", expr), "i32", True)
+
+
+class SymKeyArray:
+    """The keys of ``split(key, num)``."""
+
+    _abm_symkey = True
+
+    def __init__(self, keys: List[SymKey]):
+        self.keys = keys
+
+    @property
+    def shape(self):
+        return torch.Size((len(self.keys), 2))
+
+    def __len__(self):
+        return len(self.keys)
+
+    def __iter__(self):
+        return iter(self.keys)
+
+    def __getitem__(self, i):
+        if isinstance(i, int):
+            return self.keys[i]
+        raise Unsupported("non-integer index into a key array")
+
+
+# ---------------------------------------------------------------- the trace
+class _Trace:
+    def __init__(self, n: int, mode: int):
+        self.n = n
+        self.mode = int(mode)
+        self.lines: List[str] = []
+        self.words: List[Tuple[str, Any]] = []   # ("ptr", tensor) | ("f64", v) | ("i64", v)
+        self.count = 0
+
+    def fresh(self, prefix: str = "v") -> str:
+        self.count += 1
+        return f"{prefix}{self.count}"
+
+    def tmp(self, ct: str, expr: str) -> str:
+        name = self.fresh()
+        self.lines.append(f"const {ct} {name} = {expr};")
+        return name
+
+    def word(self, kind: str, value) -> int:
+        self.words.append((kind, value))
+        return len(self.words) - 1
+
+    # -- leaves
+    def agent_input(self, t: torch.Tensor) -> Sym:
+        kind = _KIND.get(t.dtype)
+        if kind is None or t.dim() != 1 or t.shape[0] != self.n or not t.is_contiguous():
+            raise Unsupported("a state variable that is not one scalar per agent")
+        w = self.word("ptr", t)
+        name = self.tmp(_CT[kind], f"((const {_CT[kind]} *)a.w[{w}])[i]")
+        return Sym(self, name, kind, True)
+
+    def uniform_tensor(self, t: torch.Tensor) -> Sym:
+        kind = _KIND.get(t.dtype)
+        if kind is None or t.dim() != 0 or not t.is_cuda:
+            raise Unsupported("an env tensor that is not a 0-d device scalar")
+        w = self.word("ptr", t)
+        return Sym(self, self.tmp(_CT[kind], f"*((const {_CT[kind]} *)a.w[{w}])"), kind, False)
+
+    def python_param(self, v) -> Sym:
+        if isinstance(v, bool):
+            w = self.word("i64", int(v))
+            return Sym(self, self.tmp("bool", f"(a.w[{w}] != 0ull)"), "pyb", False)
+        if isinstance(v, int):
+            if not -(1 << 63) <= v < (1 << 63):
+                raise Unsupported("Python int out of int64 range")
+            w = self.word("i64", v)
+            return Sym(self, self.tmp("int64_t", f"(int64_t)a.w[{w}]"), "pyi", False)
+        w = self.word("f64", float(v))
+        return Sym(self, self.tmp("double", f"__longlong_as_double((long long)a.w[{w}])"),
+                   "pyf", False)
+
+    def const(self, v) -> Sym:
+        if isinstance(v, bool):
+            return Sym(self, _lit(v, "pyb"), "pyb", False)
+        if isinstance(v, int):
+            return Sym(self, _lit(v, "pyi"), "pyi", False)
+        if isinstance(v, float):
+            return Sym(self, _lit(v, "pyf"), "pyf", False)
+        raise Unsupported(f"constant of type {type(v).__name__}")
+
+    def operand(self, x) -> Sym:
+        if isinstance(x, Sym):
+            if x.tr is not self:
+                raise Unsupported("values of two traces")
+            return x
+        if isinstance(x, (bool, int, float)):
+            return self.const(x)
+        if isinstance(x, np.generic) and not isinstance(x, np.ndarray):
+            kind = _KIND.get(torch.from_numpy(np.asarray(x)).dtype)
+            if kind is None:
+                raise Unsupported("numpy scalar type")
+            s = self.const(x.item())
+            return self.cast(s, kind) if not s.weak or kind not in ("f64", "i64") else s
+        if isinstance(x, torch.Tensor) and x.dim() == 0 and x.is_cuda:
+            return self.uniform_tensor(x)
+        raise Unsupported(f"operand of type {type(x).__name__}")
+
+    def bound(self, v) -> Sym:
+        """random.uniform's tensor-bound branch: bounds as float32 0-d values."""
+        if isinstance(v, Sym):
+            return self.cast(v, "f32")
+        if isinstance(v, torch.Tensor):
+            return self.cast(self.operand(v), "f32")
+        return self.cast(self.const(float(v)), "f32")
+
+    # -- casts and promotion
+    def cast(self, x: Sym, kind: str) -> Sym:
+        x = self.operand(x)
+        if x.kind == kind:
+            return x
+        if kind in _WEAK:
+            raise Unsupported("cast to a Python scalar")
+        return Sym(self, self.tmp(_CT[kind], f"(({_CT[kind]})({x.name}))"), kind, x.varying)
+
+    @staticmethod
+    def _stand_in(x: Sym):
+        if x.weak:
+            return {"pyf": 0.0, "pyi": 0, "pyb": False}[x.kind]
+        return torch.empty((1,) if x.varying else (), dtype=_TORCH[x.kind])
+
+    def promote(self, a: Sym, b: Sym) -> str:
+        if a.weak and b.weak:
+            if "pyf" in (a.kind, b.kind):
+                return "pyf"
+            if "pyi" in (a.kind, b.kind):
+                return "pyi"
+            return "pyb"
+        return _KIND[torch.result_type(self._stand_in(a), self._stand_in(b))]
+
+    # -- element operations
+    def binary(self, op: str, a, b) -> Sym:
+        a, b = self.operand(a), self.operand(b)
+        varying = a.varying or b.varying
+        kind = self.promote(a, b)
+        weak = kind in _WEAK
+        if op == "div":
+            if kind not in _FLOATS:
+                kind = "pyf" if weak else "f32"
+        if op in ("lt", "le", "gt", "ge", "eq", "ne"):
+            sym = {"lt": "<", "le": "<=", "gt": ">", "ge": ">=", "eq": "==", "ne": "!="}[op]
+            if kind == "pyb" and op in ("lt", "le", "gt", "ge"):
+                kind = "pyi"
+            ca, cb = self._as(a, kind), self._as(b, kind)
+            return Sym(self, self.tmp("bool", f"({ca} {sym} {cb})"), "pyb" if weak else "bool",
+                       varying)
+        if op in ("and", "or", "xor"):
+            if kind in ("bool", "pyb"):
+                expr = {"and": "{} && {}", "or": "{} || {}", "xor": "{} != {}"}[op]
+            elif kind in _FLOATS:
+                raise Unsupported("bitwise operation on floats")
+            else:
+                expr = {"and": "{} & {}", "or": "{} | {}", "xor": "{} ^ {}"}[op]
+            ca, cb = self._as(a, kind), self._as(b, kind)
+            return Sym(self, self.tmp(_CT[kind], "(" + expr.format(ca, cb) + ")"), kind, varying)
+        if kind in ("bool", "pyb"):
+            if weak and op in ("add", "sub", "mul", "floordiv", "mod"):
+                kind = "pyi"          # Python: True + True == 2
+            else:
+                raise Unsupported(f"{op} on booleans")
+        ca, cb = self._as(a, kind), self._as(b, kind)
+        ct = _CT[kind]
+        # torch divides a tensor by a Python scalar as a multiply by the
+        # scalar's reciprocal (computed in the result type), and a Python
+        # scalar by a tensor as reciprocal(tensor) * scalar
+        scalar_b = b.weak and not a.weak
+        scalar_a = a.weak and not b.weak
+        if op == "div" and scalar_b:
+            expr = f"({ca} * (({ct})1 / {cb}))"
+        elif op == "div" and scalar_a:
+            expr = f"((({ct})1 / {cb}) * {ca})"
+        elif op in ("add", "sub", "mul", "div"):
+            expr = f"({ca} {dict(add='+', sub='-', mul='*', div='/')[op]} {cb})"
+        elif op == "floordiv":
+            if kind in _FLOATS and scalar_b:
+                expr = f"abm_floordiv_fs<{ct}>({ca}, {cb})"
+            elif kind in _FLOATS:
+                expr = f"abm_floordiv_f<{ct}>({ca}, {cb})"
+            else:
+                expr = f"abm_floordiv_i<{ct}>({ca}, {cb})"
+        elif op == "mod":
+            if kind == "pyf":
+                expr = f"abm_pymod_f({ca}, {cb})"
+            elif kind in _FLOATS:
+                expr = f"abm_mod_f<{ct}>({ca}, {cb})"
+            else:
+                expr = f"abm_mod_i<{ct}>({ca}, {cb})"
+        else:
+            raise Unsupported(op)
+        return Sym(self, self.tmp(ct, expr), kind, varying)
+
+    def _as(self, x: Sym, kind: str) -> str:
+        if x.kind == kind:
+            return x.name
+        return f"(({_CT[kind]})({x.name}))"
+
+    def power(self, x, e) -> Sym:
+        if not isinstance(e, int) or isinstance(e, bool) or e not in (2, 3):
+            raise Unsupported("power other than x**2 or x**3")
+        x = self.operand(x)
+        kind = self.promote(x, self.const(e))
+        cx = self._as(x, kind)
+        expr = f"({cx} * {cx})" if e == 2 else f"({cx} * {cx} * {cx})"
+        return Sym(self, self.tmp(_CT[kind], expr), kind, x.varying)
+
+    _UNARY_F = {"exp": "expf", "log": "logf", "log1p": "log1pf", "expm1": "expm1f",
+                "sqrt": "sqrtf", "sin": "sinf", "cos": "cosf", "tan": "tanf", "tanh": "tanhf",
+                "atan": "atanf"}
+
+    def unary(self, op: str, x) -> Sym:
+        x = self.operand(x)
+        if x.weak:
+            if op == "neg":
+                if x.kind == "pyb":
+                    raise Unsupported("negation of a bool")
+                return Sym(self, self.tmp(_CT[x.kind], f"(-{x.name})"), x.kind, x.varying)
+            if op == "abs":
+                k = "pyi" if x.kind == "pyb" else x.kind
+                fn = "fabs" if k == "pyf" else "llabs"
+                return Sym(self, self.tmp(_CT[k], f"{fn}({self._as(x, k)})"), k, x.varying)
+            x = x.as_array()
+        k, ct = x.kind, _CT[x.kind]
+        if op == "neg":
+            if k == "bool":
+                raise Unsupported("negation of a bool tensor")
+            return Sym(self, self.tmp(ct, f"(({ct})(-{x.name}))"), k, x.varying)
+        if op == "abs":
+            if k == "bool":
+                raise Unsupported("abs of a bool tensor")
+            expr = (f"fabsf({x.name})" if k == "f32" else f"fabs({x.name})" if k == "f64"
+                    else f"(({ct})({x.name} < 0 ? -{x.name} : {x.name}))")
+            return Sym(self, self.tmp(ct, expr), k, x.varying)
+        if op == "invert":
+            expr = f"(!{x.name})" if k == "bool" else f"(({ct})(~{x.name}))"
+            if k in _FLOATS:
+                raise Unsupported("~ on floats")
+            return Sym(self, self.tmp(ct, expr), k, x.varying)
+        if op in self._UNARY_F:
+            if k not in _FLOATS:
+                x = self.cast(x, "f32")
+            if x.kind == "f64":
+                return Sym(self, self.tmp("double", f"{self._UNARY_F[op][:-1]}({x.name})"),
+                           "f64", x.varying)
+            return Sym(self, self.tmp("float", f"{self._UNARY_F[op]}({x.name})"), "f32",
+                       x.varying)
+        if op in ("floor", "ceil", "round", "trunc"):
+            if k not in _FLOATS:
+                return x
+            fn = {"floor": "floor", "ceil": "ceil", "round": "rint", "trunc": "trunc"}[op]
+            fn = fn + ("f" if k == "f32" else "")
+            return Sym(self, self.tmp(ct, f"{fn}({x.name})"), k, x.varying)
+        if op == "square":
+            return self.binary("mul", x, x)
+        if op in ("isnan", "isfinite"):
+            if k not in _FLOATS:
+                return Sym(self, "true" if op == "isfinite" else "false", "bool", x.varying)
+            fn = "isnan" if op == "isnan" else "isfinite"
+            return Sym(self, self.tmp("bool", f"{fn}({x.name})"), "bool", x.varying)
+        if op == "sign":
+            if k == "bool":
+                return x
+            expr = f"(({ct})(({x.name} > 0) - ({x.name} < 0)))"   # 0 for NaN, as torch
+            return Sym(self, self.tmp(ct, expr), k, x.varying)
+        if op == "logical_not":
+            return Sym(self, self.tmp("bool", f"(!(bool)({x.name}))"), "bool", x.varying)
+        raise Unsupported(op)
+
+    def where(self, c, a, b) -> Sym:
+        c = self.operand(c)
+        a, b = self.operand(a), self.operand(b)
+        kind = self.promote(a, b)
+        if kind in _WEAK:
+            kind = _DEFAULT[kind]
+        cb = c.name if c.kind in ("bool", "pyb") else f"((bool)({c.name}))"
+        expr = f"({cb} ? {self._as(a, kind)} : {self._as(b, kind)})"
+        return Sym(self, self.tmp(_CT[kind], expr), kind, c.varying or a.varying or b.varying)
+
+    def minmax(self, op: str, a, b) -> Sym:
+        a, b = self.operand(a), self.operand(b)
+        if a.weak and b.weak:
+            a = a.as_array()
+        kind = self.promote(a, b)
+        ca, cb = self._as(a, kind), self._as(b, kind)
+        fn = "abm_min" if op == "min" else "abm_max"
+        return Sym(self, self.tmp(_CT[kind], f"{fn}<{_CT[kind]}>({ca}, {cb})"), kind,
+                   a.varying or b.varying)
+
+    def clamp(self, x, lo, hi) -> Sym:
+        x = self.operand(x)
+        if x.weak:
+            x = x.as_array()
+        kind = x.kind
+        for v in (lo, hi):
+            if v is not None:
+                kind = self.promote(Sym(self, "", kind, x.varying), self.operand(v))
+        out = self.cast(x, kind)
+        if lo is not None:
+            out = Sym(self, self.tmp(_CT[kind], f"abm_clamp_lo<{_CT[kind]}>({out.name}, "
+                                                f"{self._as(self.operand(lo), kind)})"),
+                      kind, x.varying)
+        if hi is not None:
+            out = Sym(self, self.tmp(_CT[kind], f"abm_clamp_hi<{_CT[kind]}>({out.name}, "
+                                                f"{self._as(self.operand(hi), kind)})"),
+                      kind, x.varying)
+        return out
+
+    # -- torch functions called with traced values
+    def torch_call(self, func, args, kwargs):
+        name = getattr(func, "__name__", "")
+        unary = {"abs": "abs", "absolute": "abs", "neg": "neg", "negative": "neg",
+                 "exp": "exp", "log": "log", "log1p": "log1p", "expm1": "expm1",
+                 "sqrt": "sqrt", "sin": "sin", "cos": "cos", "tan": "tan", "tanh": "tanh",
+                 "atan": "atan", "arctan": "atan", "floor": "floor", "ceil": "ceil",
+                 "trunc": "trunc", "square": "square", "isnan": "isnan",
+                 "isfinite": "isfinite", "sign": "sign", "logical_not": "logical_not"}
+        if name in unary and len(args) == 1 and not kwargs:
+            return self.unary(unary[name], args[0])
+        if name == "round" and len(args) == 1 and kwargs.get("decimals", 0) == 0:
+            return self.unary("round", args[0])
+        binary = {"add": "add", "sub": "sub", "subtract": "sub", "mul": "mul",
+                  "multiply": "mul", "div": "div", "true_divide": "div", "divide": "div",
+                  "floor_divide": "floordiv", "remainder": "mod", "lt": "lt", "less": "lt",
+                  "le": "le", "gt": "gt", "greater": "gt", "ge": "ge", "eq": "eq", "ne": "ne",
+                  "bitwise_and": "and", "bitwise_or": "or", "bitwise_xor": "xor"}
+        if name in binary and len(args) == 2 and not kwargs:
+            return self.binary(binary[name], args[0], args[1])
+        dunder = {"__add__": "add", "__sub__": "sub", "__mul__": "mul", "__truediv__": "div",
+                  "__div__": "div", "__floordiv__": "floordiv", "__mod__": "mod",
+                  "__lt__": "lt", "__le__": "le", "__gt__": "gt", "__ge__": "ge",
+                  "__eq__": "eq", "__ne__": "ne", "__and__": "and", "__or__": "or",
+                  "__xor__": "xor"}
+        if name in dunder and len(args) == 2:
+            return self.binary(dunder[name], args[0], args[1])
+        if name.startswith("__r") and "__" + name[3:] in dunder and len(args) == 2:
+            return self.binary(dunder["__" + name[3:]], args[1], args[0])
+        if name == "__pow__" and len(args) == 2:
+            return self.power(args[0], args[1])
+        if name in ("__neg__", "__abs__", "__invert__") and len(args) == 1:
+            return self.unary(name.strip("_").replace("invert", "invert"), args[0])
+        if name in ("logical_and", "logical_or", "logical_xor") and len(args) == 2:
+            a = self.cast(self.operand(args[0]).as_array(), "bool")
+            b = self.cast(self.operand(args[1]).as_array(), "bool")
+            return self.binary(name.split("_")[1], a, b)
+        if name == "where" and len(args) == 3 and not kwargs:
+            return self.where(*args)
+        if name in ("minimum", "maximum", "fmin", "fmax") and len(args) == 2 and not kwargs:
+            if name in ("fmin", "fmax"):
+                raise Unsupported(name)
+            return self.minmax("min" if name == "minimum" else "max", *args)
+        if name in ("clamp", "clip"):
+            x = args[0]
+            lo = args[1] if len(args) > 1 else kwargs.get("min")
+            hi = args[2] if len(args) > 2 else kwargs.get("max")
+            return self.clamp(x, lo, hi)
+        if name == "pow" and len(args) == 2 and not kwargs:
+            return self.power(args[0], args[1])
+        raise Unsupported(f"torch.{name}")
+
+
+_HELPERS = r"""
+// element functions with the semantics of torch's (remainder, floor_divide,
+// clamp, minimum, maximum) and of Python's // and % on scalars
+template <typename T> __device__ __forceinline__ T abm_mod_i(T a, T b) {
+  T r = a % b;
+  if (r != 0 && ((r < 0) != (b < 0))) r += b;
+  return r;
+}
+template <typename T> __device__ __forceinline__ T abm_floordiv_i(T a, T b) {
+  T q = a / b;
+  if ((a % b != 0) && ((a < 0) != (b < 0))) q -= 1;
+  return q;
+}
+template <typename T> __device__ __forceinline__ T abm_mod_f(T a, T b) {
+  T m = fmod(a, b);
+  if (m != 0 && ((b < 0) != (m < 0))) m += b;
+  return m;
+}
+__device__ __forceinline__ double abm_pymod_f(double a, double b) {
+  double m = fmod(a, b);
+  if (m != 0) {
+    if ((b < 0) != (m < 0)) m += b;
+  } else {
+    m = copysign(0.0, b);
+  }
+  return m;
+}
+template <typename T> __device__ __forceinline__ T abm_floordiv_f(T a, T b) {
+  if (b == 0) return a / b;
+  T m = fmod(a, b);
+  T d = (a - m) / b;
+  if (m != 0 && ((b < 0) != (m < 0))) d -= 1;
+  T f;
+  if (d != 0) {
+    f = floor(d);
+    if (d - f > (T)0.5) f += 1;
+  } else {
+    f = copysign((T)0, a / b);
+  }
+  return f;
+}
+// floor division by a Python scalar: torch multiplies by the reciprocal
+template <typename T> __device__ __forceinline__ T abm_floordiv_fs(T a, T b) {
+  if (b == 0) return a / b;
+  const T inv_b = (T)1 / b;
+  T m = fmod(a, b);
+  T d = (a - m) * inv_b;
+  if (m != 0 && ((b < 0) != (m < 0))) d -= 1;
+  T f;
+  if (d != 0) {
+    f = floor(d);
+    if (d - f > (T)0.5) f += 1;
+  } else {
+    f = copysign((T)0, a * inv_b);
+  }
+  return f;
+}
+template <typename T> __device__ __forceinline__ bool abm_isnan(T a) { return a != a; }
+template <typename T> __device__ __forceinline__ T abm_lo2(T a, T b) { return b < a ? b : a; }
+template <typename T> __device__ __forceinline__ T abm_hi2(T a, T b) { return a < b ? b : a; }
+template <> __device__ __forceinline__ float abm_lo2<float>(float a, float b) { return fminf(a, b); }
+template <> __device__ __forceinline__ float abm_hi2<float>(float a, float b) { return fmaxf(a, b); }
+template <> __device__ __forceinline__ double abm_lo2<double>(double a, double b) { return fmin(a, b); }
+template <> __device__ __forceinline__ double abm_hi2<double>(double a, double b) { return fmax(a, b); }
+template <typename T> __device__ __forceinline__ T abm_min(T a, T b) {
+  if (abm_isnan(a)) return a;
+  if (abm_isnan(b)) return b;
+  return abm_lo2<T>(a, b);
+}
+template <typename T> __device__ __forceinline__ T abm_max(T a, T b) {
+  if (abm_isnan(a)) return a;
+  if (abm_isnan(b)) return b;
+  return abm_hi2<T>(a, b);
+}
+template <typename T> __device__ __forceinline__ T abm_clamp_lo(T x, T lo) {
+  if (abm_isnan(x)) return x;
+  return abm_hi2<T>(x, lo);
+}
+template <typename T> __device__ __forceinline__ T abm_clamp_hi(T x, T hi) {
+  if (abm_isnan(x)) return x;
+  return abm_lo2<T>(x, hi);
+}
+"""
+
+_RTC_TYPES = "".join(f"typedef __hip_internal::{t} {t};\n" for t in (
+    "int8_t", "int16_t", "int32_t", "int64_t", "uint8_t", "uint16_t", "uint32_t", "uint64_t"))
+
+_prelude_text: Optional[str] = None
+
+
+def _prelude() -> str:
+    global _prelude_text
+    if _prelude_text is None:
+        with open(_PRELUDE_PATH) as f:
+            _prelude_text = _RTC_TYPES + f.read() + _HELPERS
+    return _prelude_text
+
+
+def _source(tr: _Trace, body_outputs: List[str], name: str) -> str:
+    nw = max(len(tr.words), 1)
+    body = "\n  ".join(tr.lines + body_outputs)
+    return (f"{_prelude()}\nstruct AbmWords {{ unsigned long long w[{nw}]; }};\n"
+            f"extern \"C\" __global__ void __launch_bounds__(256) {name}(const AbmWords a) {{\n"
+            f"  const int64_t n = (int64_t)a.w[0];\n"
+            f"  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n"
+            f"  if (i >= n) return;\n  {body}\n}}\n")
+
+
+# ---------------------------------------------------------------- compile cache
+class _Kernel:
+    def __init__(self, src: str, name: str):
+        from ._lib import check, lib
+
+        code, size = C.c_void_p(), C.c_size_t()
+        check(lib().abm_rtc_compile(src.encode(), _ARCH.encode(), C.byref(code), C.byref(size)),
+              "rtc_compile")
+        try:
+            h = C.c_void_p()
+            check(lib().abm_rtc_load(code, name.encode(), C.byref(h)), "rtc_load")
+        finally:
+            lib().abm_rtc_code_free(code)
+        self.handle = h
+        self.name = name
+
+    def launch(self, n: int, words: np.ndarray, stream) -> None:
+        from ._lib import check, lib
+
+        grid = max(1, (n + 255) // 256)
+        check(lib().abm_rtc_launch(self.handle, grid, 256, words.ctypes.data, words.nbytes,
+                                   stream), "rtc_launch")
+
+
+_kernels: Dict[str, _Kernel] = {}
+_refused: Dict[Any, str] = {}
+_launches = 0
+
+
+def compile_source(src: str) -> None:
+    """Compile ``src`` with hiprtc without loading it (a check that needs no
+    GPU); raises on a compile error."""
+    from ._lib import check, lib
+
+    code, size = C.c_void_p(), C.c_size_t()
+    check(lib().abm_rtc_compile(src.encode(), _ARCH.encode(), C.byref(code), C.byref(size)),
+          "rtc_compile")
+    lib().abm_rtc_code_free(code)
+
+
+def _pack(tr: _Trace) -> np.ndarray:
+    out = np.zeros(max(len(tr.words), 1), np.uint64)
+    for j, (kind, v) in enumerate(tr.words):
+        if kind == "ptr":
+            out[j] = np.uint64(v.data_ptr())
+        elif kind == "f64":
+            out[j] = np.frombuffer(struct.pack("<d", v), np.uint64)[0]
+        else:
+            out[j] = np.frombuffer(struct.pack("<q", int(v)), np.uint64)[0]
+    return out
+
+
+def _wrap_model_state(tr: _Trace, model_state):
+    if not isinstance(model_state, dict):
+        return model_state
+
+    def wrap(v):
+        if isinstance(v, (bool, int, float)):
+            return tr.python_param(v)
+        if isinstance(v, torch.Tensor) and v.dim() == 0 and v.is_cuda and v.dtype in _KIND:
+            return tr.uniform_tensor(v)
+        return v
+
+    out = {}
+    for k, v in model_state.items():
+        if k == "env" and isinstance(v, dict):
+            out[k] = {ek: wrap(ev) for ek, ev in v.items()}
+        elif k == "time_step":
+            out[k] = wrap(v)
+        else:
+            out[k] = v
+    return out
+
+
+def _signature(fn, states, model_state) -> Tuple:
+    def sig(v):
+        if isinstance(v, torch.Tensor):
+            return ("t", v.dtype, tuple(v.shape))
+        if isinstance(v, (bool, int, float)):
+            return ("p", type(v).__name__)
+        return ("o", type(v).__name__)
+
+    ms = ()
+    if isinstance(model_state, dict):
+        ms = tuple(sorted((k, sig(v)) for k, v in model_state.items() if k != "env"))
+        env = model_state.get("env")
+        if isinstance(env, dict):
+            ms += tuple(sorted((k, sig(v)) for k, v in env.items()))
+    st = tuple(sorted((k, sig(v)) for k, v in (states or {}).items()))
+    return (getattr(fn, "__qualname__", repr(fn)), id(getattr(fn, "__func__", fn)), st, ms)
+
+
+def trace(n: int, key: torch.Tensor, mode: int, states: Optional[Dict[str, Any]], model_state,
+          call, device=None):
+    """Trace ``call(states_sym, model_state_sym, agent_key_sym)`` for n
+    agents; returns (kernel name, source, trace, output tensors).  Raises
+    Unsupported."""
+    tr = _Trace(n, mode)
+    tr.word("i64", n)
+    kw = tr.word("ptr", key.contiguous() if key.dtype == torch.int32 else
+                 key.to(torch.int32).contiguous())
+    tr.lines.append(f"const uint32_t *kp = (const uint32_t *)a.w[{kw}];")
+    tr.lines.append(f"const abm::Key ak = abm::split_at(abm::Key{{kp[0], kp[1]}}, n, i, "
+                    f"{tr.mode});")
+    sym_states = ({k: tr.agent_input(v) for k, v in states.items()}
+                  if states is not None else None)
+    out = call(sym_states, _wrap_model_state(tr, model_state), SymKey(tr, "ak"))
+    if not isinstance(out, dict):
+        raise Unsupported("the function did not return a dict")
+    stores, outs = [], {}
+    for k, v in out.items():
+        if isinstance(v, Sym):
+            s = v if not v.weak else v.as_array()
+        elif isinstance(v, (bool, int, float)):
+            s = tr.const(v).as_array()
+        else:
+            raise Unsupported(f"output {k!r} of type {type(v).__name__}")
+        t = torch.empty((n,), dtype=_TORCH[s.kind], device=device or key.device)
+        w = tr.word("ptr", t)
+        stores.append(f"(({_CT[s.kind]} *)a.w[{w}])[i] = {s.name};")
+        outs[k] = t
+    digest = hashlib.sha1("\n".join(tr.lines + stores).encode()).hexdigest()[:16]
+    name = f"abm_fused_{digest}"
+    return name, _source(tr, stores, name), tr, outs
+
+
+def run_agents(fn, n: int, key: torch.Tensor, mode: int, states: Optional[Dict[str, Any]],
+               model_state, call) -> Optional[Dict[str, torch.Tensor]]:
+    """Run ``call(states_sym, model_state_sym, agent_key_sym)`` for all n
+    agents as one generated kernel; None when the function is not covered
+    (remembered for the function and input signature)."""
+    if not _ENABLED or not isinstance(key, torch.Tensor) or not key.is_cuda:
+        return None
+    sig = _signature(fn, states, model_state)
+    if sig in _refused:
+        return None
+    try:
+        name, src, tr, outs = trace(n, key, mode, states, model_state, call)
+    except Unsupported as e:
+        _refused[sig] = str(e)
+        return None
+    kern = _kernels.get(src)
+    if kern is None:
+        kern = _Kernel(src, name)
+        _kernels[src] = kern
+    from ._lib import stream_handle
+
+    global _launches
+    kern.launch(n, _pack(tr), stream_handle())
+    _launches += 1
+    return outs
+
+
+def stats() -> Dict[str, int]:
+    """Fused launches so far and kernels compiled."""
+    return {"launches": _launches, "kernels": len(_kernels), "refused": len(_refused)}
+
+
+def refused() -> Dict[Any, str]:
+    """Functions (and input signatures) that run on the vmapped path, with
+    the reason."""
+    return dict(_refused)

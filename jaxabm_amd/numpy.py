@@ -52,6 +52,8 @@ def _default_dtype(value):
 
 def asarray(a, dtype=None) -> torch.Tensor:
     """Array from a tensor, numpy array, Python scalar or (nested) sequence."""
+    if getattr(a, "_abm_sym", False) is True:
+        return a.as_array() if dtype is None else a.as_array().to(dtype)
     if isinstance(a, torch.Tensor):
         return a if dtype is None else a.to(dtype)
     if isinstance(a, _np.ndarray):
@@ -78,8 +80,22 @@ def asarray(a, dtype=None) -> torch.Tensor:
 array = asarray
 
 
+def _traced(x) -> bool:
+    return getattr(x, "_abm_sym", False) is True
+
+
+def _pyscalar(x) -> bool:
+    """A Python scalar, or a traced stand-in for one (env entries and the
+    time step inside a fused agent function)."""
+    return isinstance(x, (int, float)) or (_traced(x) and x.weak)
+
+
 def _t(x):
-    return x if isinstance(x, torch.Tensor) else asarray(x)
+    if isinstance(x, torch.Tensor):
+        return x
+    if _traced(x):
+        return x.as_array()
+    return asarray(x)
 
 
 def zeros(shape, dtype=float32):
@@ -130,8 +146,8 @@ def where(cond, x=None, y=None):
     if x is None and y is None:
         return torch.nonzero(_t(cond), as_tuple=True)
     c = _t(cond)
-    return torch.where(c, _t(x) if not isinstance(x, (int, float)) else x,
-                       _t(y) if not isinstance(y, (int, float)) else y)
+    return torch.where(c, _t(x) if not _pyscalar(x) else x,
+                       _t(y) if not _pyscalar(y) else y)
 
 
 def clip(x, a_min=None, a_max=None):

@@ -235,13 +235,36 @@ def _key_arg(key) -> torch.Tensor:
     return _as_key(key)
 
 
+# A traced key (jaxabm_amd.fuse: a per-agent function being lowered to one
+# kernel) takes the draws below to the trace, with the same constructions.
+def _traced_key(key) -> bool:
+    return getattr(key, "_abm_symkey", False) is True
+
+
+def _traced_scalar_shape(key, shape, mode):
+    from .fuse import SymKey, Unsupported
+
+    if not isinstance(key, SymKey):
+        raise Unsupported("a draw from a key array")
+    if _shape(shape) != ():
+        raise Unsupported("a non-scalar draw inside a per-agent function")
+    if _mode(mode) != key.tr.mode:
+        raise Unsupported("a draw in another stream mode")
+
+
 def split(key, num: int = 2, mode=None) -> torch.Tensor:
     """``jax.random.split(key, num)`` -> (num, 2)."""
+    if _traced_key(key):
+        _traced_scalar_shape(key, (), mode)
+        return key.split(int(num))
     return torch.ops.jaxabm.split(_key_arg(key), int(num), _mode(mode))
 
 
 def bits(key, shape=(), mode=None) -> torch.Tensor:
     """``jax.random.bits(key, shape, uint32)`` (returned as int32 storage)."""
+    if _traced_key(key):
+        _traced_scalar_shape(key, shape, mode)
+        return key.bits()
     return torch.ops.jaxabm.draw(_key_arg(key), list(_shape(shape)), 0, 0.0, 0.0, _mode(mode))
 
 
@@ -250,6 +273,9 @@ def uniform(key, shape=(), dtype=torch.float32, minval=0.0, maxval=1.0, mode=Non
     bounds (e.g. per agent under vmap) scale a [0, 1) draw with torch ops."""
     if dtype not in (torch.float32, None):
         raise NotImplementedError("uniform: float32 only (the reference runs with jax_enable_x64 off)")
+    if _traced_key(key):
+        _traced_scalar_shape(key, shape, mode)
+        return key.uniform(minval, maxval)
     k = _key_arg(key)
     if isinstance(minval, torch.Tensor) or isinstance(maxval, torch.Tensor):
         f = torch.ops.jaxabm.draw(k, list(_shape(shape)), 1, 0.0, 1.0, _mode(mode))
@@ -266,12 +292,15 @@ def uniform(key, shape=(), dtype=torch.float32, minval=0.0, maxval=1.0, mode=Non
 
 def normal(key, shape=(), dtype=torch.float32, mode=None) -> torch.Tensor:
     """``jax.random.normal(key, shape, float32)``."""
+    if _traced_key(key):
+        _traced_scalar_shape(key, shape, mode)
+        return key.normal()
     return torch.ops.jaxabm.draw(_key_arg(key), list(_shape(shape)), 2, 0.0, 0.0, _mode(mode))
 
 
 def bernoulli(key, p=0.5, shape=(), mode=None) -> torch.Tensor:
     """``jax.random.bernoulli(key, p, shape)`` = ``uniform(key, shape) < p``."""
-    if isinstance(p, torch.Tensor):
+    if isinstance(p, torch.Tensor) or getattr(p, "_abm_sym", False) is True:
         shp = _shape(shape) if shape not in ((), None) else tuple(p.shape)
         return uniform(key, shp, mode=mode) < p.to(torch.float32)
     return uniform(key, shape, mode=mode) < float(p)
@@ -281,6 +310,9 @@ def randint(key, shape, minval, maxval, dtype=torch.int32, mode=None) -> torch.T
     """``jax.random.randint(key, shape, minval, maxval, int32)``."""
     lo = int(np.clip(int(minval), -2**31, 2**31 - 1))
     hi = int(np.clip(int(maxval), -2**31, 2**31 - 1))
+    if _traced_key(key):
+        _traced_scalar_shape(key, shape, mode)
+        return key.randint(lo, hi)
     return torch.ops.jaxabm.randint(_key_arg(key), list(_shape(shape)), lo, hi, _mode(mode))
 
 
@@ -295,6 +327,10 @@ def choice(key, a, shape=(), mode=None):
 
 def permutation(key, x, mode=None) -> torch.Tensor:
     """``jax.random.permutation(key, n)`` for an integer n, or of a 1-D tensor."""
+    if _traced_key(key):
+        from .fuse import Unsupported
+
+        raise Unsupported("a permutation inside a per-agent function")
     k = _single(key)
     if isinstance(x, torch.Tensor):
         n = x.shape[0]

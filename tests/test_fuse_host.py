@@ -1,0 +1,105 @@
+"""Host side of the fused agent functions (jaxabm_amd.fuse): tracing, the
+dtypes of the generated outputs (torch's promotion with per-agent values as
+1-D tensors and env scalars as Python scalars), what is refused, and that
+the generated source compiles with hiprtc for gfx950 (no GPU needed)."""
+import numpy as np
+import pytest
+import torch
+
+from jaxabm_amd import fuse
+from jaxabm_amd import numpy as jnp
+from jaxabm_amd import random as jr
+
+N = 300
+
+
+def _states():
+    return {"x": torch.rand(N), "w": torch.randint(-5, 6, (N,), dtype=torch.int32),
+            "alive": torch.rand(N) < 0.5}
+
+
+def _trace(fn, states=None, ms=None, mode=1):
+    key = torch.zeros(2, dtype=torch.int32)
+    return fuse.trace(N, key, mode, states, ms, fn, device="cpu")
+
+
+def test_update_traces_with_torch_dtypes_and_compiles():
+    ms = {"time_step": 4, "env": {"rate": 0.5, "n_max": 3, "on": True, "name": "world"}}
+
+    def upd(s, m, k):
+        k1, k2 = jr.split(k)
+        x = jnp.mod(s["x"] + jr.normal(k1) * 0.01, 1.0)
+        w = jnp.where((jr.uniform(k2) < 0.5) & s["alive"], s["w"] - 1, s["w"] + 1)
+        return {
+            "x": x,
+            "w": w + m["time_step"] % 2,           # int32 + Python int
+            "f": s["w"] * m["env"]["rate"],        # int32 * Python float -> float32
+            "g": s["w"] / 2,                       # true division -> float32
+            "h": s["w"] // m["env"]["n_max"],      # floor division stays int32
+            "b": s["alive"] | (x > 0.5),
+            "bi": s["alive"] + 1,                  # bool + Python int -> int64
+            "c": 7, "t": m["time_step"] * 2.5,    # constants: int32, float32
+            "r": jr.randint(k1, (), -3, 9) + jr.bernoulli(k2, 0.25),
+            "u": jr.uniform(k2, minval=-1.0, maxval=2.0),
+            "e": jnp.exp(x) + jnp.sqrt(x) - jnp.tanh(x),
+            "cl": jnp.clip(x, 0.2, 0.7), "mx": jnp.maximum(x, s["x"]),
+            "cast": jnp.asarray(x * 10, dtype=jnp.int32),
+        }
+
+    name, src, tr, outs = _trace(upd, _states(), ms)
+    want = {"x": torch.float32, "w": torch.int32, "f": torch.float32, "g": torch.float32,
+            "h": torch.int32, "b": torch.bool, "bi": torch.int64, "c": torch.int32,
+            "t": torch.float32, "r": torch.int32, "u": torch.float32, "e": torch.float32,
+            "cl": torch.float32, "mx": torch.float32, "cast": torch.int32}
+    assert {k: v.dtype for k, v in outs.items()} == want
+    assert all(v.shape == (N,) for v in outs.values())
+    # env Python scalars and the time step are launch parameters, not literals
+    assert "world" not in src and "0x1.0000000000000p-1" not in src.split("rate")[0]
+    fuse.compile_source(src)
+
+
+def test_init_traces_and_compiles():
+    def init(s, m, k):
+        k1, k2 = jr.split(k)
+        return {"x": jr.uniform(k1), "w": jr.randint(k2, (), 0, 6), "flag": True}
+
+    name, src, tr, outs = _trace(init)
+    assert {k: v.dtype for k, v in outs.items()} == {"x": torch.float32, "w": torch.int32,
+                                                      "flag": torch.bool}
+    fuse.compile_source(src)
+
+
+def test_same_function_same_source_for_other_env_values():
+    def upd(s, m, k):
+        return {"x": s["x"] * m["env"]["rate"] + m["time_step"]}
+
+    a = _trace(upd, _states(), {"time_step": 1, "env": {"rate": 0.5}})[1]
+    b = _trace(upd, _states(), {"time_step": 9, "env": {"rate": 0.25}})[1]
+    assert a == b
+
+
+@pytest.mark.parametrize("fn,why", [
+    (lambda s, m, k: {"x": s["x"] if s["x"] > 0.5 else -s["x"]}, "control flow"),
+    (lambda s, m, k: {"x": jnp.sum(s["x"])}, "sum"),
+    (lambda s, m, k: {"x": s["x"] + m["agents_a"]["x"]}, "operand"),
+    (lambda s, m, k: {"x": jr.uniform(k, (3,))}, "non-scalar draw"),
+    (lambda s, m, k: {"x": float(s["x"])}, "float()"),
+    (lambda s, m, k: {"x": s["x"] ** 0.5}, "power"),
+    (lambda s, m, k: [s["x"]], "dict"),
+])
+def test_refused(fn, why):
+    ms = {"time_step": 0, "env": {}, "agents_a": {"x": torch.rand(N)}}
+    with pytest.raises(fuse.Unsupported):
+        _trace(fn, _states(), ms)
+
+
+def test_vector_state_refused():
+    with pytest.raises(fuse.Unsupported):
+        _trace(lambda s, m, k: {"p": s["p"] + 1.0}, {"p": torch.rand(N, 2)}, {"env": {}})
+
+
+def test_literals_keep_float64_bits():
+    assert fuse._lit(0.1, "f32") == "((float)0x1.999999999999ap-4)"
+    assert fuse._lit(-2, "i32") == "((int32_t)(-2LL))"
+    assert fuse._lit(True, "bool") == "((bool)true)"
+    assert np.float32(float.fromhex("0x1.999999999999ap-4")) == np.float32(0.1)

@@ -1,0 +1,259 @@
+"""GPU: agent functions run as one generated kernel (jaxabm_amd.fuse) give
+the vmapped torch program's results bit for bit, NaN results as NaN
+(jaxabm/agent.py:92-177:
+the reference's vmap of init_state / update), over the covered operations,
+special values (signed zeros, NaN, negative operands of // and %), env
+scalars of each kind, the time step and every draw, in both stream modes."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def J():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import jaxabm_amd
+
+    return jaxabm_amd
+
+
+class Arith:
+    def init_state(self, cfg, key):
+        from jaxabm_amd import random as jr
+
+        k1, k2, k3 = jr.split(key, 3)
+        return {"x": jr.uniform(k1, minval=-3.0, maxval=3.0), "w": jr.randint(k2, (), -7, 8),
+                "b": jr.bernoulli(k3, 0.4), "z": 0.0}
+
+    def update(self, s, ms, cfg, key):
+        from jaxabm_amd import numpy as jnp
+
+        env = ms["env"]
+        x, w, b = s["x"], s["w"], s["b"]
+        sz = x * 0.0                                   # signed zeros
+        nan = jnp.log(x)                               # NaN where x < 0
+        y = (x + env["shift"]) * env["scale"] - w / 3 + (x ** 2) / 7.0
+        q = w // 3 + w % 4 + (w * env["k"]) // -2 + jnp.mod(w, -3)
+        fq = x // 0.7 + jnp.mod(x, 1.3) + jnp.mod(x, -0.9)
+        m = jnp.minimum(nan, sz) + jnp.maximum(sz, -sz) * 2 + jnp.minimum(x, y)
+        c = jnp.clip(y, -1.0, 1.5) + jnp.clip(nan, 0.0, 1.0)
+        t = (jnp.exp(x / 4) + jnp.log(jnp.abs(x) + 1.0) + jnp.sqrt(jnp.abs(y)) + jnp.sin(x)
+             - jnp.cos(y) + jnp.tanh(x * 2) + jnp.floor(y) - jnp.ceil(x) + jnp.round(x * 3)
+             + jnp.sign(sz + x))
+        flag = (b & (x > env["thr"])) | ((w < 0) ^ env["on"])
+        k = jnp.where(flag, w + ms["time_step"], -w) + jnp.asarray(y * 10, dtype=jnp.int32)
+        # torch divides by a Python scalar through its reciprocal
+        d = (x / 0.7 + 3.0 / (x + 5.0) + x // env["scale"] + w / 3 + x / env["delta"]
+             + x // -1.3)
+        return {"x": jnp.where(b, y, t) + env["delta"], "w": q + k, "b": ~flag,
+                "z": m + c + fq + s["z"] * 0.5 + d}
+
+
+class Draws:
+    def init_state(self, cfg, key):
+        from jaxabm_amd import random as jr
+
+        return {"x": jr.normal(key), "i": jr.bits(key)}
+
+    def update(self, s, ms, cfg, key):
+        from jaxabm_amd import numpy as jnp
+        from jaxabm_amd import random as jr
+
+        k1, k2, k3, k4 = jr.split(key, 4)
+        a, b = jr.split(k1)
+        lo = jnp.minimum(s["x"], 0.0)
+        return {"x": jr.uniform(a, minval=lo, maxval=lo + 2.0) + jr.normal(b) * 0.5,
+                "i": jr.bits(k2) ^ s["i"],
+                "r": jr.randint(k3, (), -20, 11) + jr.choice(k4, 5),
+                "p": jr.bernoulli(k4, jnp.abs(s["x"]) / 4.0), "q": jr.bernoulli(b, 0.3),
+                "u": jr.uniform(k2, minval=0.5, maxval=3.25)}
+
+
+def run(J, agent, n, steps, mode, fused, env):
+    import torch
+
+    from jaxabm_amd import fuse
+    from jaxabm_amd import random as jr
+
+    fuse.enable(fused)
+    jr.set_threefry_partitionable(mode == O.MODE_PARTITIONABLE)
+    try:
+        m = J.JaxModel(config=J.ModelConfig(seed=3))
+        m.add_agent_collection("a", J.AgentCollection(agent, n))
+        for k, v in env.items():
+            m.add_env_state(k, v)
+        m.initialize()
+        init = {k: v.clone() for k, v in m.agent_collections["a"].states.items()}
+        for _ in range(steps):
+            m.step()
+        torch.cuda.synchronize()
+        return init, {k: v.clone() for k, v in m.agent_collections["a"].states.items()}
+    finally:
+        fuse.enable(True)
+        jr.set_threefry_partitionable(True)
+
+
+def bits(t):
+    """Bit patterns; every NaN compares as one NaN (positions must agree).
+    torch subtracts as a + (-1) * b with a run-time multiply, the kernel as
+    a - b, whose negate modifier flips the sign bit of a NaN operand: every
+    non-NaN result is the same, a NaN result's sign bit may differ."""
+    import torch
+
+    if t.dtype == torch.float32:
+        b = t.view(torch.int32).cpu().numpy().copy()
+        b[np.isnan(t.cpu().numpy())] = 0x7FC00000
+        return b
+    return t.cpu().numpy()
+
+
+@pytest.mark.parametrize("mode", [O.MODE_ORIGINAL, O.MODE_PARTITIONABLE])
+@pytest.mark.parametrize("agent,n", [(Arith, 1000), (Arith, 257), (Draws, 1000), (Draws, 3)])
+def test_fused_equals_vmapped(J, agent, n, mode):
+    import torch
+
+    from jaxabm_amd import fuse
+
+    env = {"shift": 0.25, "scale": 1.5, "k": 3, "thr": 0.1, "on": True,
+           "delta": torch.tensor(0.125, device="cuda")}
+    before = fuse.stats()["launches"]
+    i_ref, ref = run(J, agent(), n, 4, mode, False, env)
+    mid = fuse.stats()["launches"]
+    i_got, got = run(J, agent(), n, 4, mode, True, env)
+    after = fuse.stats()["launches"]
+    assert mid == before, "the vmapped run must not use the fused path"
+    assert after - mid == 5, (after - mid, fuse.refused())     # init + 4 updates fused
+    for init_or_end, (a, b) in {"init": (i_ref, i_got), "end": (ref, got)}.items():
+        assert set(a) == set(b)
+        for k in a:
+            assert a[k].dtype == b[k].dtype and a[k].shape == b[k].shape, (init_or_end, k)
+            np.testing.assert_array_equal(bits(a[k]), bits(b[k]), err_msg=f"{init_or_end} {k}")
+    if agent is Arith:
+        x = ref["z"]
+        assert bool(torch.isnan(x).any()), "the case should exercise NaN"
+
+
+def test_refused_function_runs_vmapped(J):
+    """A per-agent vector variable is not covered: the collection runs the
+    vmapped path (and remembers it)."""
+    from jaxabm_amd import fuse
+    from jaxabm_amd import random as jr
+
+    class Vec:
+        def init_state(self, cfg, key):
+            return {"p": jr.uniform(key, (2,))}
+
+        def update(self, s, ms, cfg, key):
+            return {"p": s["p"] * 0.5}
+
+    before = fuse.stats()["launches"]
+    c = J.AgentCollection(Vec(), 50)
+    c.init(jr.PRNGKey(1), J.ModelConfig())
+    c.update({"env": {}}, jr.PRNGKey(2), J.ModelConfig())
+    assert c.states["p"].shape == (50, 2)
+    assert fuse.stats()["launches"] == before
+    assert any("non-scalar" in r or "per agent" in r for r in fuse.refused().values())
+
+
+OPS = {
+    "add_pyf": lambda s, m: s["x"] + m["env"]["shift"],
+    "mul_pyf": lambda s, m: (s["x"] + m["env"]["shift"]) * m["env"]["scale"],
+    "div_s": lambda s, m: s["x"] / 0.7, "div_env": lambda s, m: s["x"] / m["env"]["scale"],
+    "rdiv": lambda s, m: 3.0 / s["x"], "rdiv_i": lambda s, m: 3 / s["w"],
+    "div_t": lambda s, m: s["x"] / m["env"]["delta"],
+    "div_xx": lambda s, m: s["x"] / (s["x"] + 1.0), "int_div3": lambda s, m: s["w"] / 3,
+    "xsq7": lambda s, m: (s["x"] ** 2) / 7.0,
+    "fdiv_s": lambda s, m: s["x"] // 0.7, "fdiv_sn": lambda s, m: s["x"] // -1.3,
+    "fdiv_xx": lambda s, m: s["x"] // (abs(s["x"]) * 0.3 + 0.1),
+    "fdiv_env": lambda s, m: s["x"] // m["env"]["scale"],
+    "mod_env": lambda s, m: s["x"] % m["env"]["scale"],
+    "w_fdiv3": lambda s, m: s["w"] // 3, "w_mod4": lambda s, m: s["w"] % 4,
+    "wk_fdiv_m2": lambda s, m: (s["w"] * m["env"]["k"]) // -2,
+    "x_mod": lambda s, m: s["x"] % 1.3, "x_modn": lambda s, m: s["x"] % -0.9,
+    "log": "log", "min_nan": "min_nan", "max_sz": "max_sz", "min_xy": "min_xy", "clip": "clip",
+    "clip_nan": "clip_nan", "exp4": "exp4", "log1": "log1", "sqrt": "sqrt", "sin": "sin",
+    "cos": "cos", "tanh": "tanh", "floor": "floor", "ceil": "ceil", "round": "round",
+    "sign": "sign", "cast": "cast",
+    "where": lambda s, m: (s["b"] * 0 + s["w"] + m["time_step"]),
+    "flag": lambda s, m: (s["b"] & (s["x"] > m["env"]["thr"])) | ((s["w"] < 0) ^ m["env"]["on"]),
+    "delta": lambda s, m: s["x"] + m["env"]["delta"], "invert": lambda s, m: ~s["b"],
+    "half": lambda s, m: s["x"] * 0.5, "sub_t": lambda s, m: s["x"] - s["x"] * 0.5,
+}
+
+
+def _jnp_op(name):
+    from jaxabm_amd import numpy as jnp
+
+    return {
+        "log": lambda s, m: jnp.log(s["x"]),
+        "min_nan": lambda s, m: jnp.minimum(jnp.log(s["x"]), s["x"] * 0.0),
+        "max_sz": lambda s, m: jnp.maximum(s["x"] * 0.0, -(s["x"] * 0.0)),
+        "min_xy": lambda s, m: jnp.minimum(s["x"], -s["x"]),
+        "clip": lambda s, m: jnp.clip(s["x"], -1.0, 1.5),
+        "clip_nan": lambda s, m: jnp.clip(jnp.log(s["x"]), 0.0, 1.0),
+        "exp4": lambda s, m: jnp.exp(s["x"] / 4), "log1": lambda s, m: jnp.log(jnp.abs(s["x"]) + 1.0),
+        "sqrt": lambda s, m: jnp.sqrt(jnp.abs(s["x"])), "sin": lambda s, m: jnp.sin(s["x"]),
+        "cos": lambda s, m: jnp.cos(s["x"]), "tanh": lambda s, m: jnp.tanh(s["x"] * 2),
+        "floor": lambda s, m: jnp.floor(s["x"]), "ceil": lambda s, m: jnp.ceil(s["x"]),
+        "round": lambda s, m: jnp.round(s["x"] * 3), "sign": lambda s, m: jnp.sign(s["x"]),
+        "cast": lambda s, m: jnp.asarray(s["x"] * 10, dtype=jnp.int32),
+    }[name]
+
+
+@pytest.fixture(scope="module")
+def wide_inputs(J):
+    import torch
+
+    g = torch.Generator(device="cuda").manual_seed(1)
+    n = 200_000
+    x = (torch.rand(n, device="cuda", generator=g) - 0.5) * torch.exp2(
+        torch.randint(-20, 21, (n,), device="cuda", generator=g).float())
+    x[:7] = torch.tensor([float("nan"), 0.0, -0.0, float("inf"), -float("inf"), 1e30, -1e-30])
+    w = torch.randint(-1000, 1000, (n,), device="cuda", dtype=torch.int32, generator=g)
+    b = torch.arange(n, device="cuda") % 3 == 0
+    return x, w, b
+
+
+@pytest.mark.parametrize("name", list(OPS))
+def test_single_operation_fused_equals_vmapped(J, wide_inputs, name):
+    """One operation on 200,000 values over +-2^20 with NaN, signed zeros,
+    infinities and negative integer operands."""
+    import torch
+
+    from jaxabm_amd import fuse
+    from jaxabm_amd import random as jr
+
+    f = OPS[name] if callable(OPS[name]) else _jnp_op(OPS[name])
+    x, w, b = wide_inputs
+    env = {"shift": 0.25, "scale": 1.5, "k": 3, "thr": 0.1, "on": True,
+           "delta": torch.tensor(0.125, device="cuda")}
+
+    class One:
+        def init_state(self, cfg, key):
+            return {"x": 0.0}
+
+        def update(self, s, ms, cfg, key):
+            return {"o": f(s, ms)}
+
+    res = []
+    for fused in (False, True):
+        fuse.enable(fused)
+        try:
+            before = fuse.stats()["launches"]
+            c = J.AgentCollection(One(), x.shape[0])
+            c.model_config = J.ModelConfig()
+            c._states = {"x": x.clone(), "w": w.clone(), "b": b.clone()}
+            c.update({"time_step": 2, "env": env}, jr.PRNGKey(5), J.ModelConfig())
+            assert fuse.stats()["launches"] - before == (1 if fused else 0)
+            res.append(c.states["o"].clone())
+        finally:
+            fuse.enable(True)
+    a, g = res
+    assert a.dtype == g.dtype
+    np.testing.assert_array_equal(bits(a), bits(g))
