@@ -260,7 +260,8 @@ def bench_wealth(args) -> dict:
     keys = eng.key_table(S)
     mean = torch.empty(S, dtype=torch.float32, device="cuda")
     gini = torch.empty(S, dtype=torch.float32, device="cuda")
-    desc = torch.from_numpy(eng.descriptor(keys, S, mean, gini).view(np.uint8)).to("cuda")
+    draws = eng.draw_buffer(S)
+    desc = torch.from_numpy(eng.descriptor(keys, S, mean, gini, draws).view(np.uint8)).to("cuda")
     from jaxabm_amd import random as jr
 
     ik = torch.from_numpy(jr.host_split(jr.host_prng_key(0), 2, eng.mode)[1].view(np.int32)

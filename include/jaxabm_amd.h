@@ -187,6 +187,8 @@ typedef struct abm_wealth_inst {
   const uint32_t *keys;       /* device [steps][4] = (ck, uk) per step */
   int32_t steps;
   float *mean_out, *gini_out; /* device [steps] */
+  uint32_t *draws;            /* device [steps][3][n] scratch for the draws (computed
+                               * grid-wide before the run), or NULL: drawn in the run */
 } abm_wealth_inst;
 /* x, y, wealth from the collection's init key (device uint32[2]). */
 int abm_wealth_init(const uint32_t *init_key, int64_t n, float *x, float *y, int32_t *w,
@@ -195,7 +197,9 @@ int abm_wealth_init(const uint32_t *init_key, int64_t n, float *x, float *y, int
  * workgroup per instance.  max_n: the largest n in the table (0 if unknown).
  * With 0 < max_n <= 8000 every instance runs out of LDS (20 bytes per agent
  * of dynamic LDS per workgroup); otherwise, and for an instance above max_n,
- * on its global buffers.  Same results either way. */
+ * on its global buffers.  With draws, every instance's draws are computed
+ * first by a grid-wide kernel, and an instance of n < 1024 agents keeps its
+ * state in registers.  Same results either way. */
 int abm_wealth_run_batch(const abm_wealth_inst *insts, int32_t count, int64_t max_n, int mode,
                          void *stream);
 

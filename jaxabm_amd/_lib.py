@@ -106,7 +106,8 @@ class SirGraph(C.Structure):
 # Device-side descriptor layouts (numpy structured dtypes with C alignment).
 WEALTH_INST = np.dtype([("n", np.int64), ("x", np.uint64), ("y", np.uint64), ("w", np.uint64),
                         ("inc", np.uint64), ("hist", np.uint64), ("keys", np.uint64),
-                        ("steps", np.int32), ("mean_out", np.uint64), ("gini_out", np.uint64)],
+                        ("steps", np.int32), ("mean_out", np.uint64), ("gini_out", np.uint64),
+                        ("draws", np.uint64)],
                        align=True)
 
 

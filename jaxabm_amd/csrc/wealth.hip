@@ -52,32 +52,70 @@ __device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t *red) {
 
 // The whole run of one instance on the arrays x, y, w (state), inc and hist
 // (zero on entry, left zero) -- global buffers or their LDS copies.
+// The draws of agent i at one step: the position increments 0.01 * normal
+// (rounded as the reference's product) and the exchange target.
+struct WealthDraw {
+  float mdx, mdy;
+  uint32_t target;
+};
+
+__device__ __forceinline__ WealthDraw wealth_draw(const uint32_t *keys, int32_t t, int64_t n,
+                                                  int64_t i, int mode) {
+  const Key ck{keys[4 * t + 0], keys[4 * t + 1]};
+  const Key uk{keys[4 * t + 2], keys[4 * t + 3]};
+  Key kk1, kk2, k1, k2;
+  split2(uk, kk1, kk2, mode);
+  split2(split_at(ck, n, i, mode), k1, k2, mode);
+  WealthDraw d;
+  d.mdx = 0.01f * normal_from_bits(bits_scalar(k1, mode));
+  d.mdy = 0.01f * normal_from_bits(bits_scalar(k2, mode));
+  const uint32_t span = n > 1 ? (uint32_t)(n - 1) : 1u;
+  uint32_t ti = randint_offset(bits_at(kk1, n, i, mode), bits_at(kk2, n, i, mode), span);
+  d.target = ti + (ti >= (uint32_t)i ? 1u : 0u);
+  return d;
+}
+
+// Every draw of a run, grid-wide, into draws[steps][3][n] (mdx, mdy bits,
+// target): none depends on the state, so the sequential run only applies
+// them.
+__global__ void __launch_bounds__(256)
+    k_wealth_draws(const abm_wealth_inst *__restrict__ insts, int mode) {
+  const abm_wealth_inst in = insts[blockIdx.y];
+  if (!in.draws) return;
+  const int64_t n = in.n;
+  const int64_t total = (int64_t)in.steps * n;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    const int32_t t = (int32_t)(e / n);
+    const int64_t i = e - (int64_t)t * n;
+    const WealthDraw d = wealth_draw(in.keys, t, n, i, mode);
+    uint32_t *row = in.draws + (int64_t)t * 3 * n;
+    row[i] = __float_as_uint(d.mdx);
+    row[n + i] = __float_as_uint(d.mdy);
+    row[2 * n + i] = d.target;
+  }
+}
+
 __device__ __forceinline__ void wealth_steps(const abm_wealth_inst &in, float *gx, float *gy,
                                              int32_t *gw, int32_t *ginc, int32_t *ghist,
                                              uint32_t *red, uint32_t &carry, int mode) {
   const int64_t n = in.n;
   const int tid = threadIdx.x;
-  const uint32_t span = n > 1 ? (uint32_t)(n - 1) : 1u;
   for (int32_t t = 0; t < in.steps; ++t) {
-    const Key ck{in.keys[4 * t + 0], in.keys[4 * t + 1]};
-    const Key uk{in.keys[4 * t + 2], in.keys[4 * t + 3]};
-    Key kk1, kk2;
-    split2(uk, kk1, kk2, mode);
     // agent phase + exchange targets
     for (int64_t i = tid; i < n; i += kWealthThreads) {
-      Key k1, k2;
-      split2(split_at(ck, n, i, mode), k1, k2, mode);
-      const float dx = normal_from_bits(bits_scalar(k1, mode));
-      const float dy = normal_from_bits(bits_scalar(k2, mode));
-      const float sx = gx[i] + 0.01f * dx;
-      const float sy = gy[i] + 0.01f * dy;
-      gx[i] = mod1(sx);
-      gy[i] = mod1(sy);
-      if (gw[i] > 0) {
-        uint32_t ti = randint_offset(bits_at(kk1, n, i, mode), bits_at(kk2, n, i, mode), span);
-        ti += ti >= (uint32_t)i ? 1u : 0u;
-        atomicAdd(&ginc[ti], 1);
+      WealthDraw d;
+      if (in.draws) {
+        const uint32_t *row = in.draws + (int64_t)t * 3 * n;
+        d.mdx = __uint_as_float(row[i]);
+        d.mdy = __uint_as_float(row[n + i]);
+        d.target = row[2 * n + i];
+      } else {
+        d = wealth_draw(in.keys, t, n, i, mode);
       }
+      gx[i] = mod1(gx[i] + d.mdx);
+      gy[i] = mod1(gy[i] + d.mdy);
+      if (gw[i] > 0) atomicAdd(&ginc[d.target], 1);
     }
     __syncthreads();
     // apply the exchange, histogram of wealth values
@@ -140,6 +178,89 @@ __device__ __forceinline__ void wealth_steps(const abm_wealth_inst &in, float *g
   }
 }
 
+// n + 1 <= kWealthThreads with precomputed draws: one agent and one
+// histogram bin per thread, x, y and wealth in registers for the whole run,
+// increments and histogram in LDS; four barriers per step.  T and S are
+// reduced together.
+__device__ __forceinline__ void wealth_steps_small(const abm_wealth_inst &in, int32_t *linc,
+                                                   int32_t *lhist, uint32_t *red_c,
+                                                   uint32_t *red_t, uint32_t *red_s) {
+  const int64_t n = in.n;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool agent = tid < n;
+  float x = agent ? in.x[tid] : 0.0f, y = agent ? in.y[tid] : 0.0f;
+  int32_t w = agent ? in.w[tid] : 0;
+  if (tid <= n) {
+    linc[tid] = 0;
+    lhist[tid] = 0;
+  }
+  __syncthreads();
+  for (int32_t t = 0; t < in.steps; ++t) {
+    if (agent) {
+      const uint32_t *row = in.draws + (int64_t)t * 3 * n;
+      x = mod1(x + __uint_as_float(row[tid]));
+      y = mod1(y + __uint_as_float(row[n + tid]));
+      if (w > 0) atomicAdd(&linc[row[2 * n + tid]], 1);
+    }
+    __syncthreads();
+    if (agent) {
+      w = w - (w > 0 ? 1 : 0) + linc[tid];
+      linc[tid] = 0;
+      atomicAdd(&lhist[w], 1);  // 0 <= w <= n: wealth is conserved
+    }
+    __syncthreads();
+    // bin `tid` of the histogram: agents of wealth tid and their rank sum
+    const uint32_t c = tid <= n ? (uint32_t)lhist[tid] : 0u;
+    if (c) lhist[tid] = 0;
+    uint32_t inc = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t v = __shfl_up(inc, off);
+      if (lane >= off) inc += v;
+    }
+    uint32_t tw = agent ? (uint32_t)w : 0u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) tw += __shfl_xor(tw, off);
+    if (lane == 63) red_c[wave] = inc;
+    if (lane == 0) red_t[wave] = tw;
+    __syncthreads();
+    uint32_t before = inc - c;
+    for (int q = 0; q < wave; ++q) before += red_c[q];
+    uint32_t sp = 0;
+    if (c) {
+      const uint64_t ranksum = (uint64_t)c * before + (uint64_t)c * (c + 1) / 2;
+      sp = (uint32_t)tid * (uint32_t)ranksum;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) sp += __shfl_xor(sp, off);
+    if (lane == 0) red_s[wave] = sp;
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t T = 0, S = 0;
+      for (int q = 0; q < kWealthThreads / 64; ++q) {
+        T += red_t[q];
+        S += red_s[q];
+      }
+      in.mean_out[t] = (float)(int32_t)T / (float)n;
+      const int32_t Ti = (int32_t)T;
+      float g = 0.0f;
+      if ((float)Ti > 1e-6f) {
+        const int32_t num = (int32_t)(2u * S);
+        const int32_t den = (int32_t)((uint32_t)n * T);
+        const float q = (float)num / (float)den;
+        const float cst = (float)((double)(n + 1) / (double)n);
+        g = q - cst;
+      }
+      in.gini_out[t] = g;
+    }
+  }
+  if (agent) {
+    in.x[tid] = x;
+    in.y[tid] = y;
+    in.w[tid] = w;
+  }
+}
+
 // kLds: the launch carries dynamic LDS for max_lds agents; instances with
 // n <= max_lds run out of it, others on their global buffers.
 template <bool kLds>
@@ -148,10 +269,20 @@ __global__ void __launch_bounds__(kWealthThreads)
   extern __shared__ int32_t lds_agents[];
   __shared__ uint32_t red[16];
   __shared__ uint32_t carry;
+  __shared__ uint32_t red_t[kWealthThreads / 64], red_s[kWealthThreads / 64];
   const abm_wealth_inst in = insts[blockIdx.x];
   const int64_t n = in.n;
   const int tid = threadIdx.x;
-  if (kLds && n <= max_lds) {
+  if (in.draws && n + 1 <= kWealthThreads) {
+    // increments and histogram: the dynamic LDS (at least 2 x kWealthThreads
+    // words in this launch), or a static array where the launch has none
+    if constexpr (kLds) {
+      wealth_steps_small(in, lds_agents, lds_agents + kWealthThreads, red, red_t, red_s);
+    } else {
+      __shared__ int32_t small_lds[2 * kWealthThreads];
+      wealth_steps_small(in, small_lds, small_lds + kWealthThreads, red, red_t, red_s);
+    }
+  } else if (kLds && n <= max_lds) {
     float *lx = reinterpret_cast<float *>(lds_agents);
     float *ly = lx + n;
     int32_t *lw = reinterpret_cast<int32_t *>(ly + n);
@@ -211,6 +342,8 @@ int abm_wealth_run_batch(const abm_wealth_inst *insts, int32_t count, int64_t ma
   ABM_REQUIRE(insts && count >= 0 && max_n >= 0 && (mode == 0 || mode == 1),
               "abm_wealth_run_batch: bad arguments");
   if (count == 0) return kOk;
+  hipLaunchKernelGGL(k_wealth_draws, dim3(512u, (unsigned)count), dim3(256), 0,
+                     (hipStream_t)stream, insts, mode);
   if (max_n > 0 && max_n <= kWealthLdsAgents) {
     static bool attr_set = false;
     if (!attr_set) {
@@ -219,9 +352,10 @@ int abm_wealth_run_batch(const abm_wealth_inst *insts, int32_t count, int64_t ma
                                         (int)(kWealthLdsAgents * kWealthLdsBytesPerAgent + 4)));
       attr_set = true;
     }
-    hipLaunchKernelGGL(k_wealth_run<true>, dim3((unsigned)count), dim3(kWealthThreads),
-                       (size_t)(max_n * kWealthLdsBytesPerAgent + 4), (hipStream_t)stream,
-                       insts, max_n, mode);
+    size_t dyn = (size_t)(max_n * kWealthLdsBytesPerAgent + 4);
+    if (dyn < 2 * kWealthThreads * sizeof(int32_t)) dyn = 2 * kWealthThreads * sizeof(int32_t);
+    hipLaunchKernelGGL(k_wealth_run<true>, dim3((unsigned)count), dim3(kWealthThreads), dyn,
+                       (hipStream_t)stream, insts, max_n, mode);
   } else {
     hipLaunchKernelGGL(k_wealth_run<false>, dim3((unsigned)count), dim3(kWealthThreads), 0,
                        (hipStream_t)stream, insts, (int64_t)0, mode);

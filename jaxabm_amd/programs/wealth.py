@@ -53,14 +53,19 @@ class WealthEngine:
         tab = jr.step_key_table(self.rng, steps, 1, True, False, self.mode)
         return torch.from_numpy(np.ascontiguousarray(tab).view(np.int32)).to(self.device)
 
+    def draw_buffer(self, steps: int) -> torch.Tensor:
+        """Scratch for the run's draws: [steps][3][n] 32-bit words."""
+        return torch.empty(max(steps, 1) * 3 * self.n, dtype=torch.int32, device=self.device)
+
     def descriptor(self, keys: torch.Tensor, steps: int, mean: torch.Tensor,
-                   gini: torch.Tensor) -> np.ndarray:
+                   gini: torch.Tensor, draws: torch.Tensor | None = None) -> np.ndarray:
         d = np.zeros(1, WEALTH_INST)
         d["n"] = self.n
         d["x"], d["y"], d["w"] = self.x.data_ptr(), self.y.data_ptr(), self.w.data_ptr()
         d["inc"], d["hist"] = self.inc.data_ptr(), self.hist.data_ptr()
         d["keys"], d["steps"] = keys.data_ptr(), steps
         d["mean_out"], d["gini_out"] = mean.data_ptr(), gini.data_ptr()
+        d["draws"] = draws.data_ptr() if draws is not None else 0
         return d
 
     def run(self, steps: int):
@@ -68,7 +73,9 @@ class WealthEngine:
         keys = self.key_table(steps)
         mean = torch.empty(max(steps, 1), dtype=torch.float32, device=self.device)
         gini = torch.empty(max(steps, 1), dtype=torch.float32, device=self.device)
-        desc = torch.from_numpy(self.descriptor(keys, steps, mean, gini).view(np.uint8)).to(self.device)
+        draws = self.draw_buffer(steps)
+        desc = torch.from_numpy(self.descriptor(keys, steps, mean, gini, draws)
+                                .view(np.uint8)).to(self.device)
         check(lib().abm_wealth_run_batch(ptr(desc), 1, self.n, self.mode, stream_handle()),
               "wealth_run")
         return mean[:steps].cpu().numpy(), gini[:steps].cpu().numpy()
@@ -88,8 +95,9 @@ def run_batch(engines: list[WealthEngine], steps: list[int], stream=None):
         keys = e.key_table(s)
         mean = torch.empty(max(s, 1), dtype=torch.float32, device=dev)
         gini = torch.empty(max(s, 1), dtype=torch.float32, device=dev)
-        keep += [keys, mean, gini]
-        descs.append(e.descriptor(keys, s, mean, gini))
+        draws = e.draw_buffer(s)
+        keep += [keys, mean, gini, draws]
+        descs.append(e.descriptor(keys, s, mean, gini, draws))
         outs.append((mean, gini, s))
     table = torch.from_numpy(np.concatenate(descs).view(np.uint8)).to(dev)
     max_n = max(e.n for e in engines)

@@ -32,7 +32,8 @@ def torus_close(a, b, tol=1e-5):
 
 
 @pytest.mark.parametrize("mode", MODES)
-@pytest.mark.parametrize("n,steps,seed", [(1000, 100, 0), (2, 10, 1), (3000, 20, 5), (1025, 7, 9)])
+@pytest.mark.parametrize("n,steps,seed", [(1000, 100, 0), (2, 10, 1), (3000, 20, 5), (1025, 7, 9),
+                                          (1023, 12, 4)])
 def test_wealth_matches_oracle(wealth, mode, n, steps, seed):
     eng = wealth.WealthEngine(n, seed=seed, mode=mode)
     x, y, w, rng = O.wealth_init(n, seed, mode)
@@ -58,28 +59,34 @@ def test_wealth_batch_equals_alone(wealth):
         np.testing.assert_array_equal(g1, g2)
 
 
-def _run_raw(wealth, eng, steps, max_n):
-    """One instance through abm_wealth_run_batch with an explicit max_n."""
+def _run_raw(wealth, eng, steps, max_n, draws=False):
+    """One instance through abm_wealth_run_batch with an explicit max_n, with
+    or without the precomputed-draw buffer."""
     import torch
     from jaxabm_amd._lib import check, lib, ptr, stream_handle
 
     keys = eng.key_table(steps)
     mean = torch.empty(steps, dtype=torch.float32, device=eng.device)
     gini = torch.empty(steps, dtype=torch.float32, device=eng.device)
-    desc = torch.from_numpy(eng.descriptor(keys, steps, mean, gini).view(np.uint8)).to(eng.device)
+    buf = eng.draw_buffer(steps) if draws else None
+    desc = torch.from_numpy(eng.descriptor(keys, steps, mean, gini, buf).view(np.uint8)).to(eng.device)
     check(lib().abm_wealth_run_batch(ptr(desc), 1, max_n, eng.mode, stream_handle()), "run")
     return mean.cpu().numpy(), gini.cpu().numpy()
 
 
-@pytest.mark.parametrize("n,max_n", [(1000, 0), (1000, 500), (5000, 8000), (8000, 8000)])
-def test_wealth_lds_and_global_paths_agree(wealth, n, max_n):
-    """LDS-resident runs (n <= max_n <= 8000) and global-buffer runs (max_n = 0,
-    or an instance above max_n) give identical records and state."""
+@pytest.mark.parametrize("draws", [False, True])
+@pytest.mark.parametrize("n,max_n", [(1000, 0), (1000, 500), (5000, 8000), (8000, 8000),
+                                     (300, 100)])
+def test_wealth_lds_and_global_paths_agree(wealth, n, max_n, draws):
+    """Every path gives identical records and state: draws computed in the
+    run or grid-wide before it; small instances with state in registers;
+    LDS-resident runs (n <= max_n <= 8000) and global-buffer runs (max_n = 0,
+    or an instance above max_n)."""
     steps = 40
     a = wealth.WealthEngine(n, seed=3)
     b = wealth.WealthEngine(n, seed=3)
-    ma, ga = a.run(steps)                     # max_n = n: resident when n <= 8000
-    mb, gb = _run_raw(wealth, b, steps, max_n)
+    ma, ga = a.run(steps)                     # precomputed draws, max_n = n
+    mb, gb = _run_raw(wealth, b, steps, max_n, draws)
     np.testing.assert_array_equal(ma, mb)
     np.testing.assert_array_equal(ga, gb)
     for t in ("w", "x", "y"):
