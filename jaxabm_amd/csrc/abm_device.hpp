@@ -119,8 +119,11 @@ ABM_HD uint32_t randint_offset(uint32_t hb, uint32_t lb, uint32_t span) {
 }
 
 // jnp.mod(x, 1.0): truncated remainder, plus one when negative and non-zero.
+// For finite x the truncated remainder fmod(x, 1) is x - trunc(x) exactly
+// (the fraction bits of x, subtracted without rounding) with the sign of x
+// (-0 for a negative integer): two instructions instead of fmodf's loop.
 __device__ __forceinline__ float mod1(float x) {
-  float r = fmodf(x, 1.0f);
+  float r = copysignf(x - truncf(x), x);
   if (r != 0.0f && r < 0.0f) r = r + 1.0f;
   return r;
 }

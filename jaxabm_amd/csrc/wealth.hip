@@ -194,13 +194,28 @@ __device__ __forceinline__ void wealth_steps_small(const abm_wealth_inst &in, in
     linc[tid] = 0;
     lhist[tid] = 0;
   }
+  // the draws of step t + 1 are loaded during step t (barriers wait on LDS
+  // only, so the loads overlap the histogram phases)
+  uint32_t dx = 0u, dy = 0u, dt = 0u;
+  if (agent && in.steps > 0) {
+    dx = in.draws[tid];
+    dy = in.draws[n + tid];
+    dt = in.draws[2 * n + tid];
+  }
   __syncthreads();
   for (int32_t t = 0; t < in.steps; ++t) {
+    const float mdx = __uint_as_float(dx), mdy = __uint_as_float(dy);
+    const uint32_t target = dt;
+    if (agent && t + 1 < in.steps) {
+      const uint32_t *row = in.draws + (int64_t)(t + 1) * 3 * n;
+      dx = row[tid];
+      dy = row[n + tid];
+      dt = row[2 * n + tid];
+    }
     if (agent) {
-      const uint32_t *row = in.draws + (int64_t)t * 3 * n;
-      x = mod1(x + __uint_as_float(row[tid]));
-      y = mod1(y + __uint_as_float(row[n + tid]));
-      if (w > 0) atomicAdd(&linc[row[2 * n + tid]], 1);
+      x = mod1(x + mdx);
+      y = mod1(y + mdy);
+      if (w > 0) atomicAdd(&linc[target], 1);
     }
     __syncthreads();
     if (agent) {
@@ -224,8 +239,11 @@ __device__ __forceinline__ void wealth_steps_small(const abm_wealth_inst &in, in
     if (lane == 63) red_c[wave] = inc;
     if (lane == 0) red_t[wave] = tw;
     __syncthreads();
-    uint32_t before = inc - c;
-    for (int q = 0; q < wave; ++q) before += red_c[q];
+    // bins before this wave: lanes below `wave` each read one wave total
+    uint32_t below = lane < wave ? red_c[lane] : 0u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) below += __shfl_xor(below, off);
+    const uint32_t before = below + inc - c;
     uint32_t sp = 0;
     if (c) {
       const uint64_t ranksum = (uint64_t)c * before + (uint64_t)c * (c + 1) / 2;
