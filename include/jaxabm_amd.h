@@ -352,6 +352,31 @@ int abm_graph_sanitize(void *graph, int32_t *counts);
 /* Stream-ordered device-to-device copy (capturable). */
 int abm_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
 
+/* ---- record-time reductions over one agent variable (jaxabm/model.py:194-200,
+ *      jaxabm/agentpy.py:926-942, jaxabm/agent.py:198-211; Gini as
+ *      examples/models/advanced_economic_model.py:1798-1811).  Device inputs
+ *      and outputs, no host synchronisation, no memset (graph-capturable). ---- */
+/* counts[v] = number of states[i] == v for v < k (1 <= k <= 256), exact;
+ * states may be NULL when n == 0. */
+int abm_reduce_counts_u8(const uint8_t *states, int64_t n, int32_t k, int32_t *counts,
+                         void *stream);
+/* Scratch bytes for any reduction below over n values. */
+size_t abm_reduce_workspace(int64_t n);
+/* *out = mean: float32 input summed in float64 in a fixed order, rounded once;
+ * int32 input summed exactly, then f32(sum) / f32(n) (jnp.mean of int32). */
+int abm_mean_f32(const float *x, int64_t n, float *out, void *ws, size_t ws_bytes,
+                 void *stream);
+int abm_mean_i32(const int32_t *x, int64_t n, float *out, void *ws, size_t ws_bytes,
+                 void *stream);
+/* *out = Gini of int32 values: sorted x, S = sum (i+1) x_(i), T = sum x in
+ * int32 (wrapping), G = f32(2S) / f32(nT) - f32((n+1)/n), 0 when T <= 1e-6. */
+int abm_gini_i32(const int32_t *x, int64_t n, float *out, void *ws, size_t ws_bytes,
+                 void *stream);
+/* out[k] (float64, k < bins <= 256) = mean over i of
+ * sigmoid(slope (x_i - k/bins)) - sigmoid(slope (x_i - (k+1)/bins)). */
+int abm_soft_hist_f32(const float *x, int64_t n, int32_t bins, double slope, double *out,
+                      void *ws, size_t ws_bytes, void *stream);
+
 /* ---- generated agent-update kernels (replace the jax.vmap of a per-agent
  *      function in AgentCollection.update, jaxabm/agent.py:156-177, which XLA
  *      fuses) ---- */

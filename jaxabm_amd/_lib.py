@@ -180,6 +180,12 @@ SIGNATURES = {
     "abm_graph_launch": (_int, [_p, _p]),
     "abm_graph_destroy": (_int, [_p]),
     "abm_graph_sanitize": (_int, [_p, _p]),
+    "abm_reduce_counts_u8": (_int, [_p, _i64, _i32, _p, _p]),
+    "abm_reduce_workspace": (_sz, [_i64]),
+    "abm_mean_f32": (_int, [_p, _i64, _p, _p, _sz, _p]),
+    "abm_mean_i32": (_int, [_p, _i64, _p, _p, _sz, _p]),
+    "abm_gini_i32": (_int, [_p, _i64, _p, _p, _sz, _p]),
+    "abm_soft_hist_f32": (_int, [_p, _i64, _i32, C.c_double, _p, _p, _sz, _p]),
     "abm_rtc_compile": (_int, [C.c_char_p, C.c_char_p, C.POINTER(_p), C.POINTER(C.c_size_t)]),
     "abm_rtc_code_free": (_int, [_p]),
     "abm_rtc_load": (_int, [_p, C.c_char_p, C.POINTER(_p)]),

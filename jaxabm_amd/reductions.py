@@ -1,0 +1,91 @@
+"""Record-time reductions over one agent variable on the GPU: state counts,
+mean, Gini and the soft histogram.
+
+Replaces the metric reductions a reference model computes each step with
+jnp (jaxabm/model.py:194-200 metrics_fn, jaxabm/agentpy.py:926-942
+compute_metrics, jaxabm/agent.py:198-211 aggregate) and the Gini definition
+of examples/models/advanced_economic_model.py:1798-1811.  Results are device
+tensors: nothing waits on the host, and the kernels clear their outputs and
+scratch with fill kernels (no memset), so a step that uses them can be
+replayed as a graph (``Model.run(graph=True)``).
+
+* ``state_counts(s, k)``: int32 counts of each value < k of a uint8 array,
+  exact.
+* ``mean(x)``: int32 input as the reference's jnp.mean of int32 (exact sum,
+  then float32(sum) / float32(n)); float32 input summed in float64 in a fixed
+  order and rounded once (deterministic; the reference's float32 summation
+  order is XLA's, so float means agree to rounding).
+* ``gini(x)``: int32 input, the reference's formula bit for bit (int32 sums
+  with wrap-around, float32 quotient).
+* ``soft_histogram(x, bins, slope)``: float64 bin means of sigmoid
+  differences with edges k / bins (the Deffuant calibration statistic).
+"""
+from __future__ import annotations
+
+import torch
+
+from ._lib import check, lib, ptr, require_gpu, stream_handle
+
+
+
+def _workspace(n: int, device) -> torch.Tensor:
+    nbytes = int(lib().abm_reduce_workspace(int(n)))
+    return torch.empty(nbytes, dtype=torch.uint8, device=device)
+
+
+def _check_input(x: torch.Tensor, dtypes) -> torch.Tensor:
+    require_gpu()
+    if not isinstance(x, torch.Tensor) or not x.is_cuda:
+        raise TypeError("reductions take a GPU tensor")
+    if x.dtype not in dtypes:
+        raise TypeError(f"dtype {x.dtype} not supported here (expected one of {dtypes})")
+    x = x.reshape(-1)
+    return x if x.is_contiguous() else x.contiguous()
+
+
+def state_counts(states: torch.Tensor, k: int) -> torch.Tensor:
+    """int32[k]: how many entries of ``states`` (uint8) equal 0, 1, ..., k-1."""
+    s = _check_input(states, (torch.uint8,))
+    out = torch.empty(int(k), dtype=torch.int32, device=s.device)
+    check(lib().abm_reduce_counts_u8(ptr(s), s.numel(), int(k), ptr(out), stream_handle()),
+          "reduce_counts_u8")
+    return out
+
+
+def mean(x: torch.Tensor) -> torch.Tensor:
+    """0-d float32 mean of an int32 or float32 array."""
+    v = _check_input(x, (torch.int32, torch.float32))
+    if v.numel() == 0:
+        raise ValueError("mean of an empty array")
+    out = torch.empty((), dtype=torch.float32, device=v.device)
+    ws = _workspace(v.numel(), v.device)
+    fn = lib().abm_mean_i32 if v.dtype == torch.int32 else lib().abm_mean_f32
+    check(fn(ptr(v), v.numel(), ptr(out), ptr(ws), ws.numel(), stream_handle()), "mean")
+    return out
+
+
+def gini(x: torch.Tensor) -> torch.Tensor:
+    """0-d float32 Gini coefficient of an int32 array (the reference's
+    definition: 2 sum_i i x_(i) / (n sum x) - (n + 1) / n, 0 when the sum is
+    not positive)."""
+    v = _check_input(x, (torch.int32,))
+    if v.numel() == 0:
+        raise ValueError("gini of an empty array")
+    out = torch.empty((), dtype=torch.float32, device=v.device)
+    ws = _workspace(v.numel(), v.device)
+    check(lib().abm_gini_i32(ptr(v), v.numel(), ptr(out), ptr(ws), ws.numel(), stream_handle()),
+          "gini_i32")
+    return out
+
+
+def soft_histogram(x: torch.Tensor, bins: int = 32, slope: float = 128.0) -> torch.Tensor:
+    """float64[bins]: mean over the values of sigmoid(slope (x - k/bins)) -
+    sigmoid(slope (x - (k+1)/bins))."""
+    v = _check_input(x, (torch.float32,))
+    if v.numel() == 0:
+        raise ValueError("soft histogram of an empty array")
+    out = torch.empty(int(bins), dtype=torch.float64, device=v.device)
+    ws = _workspace(v.numel(), v.device)
+    check(lib().abm_soft_hist_f32(ptr(v), v.numel(), int(bins), float(slope), ptr(out), ptr(ws),
+                                  ws.numel(), stream_handle()), "soft_hist_f32")
+    return out

@@ -1,0 +1,107 @@
+"""GPU: record-time reductions (jaxabm_amd.reductions) against the oracle --
+Gini and the int32 mean bit for bit (the reference's int32 / float32
+formulas), counts exactly, the float32 mean to its correctly rounded value
+within one ulp, the soft histogram at 1e-10 -- and inside a step replayed as
+a graph (no memset nodes)."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def R():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from jaxabm_amd import reductions
+
+    return reductions
+
+
+def _dev(a):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 1000, 100_003])
+def test_gini_and_int_mean_match_oracle(R, n):
+    rs = np.random.default_rng(n)
+    cases = [rs.integers(0, 50, n), rs.integers(-20, 20, n), np.zeros(n),
+             rs.integers(0, 2**30, n),        # sums wrap around int32
+             np.ones(n)]
+    for v in cases:
+        v = v.astype(np.int32)
+        assert R.gini(_dev(v)).item() == O.gini_i32(v), v[:8]
+        assert R.mean(_dev(v)).item() == O.mean_i32(v)
+
+
+def test_float_mean_correctly_rounded(R):
+    rs = np.random.default_rng(3)
+    for n in (1, 33, 65_536, 1_000_003):
+        x = (rs.standard_normal(n) * 10.0 ** rs.integers(-3, 4, n)).astype(np.float32)
+        got = np.float32(R.mean(_dev(x)).item())
+        exact = np.float32(x.astype(np.float64).sum() / n)
+        assert got == exact or abs(int(got.view(np.int32)) - int(exact.view(np.int32))) <= 1
+
+
+@pytest.mark.parametrize("n,k", [(0, 3), (1, 1), (1000, 3), (1_000_000, 3), (4097, 256)])
+def test_state_counts(R, n, k):
+    rs = np.random.default_rng(n + k)
+    s = rs.integers(0, min(k + 2, 256), n).astype(np.uint8)   # values >= k are not counted
+    got = R.state_counts(_dev(s), k).cpu().numpy()
+    want = np.bincount(s, minlength=256)[:k]
+    np.testing.assert_array_equal(got, want)
+
+
+def test_soft_histogram_matches_deffuant_statistics(R):
+    rs = np.random.default_rng(5)
+    for n in (1, 1000, 65_536):
+        x = rs.random(n).astype(np.float32)
+        got = R.soft_histogram(_dev(x), 32, 128.0).cpu().numpy()
+        want = O.deffuant_stats(x.reshape(n, 1))[0, 2:34]
+        np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-14)
+        assert abs(got.sum() - want.sum()) < 1e-12
+
+
+def test_reductions_in_a_replayed_step(R):
+    """A generic model whose metrics use these reductions replays as a graph
+    with no memset node and the eager loop's results."""
+    import torch
+
+    import jaxabm_amd as J
+    from jaxabm_amd import numpy as jnp
+    from jaxabm_amd import random as jr
+
+    class Giver:
+        def init_state(self, cfg, key):
+            return {"w": jr.randint(key, (), 0, 5), "s": jr.randint(key, (), 0, 3)}
+
+        def update(self, state, ms, cfg, key):
+            k1, k2 = jr.split(key)
+            w = jnp.where(jr.uniform(k1) < 0.5, state["w"] + 1, jnp.maximum(state["w"] - 1, 0))
+            return {"w": w, "s": jr.randint(k2, (), 0, 3)}
+
+    def metrics(env, agents, params):
+        a = agents["g"]
+        return {"gini": R.gini(a["w"]), "mean": R.mean(a["w"]),
+                "counts": R.state_counts(a["s"].to(torch.uint8), 3)}
+
+    def make():
+        m = J.JaxModel(config=J.ModelConfig(seed=2), metrics_fn=metrics)
+        m.add_agent_collection("g", J.AgentCollection(Giver(), 50_000))
+        return m
+
+    a, b = make(), make()
+    ra, rb = a.run(12), b.run(12, graph=True)
+    info = b.last_run_info
+    assert info["mode"] == "graph" and info["memsets_replaced"] == 0, info
+    for k in ("gini", "mean", "counts"):
+        for i, (x, y) in enumerate(zip(ra[k], rb[k])):
+            np.testing.assert_array_equal(x.cpu().numpy(), y.cpu().numpy(), err_msg=f"{k} {i}")
+    w = a.agent_collections["g"].states["w"].cpu().numpy()
+    assert ra["gini"][-1].item() == O.gini_i32(w)
