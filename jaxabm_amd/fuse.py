@@ -245,12 +245,12 @@ class Sym:
         kwargs = kwargs or {}
         tr = None
         for a in list(args) + list(kwargs.values()):
-            if isinstance(a, Sym):
+            if isinstance(a, (Sym, SymVec)):
                 tr = a.tr
                 break
             if isinstance(a, (list, tuple)):
                 for b in a:
-                    if isinstance(b, Sym):
+                    if isinstance(b, (Sym, SymVec)):
                         tr = b.tr
         if tr is None:
             raise Unsupported(f"{getattr(func, '__name__', func)}")
@@ -270,8 +270,19 @@ class SymKey:
     def shape(self):
         return torch.Size((2,))
 
-    def _bits(self) -> str:
-        return self.tr.tmp("uint32_t", f"abm::bits_scalar({self.name}, {self.tr.mode})")
+    def _bits(self, per: int = 1, j: int = 0) -> str:
+        if per == 1:
+            return self.tr.tmp("uint32_t", f"abm::bits_scalar({self.name}, {self.tr.mode})")
+        return self.tr.tmp("uint32_t", f"abm::bits_at({self.name}, {per}LL, {j}LL, {self.tr.mode})")
+
+    def _shaped(self, shape, one):
+        """one(per, j) for every element of a draw of ``shape`` (() or (k,))."""
+        shape = tuple(shape)
+        if shape == ():
+            return one(1, 0)
+        if len(shape) != 1 or not 0 < shape[0] <= _MAX_VEC:
+            raise Unsupported("a draw of more than one dimension or 16 elements")
+        return SymVec(self.tr, [one(shape[0], j) for j in range(shape[0])])
 
     def split(self, num: int) -> "SymKeyArray":
         num = int(num)
@@ -290,34 +301,45 @@ class SymKey:
             out.append(SymKey(tr, k))
         return SymKeyArray(out)
 
-    def bits(self) -> Sym:
-        return Sym(self.tr, self.tr.tmp("int32_t", f"(int32_t){self._bits()}"), "i32", True)
+    def bits(self, shape=()):
+        return self._shaped(shape, lambda per, j: Sym(
+            self.tr, self.tr.tmp("int32_t", f"(int32_t){self._bits(per, j)}"), "i32", True))
 
-    def uniform(self, minval, maxval) -> Sym:
+    def uniform(self, minval, maxval, shape=()):
         tr = self.tr
-        if isinstance(minval, (Sym, torch.Tensor)) or isinstance(maxval, (Sym, torch.Tensor)):
-            f = self.uniform(0.0, 1.0)
+        if isinstance(minval, (Sym, SymVec, torch.Tensor)) or \
+                isinstance(maxval, (Sym, SymVec, torch.Tensor)):
+            f = self.uniform(0.0, 1.0, shape)
             lo, hi = tr.bound(minval), tr.bound(maxval)
             return tr.torch_call(torch.maximum, (lo, f * (hi - lo) + lo), {})
         lo = np.float32(float(minval))
         span = np.float32(np.float32(float(maxval)) - lo)
-        expr = (f"abm::scale_uniform(abm::unit_float({self._bits()}), "
-                f"{_lit(float(lo), 'f32')}, {_lit(float(span), 'f32')})")
-        return Sym(tr, tr.tmp("float", expr), "f32", True)
 
-    def normal(self) -> Sym:
-        return Sym(self.tr, self.tr.tmp("float", f"abm::normal_from_bits({self._bits()})"),
-                   "f32", True)
+        def one(per, j):
+            expr = (f"abm::scale_uniform(abm::unit_float({self._bits(per, j)}), "
+                    f"{_lit(float(lo), 'f32')}, {_lit(float(span), 'f32')})")
+            return Sym(tr, tr.tmp("float", expr), "f32", True)
 
-    def randint(self, lo: int, hi: int) -> Sym:
+        return self._shaped(shape, one)
+
+    def normal(self, shape=()):
+        return self._shaped(shape, lambda per, j: Sym(
+            self.tr, self.tr.tmp("float", f"abm::normal_from_bits({self._bits(per, j)})"),
+            "f32", True))
+
+    def randint(self, lo: int, hi: int, shape=()):
         tr = self.tr
         span = 1 if hi <= lo else hi - lo
         a, b = tr.fresh("k"), tr.fresh("k")
         tr.lines.append(f"abm::Key {a}, {b}; abm::split2({self.name}, {a}, {b}, {tr.mode});")
-        hb = tr.tmp("uint32_t", f"abm::bits_at({a}, 1LL, 0LL, {tr.mode})")
-        lb = tr.tmp("uint32_t", f"abm::bits_at({b}, 1LL, 0LL, {tr.mode})")
-        expr = f"(int32_t)((uint32_t)({lo}) + abm::randint_offset({hb}, {lb}, {span}u))"
-        return Sym(tr, tr.tmp("int32_t", expr), "i32", True)
+
+        def one(per, j):
+            hb = tr.tmp("uint32_t", f"abm::bits_at({a}, {per}LL, {j}LL, {tr.mode})")
+            lb = tr.tmp("uint32_t", f"abm::bits_at({b}, {per}LL, {j}LL, {tr.mode})")
+            expr = f"(int32_t)((uint32_t)({lo}) + abm::randint_offset({hb}, {lb}, {span}u))"
+            return Sym(tr, tr.tmp("int32_t", expr), "i32", True)
+
+        return self._shaped(shape, one)
 
 
 class SymKeyArray:
@@ -344,11 +366,150 @@ class SymKeyArray:
         raise Unsupported("non-integer index into a key array")
 
 
+_MAX_VEC = 16   # per-agent vectors up to this length are unrolled
+
+
+class SymVec:
+    """A traced per-agent (or uniform) vector of static length k <= 16: its
+    elements as traced scalars of one dtype.  Elementwise operations are
+    unrolled; promotion is decided on the whole vector (a per-agent vector
+    is a 2-D tensor under vmap, a uniform one 1-D)."""
+
+    _abm_sym = True
+    __slots__ = ("tr", "items")
+
+    def __init__(self, tr: "_Trace", items: List[Sym]):
+        if not items or len(items) > _MAX_VEC:
+            raise Unsupported("a per-agent vector longer than 16")
+        self.tr, self.items = tr, list(items)
+
+    weak = False
+
+    @property
+    def kind(self) -> str:
+        return self.items[0].kind
+
+    @property
+    def varying(self) -> bool:
+        return any(x.varying for x in self.items)
+
+    @property
+    def dtype(self):
+        return _TORCH[self.kind]
+
+    @property
+    def shape(self):
+        return torch.Size((len(self.items),))
+
+    def dim(self):
+        return 1
+
+    def size(self, *a):
+        return self.shape if not a else len(self.items)
+
+    def __len__(self):
+        return len(self.items)
+
+    def __iter__(self):
+        return iter(self.items)
+
+    def is_floating_point(self):
+        return self.kind in _FLOATS
+
+    def as_array(self):
+        return self
+
+    def __getitem__(self, i):
+        if isinstance(i, bool) or not isinstance(i, (int, slice)):
+            raise Unsupported("indexing a per-agent vector by other than an int or a slice")
+        if isinstance(i, slice):
+            return SymVec(self.tr, self.items[i])
+        return self.items[i]
+
+    def to(self, *args, **kwargs):
+        return SymVec(self.tr, [x.to(*args, **kwargs) for x in self.items])
+
+    def astype(self, dt):
+        return self.to(dt)
+
+    def float(self):
+        return self.to(torch.float32)
+
+    def int(self):
+        return self.to(torch.int32)
+
+    def long(self):
+        return self.to(torch.int64)
+
+    def bool(self):
+        return self.to(torch.bool)
+
+    def clone(self):
+        return self
+
+    def contiguous(self):
+        return self
+
+    def detach(self):
+        return self
+
+    def __bool__(self):
+        raise Unsupported("Python control flow on a traced value")
+
+    def __float__(self):
+        raise Unsupported("float() of a traced value")
+
+    def item(self):
+        raise Unsupported("item() of a traced value")
+
+    def tolist(self):
+        raise Unsupported("tolist() of a traced value")
+
+    def sum(self, *a, **k):
+        raise Unsupported("a reduction over a per-agent vector")
+
+    mean = sum
+
+    __add__ = Sym.__add__
+    __radd__ = Sym.__radd__
+    __sub__ = Sym.__sub__
+    __rsub__ = Sym.__rsub__
+    __mul__ = Sym.__mul__
+    __rmul__ = Sym.__rmul__
+    __truediv__ = Sym.__truediv__
+    __rtruediv__ = Sym.__rtruediv__
+    __floordiv__ = Sym.__floordiv__
+    __rfloordiv__ = Sym.__rfloordiv__
+    __mod__ = Sym.__mod__
+    __rmod__ = Sym.__rmod__
+    __pow__ = Sym.__pow__
+    __lt__ = Sym.__lt__
+    __le__ = Sym.__le__
+    __gt__ = Sym.__gt__
+    __ge__ = Sym.__ge__
+    __eq__ = Sym.__eq__
+    __ne__ = Sym.__ne__
+    __and__ = Sym.__and__
+    __rand__ = Sym.__rand__
+    __or__ = Sym.__or__
+    __ror__ = Sym.__ror__
+    __xor__ = Sym.__xor__
+    __rxor__ = Sym.__rxor__
+    __neg__ = Sym.__neg__
+    __pos__ = Sym.__pos__
+    __abs__ = Sym.__abs__
+    __invert__ = Sym.__invert__
+    __hash__ = object.__hash__
+    __torch_function__ = Sym.__torch_function__
+
+
 # ---------------------------------------------------------------- the trace
 class _Trace:
-    def __init__(self, n: int, mode: int):
+    def __init__(self, n: int, mode: int, allow_host: bool = False):
         self.n = n
         self.mode = int(mode)
+        # host tensors accepted as env values (tracing for a compile check only)
+        self.allow_host = allow_host
         self.lines: List[str] = []
         self.words: List[Tuple[str, Any]] = []   # ("ptr", tensor) | ("f64", v) | ("i64", v)
         self.count = 0
@@ -367,17 +528,23 @@ class _Trace:
         return len(self.words) - 1
 
     # -- leaves
-    def agent_input(self, t: torch.Tensor) -> Sym:
+    def agent_input(self, t: torch.Tensor):
         kind = _KIND.get(t.dtype)
-        if kind is None or t.dim() != 1 or t.shape[0] != self.n or not t.is_contiguous():
-            raise Unsupported("a state variable that is not one scalar per agent")
+        if (kind is None or t.dim() not in (1, 2) or t.shape[0] != self.n or
+                not t.is_contiguous() or (t.dim() == 2 and not 0 < t.shape[1] <= _MAX_VEC)):
+            raise Unsupported("a state variable that is not a scalar or short vector per agent")
         w = self.word("ptr", t)
-        name = self.tmp(_CT[kind], f"((const {_CT[kind]} *)a.w[{w}])[i]")
-        return Sym(self, name, kind, True)
+        if t.dim() == 1:
+            name = self.tmp(_CT[kind], f"((const {_CT[kind]} *)a.w[{w}])[i]")
+            return Sym(self, name, kind, True)
+        k = t.shape[1]
+        return SymVec(self, [Sym(self, self.tmp(_CT[kind], f"((const {_CT[kind]} *)a.w[{w}])"
+                                                        f"[i * {k}LL + {j}]"), kind, True)
+                             for j in range(k)])
 
     def uniform_tensor(self, t: torch.Tensor) -> Sym:
         kind = _KIND.get(t.dtype)
-        if kind is None or t.dim() != 0 or not t.is_cuda:
+        if kind is None or t.dim() != 0 or not (t.is_cuda or self.allow_host):
             raise Unsupported("an env tensor that is not a 0-d device scalar")
         w = self.word("ptr", t)
         return Sym(self, self.tmp(_CT[kind], f"*((const {_CT[kind]} *)a.w[{w}])"), kind, False)
@@ -417,20 +584,96 @@ class _Trace:
                 raise Unsupported("numpy scalar type")
             s = self.const(x.item())
             return self.cast(s, kind) if not s.weak or kind not in ("f64", "i64") else s
-        if isinstance(x, torch.Tensor) and x.dim() == 0 and x.is_cuda:
+        if isinstance(x, torch.Tensor) and x.dim() == 0 and (x.is_cuda or self.allow_host):
             return self.uniform_tensor(x)
         raise Unsupported(f"operand of type {type(x).__name__}")
 
-    def bound(self, v) -> Sym:
-        """random.uniform's tensor-bound branch: bounds as float32 0-d values."""
-        if isinstance(v, Sym):
+    # -- vectors
+    def voperand(self, x):
+        """A traced scalar or vector operand (vectors: SymVec or a small 1-D
+        device tensor, read as a uniform vector)."""
+        if isinstance(x, SymVec):
+            if x.tr is not self:
+                raise Unsupported("values of two traces")
+            return x
+        if (isinstance(x, torch.Tensor) and x.dim() == 1 and (x.is_cuda or self.allow_host) and
+                0 < x.shape[0] <= _MAX_VEC):
+            return self.uniform_vector(x)
+        return self.operand(x)
+
+    def uniform_vector(self, t: torch.Tensor) -> SymVec:
+        kind = _KIND.get(t.dtype)
+        if kind is None or not t.is_contiguous():
+            raise Unsupported("an env vector of an unsupported dtype")
+        w = self.word("ptr", t)
+        return SymVec(self, [Sym(self, self.tmp(_CT[kind], f"((const {_CT[kind]} *)a.w[{w}])[{j}]"),
+                                 kind, False) for j in range(t.shape[0])])
+
+    @staticmethod
+    def _vstand_in(x):
+        if isinstance(x, SymVec):
+            return torch.empty((1, len(x)) if x.varying else (len(x),), dtype=_TORCH[x.kind])
+        return _Trace._stand_in(x)
+
+    def _vlen(self, *xs) -> int:
+        k = 1
+        for x in xs:
+            if isinstance(x, SymVec):
+                if k not in (1, len(x)):
+                    raise Unsupported("vectors of different lengths")
+                k = len(x)
+        return k
+
+    @staticmethod
+    def _elem(x, j):
+        return x.items[j] if isinstance(x, SymVec) else x
+
+    def _vkind(self, a, b) -> str:
+        if not isinstance(a, SymVec) and not isinstance(b, SymVec):
+            return self.promote(a, b)
+        return _KIND[torch.result_type(self._vstand_in(a), self._vstand_in(b))]
+
+    def _vcast(self, x, kind):
+        """Cast a non-weak operand to the vector's promoted type (a Python
+        scalar stays one: torch converts it inside the operation)."""
+        return x if x.weak or kind in _WEAK else self.cast(x, kind)
+
+    def vmap2(self, fn, a, b):
+        a, b = self.voperand(a), self.voperand(b)
+        if not isinstance(a, SymVec) and not isinstance(b, SymVec):
+            return fn(a, b)
+        kind = self._vkind(a, b)
+        k = self._vlen(a, b)
+        return SymVec(self, [fn(self._vcast(self._elem(a, j), kind),
+                                self._vcast(self._elem(b, j), kind)) for j in range(k)])
+
+    def stack(self, values) -> SymVec:
+        """jnp.array / jnp.stack of per-agent or uniform scalars (Python
+        scalars become 0-d arrays of JAX's default dtype first)."""
+        xs = [self.operand(v).as_array() for v in values]
+        if not xs:
+            raise Unsupported("an empty vector")
+        kind = xs[0].kind
+        for x in xs[1:]:
+            kind = self._stack_kind(kind, x)
+        return SymVec(self, [self.cast(x, kind) for x in xs])
+
+    def _stack_kind(self, kind: str, x: Sym) -> str:
+        return _KIND[torch.result_type(torch.empty((1,), dtype=_TORCH[kind]),
+                                       torch.empty((1,), dtype=_TORCH[x.kind]))]
+
+    def bound(self, v):
+        """random.uniform's tensor-bound branch: bounds as float32 values."""
+        if isinstance(v, (Sym, SymVec)):
             return self.cast(v, "f32")
         if isinstance(v, torch.Tensor):
-            return self.cast(self.operand(v), "f32")
+            return self.cast(self.voperand(v), "f32")
         return self.cast(self.const(float(v)), "f32")
 
     # -- casts and promotion
-    def cast(self, x: Sym, kind: str) -> Sym:
+    def cast(self, x, kind: str):
+        if isinstance(x, SymVec):
+            return SymVec(self, [self.cast(v, kind) for v in x.items])
         x = self.operand(x)
         if x.kind == kind:
             return x
@@ -454,7 +697,11 @@ class _Trace:
         return _KIND[torch.result_type(self._stand_in(a), self._stand_in(b))]
 
     # -- element operations
-    def binary(self, op: str, a, b) -> Sym:
+    def binary(self, op: str, a, b):
+        if isinstance(a, (SymVec, torch.Tensor)) or isinstance(b, (SymVec, torch.Tensor)):
+            va, vb = self.voperand(a), self.voperand(b)
+            if isinstance(va, SymVec) or isinstance(vb, SymVec):
+                return self.vmap2(lambda x, y: self.binary(op, x, y), va, vb)
         a, b = self.operand(a), self.operand(b)
         varying = a.varying or b.varying
         kind = self.promote(a, b)
@@ -519,9 +766,11 @@ class _Trace:
             return x.name
         return f"(({_CT[kind]})({x.name}))"
 
-    def power(self, x, e) -> Sym:
+    def power(self, x, e):
         if not isinstance(e, int) or isinstance(e, bool) or e not in (2, 3):
             raise Unsupported("power other than x**2 or x**3")
+        if isinstance(x, SymVec):
+            return SymVec(self, [self.power(v, e) for v in x.items])
         x = self.operand(x)
         kind = self.promote(x, self.const(e))
         cx = self._as(x, kind)
@@ -532,8 +781,10 @@ class _Trace:
                 "sqrt": "sqrtf", "sin": "sinf", "cos": "cosf", "tan": "tanf", "tanh": "tanhf",
                 "atan": "atanf"}
 
-    def unary(self, op: str, x) -> Sym:
-        x = self.operand(x)
+    def unary(self, op: str, x):
+        x = self.voperand(x)
+        if isinstance(x, SymVec):
+            return SymVec(self, [self.unary(op, v) for v in x.items])
         if x.weak:
             if op == "neg":
                 if x.kind == "pyb":
@@ -590,9 +841,16 @@ class _Trace:
             return Sym(self, self.tmp("bool", f"(!(bool)({x.name}))"), "bool", x.varying)
         raise Unsupported(op)
 
-    def where(self, c, a, b) -> Sym:
-        c = self.operand(c)
-        a, b = self.operand(a), self.operand(b)
+    def where(self, c, a, b):
+        c, a, b = self.voperand(c), self.voperand(a), self.voperand(b)
+        if any(isinstance(v, SymVec) for v in (c, a, b)):
+            kind = self._vkind(a, b)
+            if kind in _WEAK:
+                kind = _DEFAULT[kind]
+            k = self._vlen(c, a, b)
+            return SymVec(self, [self.where(self._elem(c, j), self._vcast(self._elem(a, j), kind),
+                                            self._vcast(self._elem(b, j), kind))
+                                 for j in range(k)])
         kind = self.promote(a, b)
         if kind in _WEAK:
             kind = _DEFAULT[kind]
@@ -600,8 +858,10 @@ class _Trace:
         expr = f"({cb} ? {self._as(a, kind)} : {self._as(b, kind)})"
         return Sym(self, self.tmp(_CT[kind], expr), kind, c.varying or a.varying or b.varying)
 
-    def minmax(self, op: str, a, b) -> Sym:
-        a, b = self.operand(a), self.operand(b)
+    def minmax(self, op: str, a, b):
+        a, b = self.voperand(a), self.voperand(b)
+        if isinstance(a, SymVec) or isinstance(b, SymVec):
+            return self.vmap2(lambda x, y: self.minmax(op, x, y), a, b)
         if a.weak and b.weak:
             a = a.as_array()
         kind = self.promote(a, b)
@@ -610,8 +870,13 @@ class _Trace:
         return Sym(self, self.tmp(_CT[kind], f"{fn}<{_CT[kind]}>({ca}, {cb})"), kind,
                    a.varying or b.varying)
 
-    def clamp(self, x, lo, hi) -> Sym:
-        x = self.operand(x)
+    def clamp(self, x, lo, hi):
+        x = self.voperand(x)
+        if isinstance(x, SymVec):
+            for v in (lo, hi):
+                if isinstance(v, (SymVec, torch.Tensor)) and getattr(v, "dim", lambda: 0)() > 0:
+                    raise Unsupported("clamp with vector bounds")
+            return SymVec(self, [self.clamp(v, lo, hi) for v in x.items])
         if x.weak:
             x = x.as_array()
         kind = x.kind
@@ -679,6 +944,9 @@ class _Trace:
             return self.clamp(x, lo, hi)
         if name == "pow" and len(args) == 2 and not kwargs:
             return self.power(args[0], args[1])
+        if name == "stack" and len(args) >= 1 and isinstance(args[0], (list, tuple)) and \
+                kwargs.get("dim", args[1] if len(args) > 1 else 0) in (0, -1):
+            return self.stack(args[0])
         raise Unsupported(f"torch.{name}")
 
 
@@ -814,6 +1082,14 @@ class _Kernel:
                                    stream), "rtc_launch")
 
 
+_active: List["_Trace"] = []   # traces in progress (jnp constants become literals)
+
+
+def active_trace() -> Optional["_Trace"]:
+    """The trace in progress on this thread, if any."""
+    return _active[-1] if _active else None
+
+
 _kernels: Dict[str, _Kernel] = {}
 _refused: Dict[Any, str] = {}
 _launches = 0
@@ -849,8 +1125,12 @@ def _wrap_model_state(tr: _Trace, model_state):
     def wrap(v):
         if isinstance(v, (bool, int, float)):
             return tr.python_param(v)
-        if isinstance(v, torch.Tensor) and v.dim() == 0 and v.is_cuda and v.dtype in _KIND:
+        on_device = isinstance(v, torch.Tensor) and (v.is_cuda or tr.allow_host)
+        if on_device and v.dim() == 0 and v.dtype in _KIND:
             return tr.uniform_tensor(v)
+        if (on_device and v.dim() == 1 and v.dtype in _KIND and 0 < v.shape[0] <= _MAX_VEC and
+                v.is_contiguous()):
+            return tr.uniform_vector(v)
         return v
 
     out = {}
@@ -887,7 +1167,7 @@ def trace(n: int, key: torch.Tensor, mode: int, states: Optional[Dict[str, Any]]
     """Trace ``call(states_sym, model_state_sym, agent_key_sym)`` for n
     agents; returns (kernel name, source, trace, output tensors).  Raises
     Unsupported."""
-    tr = _Trace(n, mode)
+    tr = _Trace(n, mode, allow_host=device is not None and torch.device(device).type == "cpu")
     tr.word("i64", n)
     kw = tr.word("ptr", key.contiguous() if key.dtype == torch.int32 else
                  key.to(torch.int32).contiguous())
@@ -896,11 +1176,24 @@ def trace(n: int, key: torch.Tensor, mode: int, states: Optional[Dict[str, Any]]
                     f"{tr.mode});")
     sym_states = ({k: tr.agent_input(v) for k, v in states.items()}
                   if states is not None else None)
-    out = call(sym_states, _wrap_model_state(tr, model_state), SymKey(tr, "ak"))
+    _active.append(tr)
+    try:
+        out = call(sym_states, _wrap_model_state(tr, model_state), SymKey(tr, "ak"))
+    finally:
+        _active.pop()
     if not isinstance(out, dict):
         raise Unsupported("the function did not return a dict")
     stores, outs = [], {}
     for k, v in out.items():
+        if isinstance(v, torch.Tensor) and v.dim() <= 1:
+            v = tr.voperand(v)   # a constant array: the same value for every agent
+        if isinstance(v, SymVec):
+            t = torch.empty((n, len(v)), dtype=_TORCH[v.kind], device=device or key.device)
+            w = tr.word("ptr", t)
+            for j, e in enumerate(v.items):
+                stores.append(f"(({_CT[v.kind]} *)a.w[{w}])[i * {len(v)}LL + {j}] = {e.name};")
+            outs[k] = t
+            continue
         if isinstance(v, Sym):
             s = v if not v.weak else v.as_array()
         elif isinstance(v, (bool, int, float)):

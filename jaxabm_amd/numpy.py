@@ -54,6 +54,11 @@ def asarray(a, dtype=None) -> torch.Tensor:
     """Array from a tensor, numpy array, Python scalar or (nested) sequence."""
     if getattr(a, "_abm_sym", False) is True:
         return a.as_array() if dtype is None else a.as_array().to(dtype)
+    tr = _trace()
+    if tr is not None and _python_constant(a):
+        # a constant inside a fused agent function: literals in the kernel
+        out = tr.const(a).as_array() if isinstance(a, (bool, int, float)) else tr.stack(list(a))
+        return out if dtype is None else out.to(dtype)
     if isinstance(a, torch.Tensor):
         return a if dtype is None else a.to(dtype)
     if isinstance(a, _np.ndarray):
@@ -63,7 +68,11 @@ def asarray(a, dtype=None) -> torch.Tensor:
         if dtype is None and t.dtype == torch.int64:
             dtype = torch.int32
         return t.to(device=_dev(), dtype=dtype or t.dtype)
-    if isinstance(a, (list, tuple)) and a and any(isinstance(x, torch.Tensor) for x in a):
+    if isinstance(a, (list, tuple)) and a and builtins.any(_traced(x) for x in a):
+        tr = next(x for x in a if _traced(x)).tr
+        out = tr.stack(a)      # a per-agent vector inside a fused agent function
+        return out if dtype is None else out.to(dtype)
+    if isinstance(a, (list, tuple)) and a and builtins.any(isinstance(x, torch.Tensor) for x in a):
         return torch.stack([asarray(x, dtype) for x in a])
     if isinstance(a, (bool, int, float)):
         # a fill kernel, not a host copy (no host synchronisation)
@@ -78,6 +87,19 @@ def asarray(a, dtype=None) -> torch.Tensor:
 
 
 array = asarray
+
+
+def _trace():
+    from .fuse import active_trace
+
+    return active_trace()
+
+
+def _python_constant(a) -> bool:
+    if isinstance(a, (bool, int, float)):
+        return True
+    return (isinstance(a, (list, tuple)) and 0 < len(a) <= 16 and
+            builtins.all(isinstance(x, (bool, int, float)) for x in a))
 
 
 def _traced(x) -> bool:

@@ -246,8 +246,8 @@ def _traced_scalar_shape(key, shape, mode):
 
     if not isinstance(key, SymKey):
         raise Unsupported("a draw from a key array")
-    if _shape(shape) != ():
-        raise Unsupported("a non-scalar draw inside a per-agent function")
+    if len(_shape(shape)) > 1:
+        raise Unsupported("a draw of more than one dimension inside a per-agent function")
     if _mode(mode) != key.tr.mode:
         raise Unsupported("a draw in another stream mode")
 
@@ -264,7 +264,7 @@ def bits(key, shape=(), mode=None) -> torch.Tensor:
     """``jax.random.bits(key, shape, uint32)`` (returned as int32 storage)."""
     if _traced_key(key):
         _traced_scalar_shape(key, shape, mode)
-        return key.bits()
+        return key.bits(_shape(shape))
     return torch.ops.jaxabm.draw(_key_arg(key), list(_shape(shape)), 0, 0.0, 0.0, _mode(mode))
 
 
@@ -275,7 +275,7 @@ def uniform(key, shape=(), dtype=torch.float32, minval=0.0, maxval=1.0, mode=Non
         raise NotImplementedError("uniform: float32 only (the reference runs with jax_enable_x64 off)")
     if _traced_key(key):
         _traced_scalar_shape(key, shape, mode)
-        return key.uniform(minval, maxval)
+        return key.uniform(minval, maxval, _shape(shape))
     k = _key_arg(key)
     if isinstance(minval, torch.Tensor) or isinstance(maxval, torch.Tensor):
         f = torch.ops.jaxabm.draw(k, list(_shape(shape)), 1, 0.0, 1.0, _mode(mode))
@@ -294,7 +294,7 @@ def normal(key, shape=(), dtype=torch.float32, mode=None) -> torch.Tensor:
     """``jax.random.normal(key, shape, float32)``."""
     if _traced_key(key):
         _traced_scalar_shape(key, shape, mode)
-        return key.normal()
+        return key.normal(_shape(shape))
     return torch.ops.jaxabm.draw(_key_arg(key), list(_shape(shape)), 2, 0.0, 0.0, _mode(mode))
 
 
@@ -312,7 +312,7 @@ def randint(key, shape, minval, maxval, dtype=torch.int32, mode=None) -> torch.T
     hi = int(np.clip(int(maxval), -2**31, 2**31 - 1))
     if _traced_key(key):
         _traced_scalar_shape(key, shape, mode)
-        return key.randint(lo, hi)
+        return key.randint(lo, hi, _shape(shape))
     return torch.ops.jaxabm.randint(_key_arg(key), list(_shape(shape)), lo, hi, _mode(mode))
 
 

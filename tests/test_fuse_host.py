@@ -82,7 +82,7 @@ def test_same_function_same_source_for_other_env_values():
     (lambda s, m, k: {"x": s["x"] if s["x"] > 0.5 else -s["x"]}, "control flow"),
     (lambda s, m, k: {"x": jnp.sum(s["x"])}, "sum"),
     (lambda s, m, k: {"x": s["x"] + m["agents_a"]["x"]}, "operand"),
-    (lambda s, m, k: {"x": jr.uniform(k, (3,))}, "non-scalar draw"),
+    (lambda s, m, k: {"x": jr.uniform(k, (3, 2))}, "two-dimensional draw"),
     (lambda s, m, k: {"x": float(s["x"])}, "float()"),
     (lambda s, m, k: {"x": s["x"] ** 0.5}, "power"),
     (lambda s, m, k: [s["x"]], "dict"),
@@ -93,13 +93,42 @@ def test_refused(fn, why):
         _trace(fn, _states(), ms)
 
 
-def test_vector_state_refused():
-    with pytest.raises(fuse.Unsupported):
-        _trace(lambda s, m, k: {"p": s["p"] + 1.0}, {"p": torch.rand(N, 2)}, {"env": {}})
+def test_vector_limits_refused():
+    with pytest.raises(fuse.Unsupported):      # longer than 16
+        _trace(lambda s, m, k: {"p": s["p"] + 1.0}, {"p": torch.rand(N, 17)}, {"env": {}})
+    with pytest.raises(fuse.Unsupported):      # a reduction over the vector
+        _trace(lambda s, m, k: {"r": jnp.sum(s["p"])}, {"p": torch.rand(N, 2)}, {"env": {}})
+    with pytest.raises(fuse.Unsupported):      # vectors of different lengths
+        _trace(lambda s, m, k: {"r": s["p"] + s["q"]},
+               {"p": torch.rand(N, 2), "q": torch.rand(N, 3)}, {"env": {}})
 
 
-def test_literals_keep_float64_bits():
-    assert fuse._lit(0.1, "f32") == "((float)0x1.999999999999ap-4)"
-    assert fuse._lit(-2, "i32") == "((int32_t)(-2LL))"
-    assert fuse._lit(True, "bool") == "((bool)true)"
-    assert np.float32(float.fromhex("0x1.999999999999ap-4")) == np.float32(0.1)
+def test_walker_with_vectors_traces_and_compiles():
+    """A bouncing walker in the style of the reference's basic example
+    (examples/basic_example.py:20-70): 2-vectors of position and velocity,
+    element reads, an env bounds vector, jnp.array of per-agent values and a
+    vector draw."""
+    st = {"position": torch.rand(N, 2), "velocity": torch.rand(N, 2) * 0.02,
+          "color": torch.randint(0, 2, (N,), dtype=torch.int32),
+          "steps_taken": torch.zeros(N, dtype=torch.int32)}
+
+    def step(s, m, k):
+        position, velocity = s["position"], s["velocity"]
+        bounds = m["env"].get("bounds")
+        new_position = position + velocity
+        x_bounce = (new_position[0] <= bounds[0]) | (new_position[0] >= bounds[1])
+        y_bounce = (new_position[1] <= bounds[0]) | (new_position[1] >= bounds[1])
+        new_velocity = velocity * jnp.array([1 - 2 * x_bounce, 1 - 2 * y_bounce])
+        new_position = jnp.clip(new_position, bounds[0], bounds[1])
+        any_bounce = jnp.logical_or(x_bounce, y_bounce)
+        return {"position": new_position + jr.normal(k, (2,)) * 0.001,
+                "velocity": new_velocity,
+                "color": jnp.where(any_bounce, 1 - s["color"], s["color"]),
+                "steps_taken": s["steps_taken"] + 1}
+
+    name, src, tr, outs = _trace(step, st, {"env": {"bounds": torch.tensor([0.0, 1.0])},
+                                            "time_step": 0})
+    assert {k: (v.dtype, tuple(v.shape)) for k, v in outs.items()} == {
+        "position": (torch.float32, (N, 2)), "velocity": (torch.float32, (N, 2)),
+        "color": (torch.int32, (N,)), "steps_taken": (torch.int32, (N,))}
+    fuse.compile_source(src)

@@ -75,6 +75,41 @@ class Draws:
                 "u": jr.uniform(k2, minval=0.5, maxval=3.25)}
 
 
+class Walker:
+    """A bouncing walker in the style of the reference's basic example
+    (examples/basic_example.py:20-70): per-agent 2-vectors, element reads, an
+    env bounds vector, jnp.array of per-agent values, vector draws."""
+
+    def init_state(self, cfg, key):
+        from jaxabm_amd import numpy as jnp
+        from jaxabm_amd import random as jr
+
+        k1, k2 = jr.split(key)
+        return {"position": jr.uniform(k1, (2,)), "velocity": jr.normal(k2, (2,)) * 0.05,
+                "color": 0, "steps_taken": 0, "home": jnp.array([0.5, 0.25])}
+
+    def update(self, s, ms, cfg, key):
+        from jaxabm_amd import numpy as jnp
+        from jaxabm_amd import random as jr
+
+        position, velocity = s["position"], s["velocity"]
+        bounds = ms["env"]["bounds"]
+        new_position = position + velocity
+        x_bounce = (new_position[0] <= bounds[0]) | (new_position[0] >= bounds[1])
+        y_bounce = (new_position[1] <= bounds[0]) | (new_position[1] >= bounds[1])
+        new_velocity = velocity * jnp.array([1 - 2 * x_bounce, 1 - 2 * y_bounce])
+        new_position = jnp.clip(new_position, bounds[0], bounds[1])
+        k1, k2, k3 = jr.split(key, 3)
+        jitter = jr.uniform(k1, (2,), minval=-0.01, maxval=0.01)
+        lo = jnp.minimum(new_position, 0.5)
+        return {"position": new_position + jitter * (s["home"] - new_position) ** 2,
+                "velocity": jnp.where(x_bounce | y_bounce, -new_velocity, new_velocity) / 1.5,
+                "color": jnp.where(jnp.logical_or(x_bounce, y_bounce), 1 - s["color"], s["color"]),
+                "steps_taken": s["steps_taken"] + 1, "home": s["home"],
+                "r": jr.randint(k2, (3,), -4, 9), "u": jr.uniform(k3, (2,), minval=lo,
+                                                                  maxval=lo + 1.0)}
+
+
 def run(J, agent, n, steps, mode, fused, env):
     import torch
 
@@ -114,14 +149,16 @@ def bits(t):
 
 
 @pytest.mark.parametrize("mode", [O.MODE_ORIGINAL, O.MODE_PARTITIONABLE])
-@pytest.mark.parametrize("agent,n", [(Arith, 1000), (Arith, 257), (Draws, 1000), (Draws, 3)])
+@pytest.mark.parametrize("agent,n", [(Arith, 1000), (Arith, 257), (Draws, 1000), (Draws, 3),
+                                     (Walker, 5000), (Walker, 1)])
 def test_fused_equals_vmapped(J, agent, n, mode):
     import torch
 
     from jaxabm_amd import fuse
 
     env = {"shift": 0.25, "scale": 1.5, "k": 3, "thr": 0.1, "on": True,
-           "delta": torch.tensor(0.125, device="cuda")}
+           "delta": torch.tensor(0.125, device="cuda"),
+           "bounds": torch.tensor([0.0, 1.0], device="cuda")}
     before = fuse.stats()["launches"]
     i_ref, ref = run(J, agent(), n, 4, mode, False, env)
     mid = fuse.stats()["launches"]
@@ -140,8 +177,8 @@ def test_fused_equals_vmapped(J, agent, n, mode):
 
 
 def test_refused_function_runs_vmapped(J):
-    """A per-agent vector variable is not covered: the collection runs the
-    vmapped path (and remembers it)."""
+    """A reduction over a per-agent vector is not covered: the update runs
+    the vmapped path (and the refusal is remembered); the init is fused."""
     from jaxabm_amd import fuse
     from jaxabm_amd import random as jr
 
@@ -150,15 +187,21 @@ def test_refused_function_runs_vmapped(J):
             return {"p": jr.uniform(key, (2,))}
 
         def update(self, s, ms, cfg, key):
-            return {"p": s["p"] * 0.5}
+            from jaxabm_amd import numpy as jnp
+
+            return {"p": s["p"] * jnp.sum(s["p"])}
 
     before = fuse.stats()["launches"]
     c = J.AgentCollection(Vec(), 50)
     c.init(jr.PRNGKey(1), J.ModelConfig())
+    p0 = c.states["p"].clone()
     c.update({"env": {}}, jr.PRNGKey(2), J.ModelConfig())
     assert c.states["p"].shape == (50, 2)
-    assert fuse.stats()["launches"] == before
-    assert any("non-scalar" in r or "per agent" in r for r in fuse.refused().values())
+    assert fuse.stats()["launches"] == before + 1
+    assert any("sum" in r for r in fuse.refused().values())
+    import torch
+
+    assert torch.equal(c.states["p"], p0 * p0.sum(1, keepdim=True))
 
 
 OPS = {
@@ -257,3 +300,59 @@ def test_single_operation_fused_equals_vmapped(J, wide_inputs, name):
     a, g = res
     assert a.dtype == g.dtype
     np.testing.assert_array_equal(bits(a), bits(g))
+
+
+def test_facade_walker_model_fused_equals_vmapped(J):
+    """The AgentPy-style model of the reference's basic example
+    (examples/basic_example.py:20-140): Agent.setup/step with 2-vectors and an
+    env bounds vector, run through the facade with the fused path on and
+    off; same positions, colours and records."""
+    import torch
+
+    from jaxabm_amd import fuse
+    from jaxabm_amd import numpy as jnp
+
+    class RandomWalker(J.Agent):
+        def setup(self):
+            return {"position": jnp.array([0.5, 0.5]), "velocity": jnp.array([0.013, 0.007]),
+                    "color": 0, "steps_taken": 0}
+
+        def step(self, model_state):
+            position, velocity = self._state["position"], self._state["velocity"]
+            bounds = model_state.get("env", {}).get("bounds", jnp.array([0.0, 1.0]))
+            new_position = position + velocity
+            x_bounce = (new_position[0] <= bounds[0]) | (new_position[0] >= bounds[1])
+            y_bounce = (new_position[1] <= bounds[0]) | (new_position[1] >= bounds[1])
+            new_velocity = velocity * jnp.array([1 - 2 * x_bounce, 1 - 2 * y_bounce])
+            new_position = jnp.clip(new_position, bounds[0], bounds[1])
+            any_bounce = jnp.logical_or(x_bounce, y_bounce)
+            return {"position": new_position, "velocity": new_velocity,
+                    "color": jnp.where(any_bounce, 1 - self._state["color"], self._state["color"]),
+                    "steps_taken": self._state["steps_taken"] + 1}
+
+    class RandomWalkModel(J.Model):
+        def setup(self):
+            self.walkers = self.add_agents(self.p.get("n_agents", 50), RandomWalker)
+            self.env.add_state("bounds", jnp.array([0.0, 1.0]))
+
+        def step(self):
+            self.record("mean_x", float(jnp.mean(self.walkers.position[:, 0])))
+
+    out = []
+    for fused in (False, True):
+        fuse.enable(fused)
+        try:
+            before = fuse.stats()["launches"]
+            m = RandomWalkModel({"steps": 120, "n_agents": 64, "seed": 3})
+            res = m.run()
+            launched = fuse.stats()["launches"] - before
+            st = m.walkers.states
+            out.append((res["mean_x"], {k: v.clone() for k, v in st.items()}, launched))
+        finally:
+            fuse.enable(True)
+    (ra, sa, la), (rb, sb, lb) = out
+    assert la == 0 and lb == 121, (la, lb, fuse.refused())
+    assert ra == rb
+    for k in sa:
+        assert torch.equal(sa[k], sb[k]), k
+    assert int(sb["steps_taken"][0]) == 120 and bool((sb["color"] == 1).any())
