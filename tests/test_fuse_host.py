@@ -96,8 +96,14 @@ def test_refused(fn, why):
 def test_vector_limits_refused():
     with pytest.raises(fuse.Unsupported):      # longer than 16
         _trace(lambda s, m, k: {"p": s["p"] + 1.0}, {"p": torch.rand(N, 17)}, {"env": {}})
-    with pytest.raises(fuse.Unsupported):      # a reduction over the vector
-        _trace(lambda s, m, k: {"r": jnp.sum(s["p"])}, {"p": torch.rand(N, 2)}, {"env": {}})
+    with pytest.raises(fuse.Unsupported):      # a reduction over 8 or more values
+        _trace(lambda s, m, k: {"r": jnp.sum(s["p"])}, {"p": torch.rand(N, 8)}, {"env": {}})
+    name, src, tr, outs = _trace(lambda s, m, k: {"r": jnp.sum(s["p"] * s["p"]),
+                                                  "m": jnp.mean(s["p"]),
+                                                  "c": jnp.sum(s["p"] > 0.5)},
+                                 {"p": torch.rand(N, 3)}, {"env": {}})
+    assert {k: v.dtype for k, v in outs.items()} == {"r": torch.float32, "m": torch.float32,
+                                                     "c": torch.int64}
     with pytest.raises(fuse.Unsupported):      # vectors of different lengths
         _trace(lambda s, m, k: {"r": s["p"] + s["q"]},
                {"p": torch.rand(N, 2), "q": torch.rand(N, 3)}, {"env": {}})

@@ -177,8 +177,8 @@ def test_fused_equals_vmapped(J, agent, n, mode):
 
 
 def test_refused_function_runs_vmapped(J):
-    """A reduction over a per-agent vector is not covered: the update runs
-    the vmapped path (and the refusal is remembered); the init is fused."""
+    """A product over a per-agent vector is not covered: the update runs the
+    vmapped path (and the refusal is remembered); the init is fused."""
     from jaxabm_amd import fuse
     from jaxabm_amd import random as jr
 
@@ -189,7 +189,7 @@ def test_refused_function_runs_vmapped(J):
         def update(self, s, ms, cfg, key):
             from jaxabm_amd import numpy as jnp
 
-            return {"p": s["p"] * jnp.sum(s["p"])}
+            return {"p": s["p"] * jnp.prod(s["p"])}
 
     before = fuse.stats()["launches"]
     c = J.AgentCollection(Vec(), 50)
@@ -198,10 +198,10 @@ def test_refused_function_runs_vmapped(J):
     c.update({"env": {}}, jr.PRNGKey(2), J.ModelConfig())
     assert c.states["p"].shape == (50, 2)
     assert fuse.stats()["launches"] == before + 1
-    assert any("sum" in r for r in fuse.refused().values())
+    assert any("prod" in r for r in fuse.refused().values())
     import torch
 
-    assert torch.equal(c.states["p"], p0 * p0.sum(1, keepdim=True))
+    assert torch.equal(c.states["p"], p0 * p0.prod(1, keepdim=True))
 
 
 OPS = {
@@ -356,3 +356,49 @@ def test_facade_walker_model_fused_equals_vmapped(J):
     for k in sa:
         assert torch.equal(sa[k], sb[k]), k
     assert int(sb["steps_taken"][0]) == 120 and bool((sb["color"] == 1).any())
+
+
+@pytest.mark.parametrize("k", [2, 3, 4, 5, 6, 7])
+def test_vector_reductions_fused_equal_vmapped(J, k):
+    """jnp.sum / jnp.mean over a per-agent vector of k float32 values (the
+    order torch adds them in) and integer counts, on values over +-2^10
+    with signed zeros."""
+    import torch
+
+    from jaxabm_amd import fuse
+    from jaxabm_amd import numpy as jnp
+    from jaxabm_amd import random as jr
+
+    g = torch.Generator(device="cuda").manual_seed(k)
+    n = 50_000
+    p = torch.randn(n, k, device="cuda", generator=g) * torch.exp2(
+        torch.randint(-10, 11, (n, k), device="cuda", generator=g).float())
+    p[:4, 0] = torch.tensor([-0.0, 0.0, -0.0, 1.0])
+    p[:4, 1:] = -0.0
+
+    class Red:
+        def init_state(self, cfg, key):
+            return {"q": 0.0}
+
+        def update(self, s, ms, cfg, key):
+            v = s["p"]
+            return {"s": jnp.sum(v), "s2": jnp.sum(v * v), "m": jnp.mean(v),
+                    "c": jnp.sum(v > 0.0), "d": v.sum() - jnp.mean(v * 2.0)}
+
+    res = []
+    for fused in (False, True):
+        fuse.enable(fused)
+        try:
+            before = fuse.stats()["launches"]
+            c = J.AgentCollection(Red(), n)
+            c.model_config = J.ModelConfig()
+            c._states = {"p": p.clone()}
+            c.update({"env": {}}, jr.PRNGKey(1), J.ModelConfig())
+            assert fuse.stats()["launches"] - before == (1 if fused else 0), fuse.refused()
+            res.append({kk: vv.clone() for kk, vv in c.states.items()})
+        finally:
+            fuse.enable(True)
+    a, b = res
+    for kk in a:
+        assert a[kk].dtype == b[kk].dtype, kk
+        np.testing.assert_array_equal(bits(a[kk]), bits(b[kk]), err_msg=kk)
