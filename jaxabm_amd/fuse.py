@@ -652,14 +652,15 @@ class _Trace:
         return SymVec(self, [fn(self._vcast(self._elem(a, j), kind),
                                 self._vcast(self._elem(b, j), kind)) for j in range(k)])
 
-    def vsum(self, x: SymVec, mean: bool = False) -> Sym:
+    def vsum(self, x: SymVec, mean: bool = False, out_kind: Optional[str] = None) -> Sym:
         """torch.sum / torch.mean over a per-agent vector, in the order
         torch's reduction adds k <= 7 contiguous float32 values (measured on
         this build for every row count, also under vmap): each accumulator
         starts from zero; k = 2: (0 + x0) + x1; k = 3: ((0 + x0) + x2) +
         (0 + x1); k = 4..7: four accumulators by j mod 4, then
         (a0 + a1) + (a2 + a3).  A mean multiplies the sum by float32(1/k).
-        Integer and boolean sums are int64 (exact, so any order)."""
+        Integer and boolean sums are int64, or int32 (wrapping) when asked
+        for: exact, so any order."""
         k, kind = len(x), x.kind
         if kind in ("bool", "u8", "i32", "i64"):
             if mean:
@@ -668,6 +669,9 @@ class _Trace:
             for v in x.items[1:]:
                 acc = Sym(self, self.tmp("int64_t", f"({acc.name} + (int64_t)({v.name}))"), "i64",
                           x.varying)
+            if out_kind == "i32":
+                acc = Sym(self, self.tmp("int32_t", f"(int32_t)(uint32_t)(uint64_t){acc.name}"),
+                          "i32", x.varying)
             return acc
         if kind != "f32" or not 2 <= k <= 7:
             raise Unsupported("a reduction over a vector other than 2..7 float32 values")
@@ -989,9 +993,12 @@ class _Trace:
             return self.power(args[0], args[1])
         if name in ("sum", "mean") and len(args) == 1 and isinstance(args[0], SymVec):
             dim = kwargs.get("dim")
+            dt = kwargs.get("dtype")
+            int_in = args[0].kind in ("bool", "u8", "i32")
             if dim in (None, 0, -1, (0,), (-1,), [0], [-1]) and not kwargs.get("keepdim") and \
-                    kwargs.get("dtype") is None:
-                return self.vsum(args[0], mean=name == "mean")
+                    (dt is None or (dt == torch.int32 and int_in and name == "sum")):
+                return self.vsum(args[0], mean=name == "mean",
+                                 out_kind="i32" if dt == torch.int32 else None)
         if name == "stack" and len(args) >= 1 and isinstance(args[0], (list, tuple)) and \
                 kwargs.get("dim", args[1] if len(args) > 1 else 0) in (0, -1):
             return self.stack(args[0])

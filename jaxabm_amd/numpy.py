@@ -249,15 +249,26 @@ def _reduce(fn, x, axis, keepdims, **kw):
     return fn(x, dim=axis, keepdim=keepdims, **kw)
 
 
+_SMALL_INTS = (torch.bool, torch.uint8, torch.int8, torch.int16, torch.int32)
+
+
+def _int_acc(x):
+    """JAX with x64 off (the reference's setting) sums and multiplies
+    booleans and integers into int32 (wrapping), where torch would give
+    int64."""
+    return {"dtype": torch.int32} if x.dtype in _SMALL_INTS else {}
+
+
 def sum(x, axis=None, keepdims=False):
-    return _reduce(torch.sum, x, axis, keepdims)
+    x = _t(x)
+    return _reduce(torch.sum, x, axis, keepdims, **_int_acc(x))
 
 
 def prod(x, axis=None, keepdims=False):
     x = _t(x)
     if axis is None:
-        return torch.prod(x)
-    return torch.prod(x, dim=axis, keepdim=keepdims)
+        return torch.prod(x, **_int_acc(x))
+    return torch.prod(x, dim=axis, keepdim=keepdims, **_int_acc(x))
 
 
 def mean(x, axis=None, keepdims=False):
@@ -325,7 +336,7 @@ def cumsum(x, axis=None):
     x = _t(x)
     if axis is None:
         x, axis = x.reshape(-1), 0
-    return torch.cumsum(x, dim=axis)
+    return torch.cumsum(x, dim=axis, **_int_acc(x))
 
 
 def diff(x, axis=-1):

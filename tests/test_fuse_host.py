@@ -103,7 +103,7 @@ def test_vector_limits_refused():
                                                   "c": jnp.sum(s["p"] > 0.5)},
                                  {"p": torch.rand(N, 3)}, {"env": {}})
     assert {k: v.dtype for k, v in outs.items()} == {"r": torch.float32, "m": torch.float32,
-                                                     "c": torch.int64}
+                                                     "c": torch.int32}   # JAX: int32 sums
     with pytest.raises(fuse.Unsupported):      # vectors of different lengths
         _trace(lambda s, m, k: {"r": s["p"] + s["q"]},
                {"p": torch.rand(N, 2), "q": torch.rand(N, 3)}, {"env": {}})
@@ -138,3 +138,15 @@ def test_walker_with_vectors_traces_and_compiles():
         "position": (torch.float32, (N, 2)), "velocity": (torch.float32, (N, 2)),
         "color": (torch.int32, (N,)), "steps_taken": (torch.int32, (N,))}
     fuse.compile_source(src)
+
+
+def test_integer_sums_are_int32_as_jax_x32():
+    """jnp.sum / prod / cumsum of booleans and int32 give int32 with
+    wrap-around, as JAX with x64 off (the reference's setting)."""
+    x = torch.tensor([2**31 - 1, 1], dtype=torch.int32)
+    s = jnp.sum(x)
+    assert s.dtype == torch.int32 and int(s) == -2**31
+    assert jnp.sum(torch.tensor([True, True, False])).dtype == torch.int32
+    assert jnp.prod(torch.tensor([3, 5], dtype=torch.int32)).dtype == torch.int32
+    assert jnp.cumsum(torch.tensor([1, 2, 3], dtype=torch.int32)).tolist() == [1, 3, 6]
+    assert jnp.sum(torch.tensor([0.5, 0.25])).dtype == torch.float32
