@@ -62,6 +62,16 @@ int abm_draw_each(const uint32_t *keys, int64_t n, int64_t per, int dist, float 
 /* vmap of randint(key, (per,), lo, hi) over keys[n][2] -> out[n][per]. */
 int abm_randint_each(const uint32_t *keys, int64_t n, int64_t per, int32_t lo, int32_t hi,
                      int32_t *out, int mode, void *stream);
+/* vmap over keys[n][2] of jax.random.gamma(key, a, (per,)) (kind ABM_GAMMA),
+ * loggamma (ABM_LOGGAMMA) or beta(key, a, b, (per,)) (ABM_BETA), float32,
+ * with the element parameters a[n][per] (and b[n][per]) -> out[n][per].
+ * Replaces the reference examples' random.beta calls
+ * (examples/models/advanced_economic_model.py:106,351,635,958,962). */
+#define ABM_GAMMA 0
+#define ABM_LOGGAMMA 1
+#define ABM_BETA 2
+int abm_gamma_each(const uint32_t *keys, int64_t n, int64_t per, const float *a,
+                   const float *b, int kind, float *out, int mode, void *stream);
 /* jax.random.permutation(key, n) (jaxabm/analysis.py:87); workspace from
  * abm_permutation_workspace(n). */
 size_t abm_permutation_workspace(int64_t n);

@@ -17,6 +17,8 @@ from .build import LIB_PATH
 MODE_ORIGINAL = 0
 MODE_PARTITIONABLE = 1
 
+GAMMA, LOGGAMMA, BETA = 0, 1, 2   # abm_gamma_each kinds
+
 _lock = threading.Lock()
 _lib = None
 
@@ -126,6 +128,7 @@ SIGNATURES = {
     "abm_split_each": (_int, [_p, _i64, _i64, _p, _int, _p]),
     "abm_draw_each": (_int, [_p, _i64, _i64, _int, _f32, _f32, _p, _int, _p]),
     "abm_randint_each": (_int, [_p, _i64, _i64, _i32, _i32, _p, _int, _p]),
+    "abm_gamma_each": (_int, [_p, _i64, _i64, _p, _p, _int, _p, _int, _p]),
     "abm_permutation_workspace": (_sz, [_i64]),
     "abm_permutation_rounds": (_int, [_i64]),
     "abm_permutation": (_int, [_p, _i64, _p, _p, _sz, _int, _p]),

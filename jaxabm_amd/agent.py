@@ -17,6 +17,7 @@ from typing import Any, Callable, Dict, Optional, Protocol
 import numpy as np
 import torch
 
+from . import numpy as jnp
 from . import random as jr
 from .core import ModelConfig
 
@@ -61,10 +62,36 @@ def _tensor_dict(out, what: str) -> Dict[str, torch.Tensor]:
     return {k: as_state_value(v) for k, v in out.items()}
 
 
-def vmap_agents(fn: Callable, *args, n: int) -> Dict[str, torch.Tensor]:
+_BOOL_SUB_FNS: set = set()   # agent functions that subtract booleans
+
+
+def _fn_id(fn):
+    f = getattr(fn, "__func__", fn)
+    return getattr(f, "__code__", None) or id(f)
+
+
+def vmap_agents(fn: Callable, *args, n: int, ident: Any = None) -> Dict[str, torch.Tensor]:
     """``vmap(fn)(*args)`` over the leading (agent) axis; outputs that do not
-    depend on the agent are broadcast to n agents."""
-    out = torch.func.vmap(lambda *a: _tensor_dict(fn(*a), "agent function"))(*args)
+    depend on the agent are broadcast to n agents.
+
+    A function that subtracts booleans (JAX promotes them, torch refuses) is
+    run again under ``jnp.BoolArithmetic`` and remembered by ``ident`` (agent
+    functions are pure, as under ``jax.vmap``, so the second run is the
+    first's)."""
+    mapped = torch.func.vmap(lambda *a: _tensor_dict(fn(*a), "agent function"))
+    key = ident if ident is not None else _fn_id(fn)
+    if key in _BOOL_SUB_FNS:
+        with jnp.BoolArithmetic():
+            out = mapped(*args)
+    else:
+        try:
+            out = mapped(*args)
+        except RuntimeError as e:
+            if jnp.BOOL_SUB_ERROR not in str(e):
+                raise
+            _BOOL_SUB_FNS.add(key)
+            with jnp.BoolArithmetic():
+                out = mapped(*args)
     return {k: (v if v.dim() and v.shape[0] == n else v.expand(n, *v.shape)).contiguous()
             for k, v in out.items()}
 
@@ -105,7 +132,7 @@ class AgentCollection:
             return
         agent_keys = jr.split(key, self.num_agents)
         self._states = vmap_agents(lambda k: init_method(model_config, k), agent_keys,
-                                   n=self.num_agents)
+                                   n=self.num_agents, ident=_fn_id(init_method))
 
     def update(self, model_state: Dict[str, Any], key: Any, model_config: ModelConfig) -> None:
         """Agent keys = split(key, num_agents); states = vmap(update)."""
@@ -128,7 +155,7 @@ class AgentCollection:
         agent_keys = jr.split(key, self.num_agents)
         self._states = vmap_agents(
             lambda s, k: update_method(s, model_state, model_config, k), self._states, agent_keys,
-            n=self.num_agents)
+            n=self.num_agents, ident=_fn_id(update_method))
 
     def get_states(self) -> Dict[str, Any]:
         return self._states

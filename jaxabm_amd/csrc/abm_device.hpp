@@ -167,4 +167,87 @@ __device__ __forceinline__ float normal_from_bits(uint32_t b) {
   return 1.41421354f * erfinv_f32(u);
 }
 
+// ---- gamma, log-gamma and beta draws (jax.random.gamma / loggamma / beta)
+//
+// Marsaglia & Tsang, "A simple method for generating gamma variables",
+// ACM TOMS 26(3), 2000, with jax.random's key layout (jax/_src/random.py):
+//  * element j of a draw of `count` elements uses key split(key, count)[j];
+//  * that key splits in two: the first half drives the rejection loop, the
+//    second draws the boost uniform;
+//  * each rejection round splits its key in three (next round, normal,
+//    uniform); the redraw of the normal while v <= 0 splits its key in two
+//    (next, draw);
+//  * alpha < 1 samples Gamma(alpha + 1) and scales by u^(1/alpha) with
+//    u = 1 - uniform, or in log space adds log1p(-uniform) / alpha unless
+//    that is zero.
+// float32 throughout, as jax_enable_x64 off.  Rejection rounds are capped far
+// beyond any reachable count (acceptance is at least 0.95 per round) so that
+// every lane ends; a capped lane returns NaN.
+constexpr int kGammaMaxRounds = 4096;
+
+__device__ __forceinline__ bool gamma_reject(float X, float V, float U, float d) {
+  const bool squeeze = U >= 1.0f - 0.0331f * (X * X);
+  const bool full = logf(U) >= X * 0.5f + d * ((1.0f - V) + logf(V));
+  return squeeze && full;
+}
+
+__device__ inline float gamma_one(Key key, float alpha, bool log_space, int mode) {
+  const bool boost = alpha >= 1.0f;
+  const float a = boost ? alpha : alpha + 1.0f;
+  const float third = 0.333333343f;   // float32(1/3)
+  const float d = a - third;
+  const float c = third / sqrtf(d);
+  Key k, sub;
+  split2(key, k, sub, mode);
+  float X = 0.0f, V = 1.0f, U = 2.0f;   // a start the loop condition rejects
+  for (int round = 0; gamma_reject(X, V, U, d); ++round) {
+    if (round == kGammaMaxRounds) return __builtin_nanf("");
+    const Key next = split_at(k, 3, 0, mode);
+    Key xk = split_at(k, 3, 1, mode);
+    const Key uk = split_at(k, 3, 2, mode);
+    k = next;
+    float x = 0.0f, v = -1.0f;
+    for (int redraw = 0; v <= 0.0f; ++redraw) {
+      if (redraw == kGammaMaxRounds) return __builtin_nanf("");
+      Key nk, dk;
+      split2(xk, nk, dk, mode);
+      xk = nk;
+      x = normal_from_bits(bits_scalar(dk, mode));
+      v = 1.0f + x * c;
+    }
+    X = x * x;
+    V = (v * v) * v;
+    U = unit_float(bits_scalar(uk, mode));
+  }
+  const float u = unit_float(bits_scalar(sub, mode));
+  if (log_space) {
+    const float ls = log1pf(-u);
+    const float lb = (boost || ls == 0.0f) ? 0.0f : ls * (1.0f / alpha);
+    return (logf(d) + logf(V)) + lb;
+  }
+  const float s = 1.0f - u;
+  const float b = boost ? 1.0f : powf(s, 1.0f / alpha);
+  return (d * V) * b;
+}
+
+// Element j of gamma(key, alpha, (count,)) (log_space: loggamma).
+__device__ inline float gamma_draw(Key key, int64_t count, int64_t j, float alpha,
+                                   bool log_space, int mode) {
+  return gamma_one(split_at(key, count, j, mode), alpha, log_space, mode);
+}
+
+// Element j of beta(key, a, b, (count,)): split(key) into the keys of
+// loggamma(a) and loggamma(b), then exp(la - m) / (exp(la - m) + exp(lb - m))
+// with m = max(la, lb) (NaN if either is NaN).
+__device__ inline float beta_draw(Key key, int64_t count, int64_t j, float a, float b,
+                                  int mode) {
+  Key ka, kb;
+  split2(key, ka, kb, mode);
+  const float la = gamma_draw(ka, count, j, a, true, mode);
+  const float lb = gamma_draw(kb, count, j, b, true, mode);
+  const float m = (la != la || lb != lb) ? la + lb : (la > lb ? la : lb);
+  const float ea = expf(la - m), eb = expf(lb - m);
+  return ea / (ea + eb);
+}
+
 }  // namespace abm

@@ -139,6 +139,21 @@ __global__ void k_randint_each(const uint32_t *__restrict__ keys, int64_t n,
   out[t] = (int32_t)((uint32_t)lo + randint_offset(hb, lb, span));
 }
 
+// gamma / loggamma / beta elements: out[t], t = i * per + j, from keys[i] and
+// the element's parameters a[t] (and b[t] for beta).  Rejection loops end per
+// lane (gamma_one's cap); 64-thread blocks keep a slow lane's wave short.
+__global__ void __launch_bounds__(64)
+    k_gamma_each(const uint32_t *__restrict__ keys, int64_t n, int64_t per,
+                 const float *__restrict__ a, const float *__restrict__ b,
+                 int kind, float *__restrict__ out, int mode) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * per) return;
+  const int64_t i = t / per, j = t - i * per;
+  const Key k{keys[2 * i], keys[2 * i + 1]};
+  out[t] = kind == ABM_BETA ? beta_draw(k, per, j, a[t], b[t], mode)
+                            : gamma_draw(k, per, j, a[t], kind == ABM_LOGGAMMA, mode);
+}
+
 static inline unsigned grid_for(int64_t n, int bs) {
   return (unsigned)((n + bs - 1) / bs);
 }
@@ -281,6 +296,19 @@ int abm_randint_each(const uint32_t *keys, int64_t n, int64_t per, int32_t lo,
   hipLaunchKernelGGL(k_randint_each, dim3(grid_for(n * per, 256)), dim3(256),
                      0, (hipStream_t)stream, keys, n, per, out, lo, span, mode);
   ABM_LAUNCH_CHECK("abm_randint_each");
+  return kOk;
+}
+
+int abm_gamma_each(const uint32_t *keys, int64_t n, int64_t per, const float *a,
+                   const float *b, int kind, float *out, int mode, void *stream) {
+  ABM_REQUIRE(n >= 0 && per >= 0 && (mode == 0 || mode == 1) &&
+                  (kind == ABM_GAMMA || kind == ABM_LOGGAMMA || kind == ABM_BETA),
+              "abm_gamma_each: bad n, per, kind or mode");
+  if (n == 0 || per == 0) return kOk;
+  ABM_REQUIRE(keys && a && out && (kind != ABM_BETA || b), "abm_gamma_each: null buffer");
+  hipLaunchKernelGGL(k_gamma_each, dim3(grid_for(n * per, 64)), dim3(64), 0,
+                     (hipStream_t)stream, keys, n, per, a, b, kind, out, mode);
+  ABM_LAUNCH_CHECK("abm_gamma_each");
   return kOk;
 }
 

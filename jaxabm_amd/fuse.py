@@ -31,12 +31,16 @@ operand's sign.
   them.
 * Draws are the ``abm_device.hpp`` functions the vmapped draw kernels call:
   agent i's key is ``split(key, n)[i]``; ``split``, ``bits``, ``uniform``,
-  ``normal``, ``bernoulli`` and ``randint`` on it give JAX's bits.
+  ``normal``, ``bernoulli``, ``randint``, ``gamma``, ``loggamma`` and
+  ``beta`` on it give the same values as the vmapped draws (``lognormal``
+  and ``exponential`` are built from ``normal`` and ``uniform``).
+* Per-agent vectors of up to 16 values are unrolled.  float32 sums and means
+  over 2 to 7 of them follow torch's summation order; integer sums are exact.
 
 Anything else is not traced.  That includes reading other agents' arrays,
-per-agent vectors, reductions, Python control flow on a traced value and
-host reads; ``Unsupported`` is raised, and the collection runs the vmapped
-path, which is remembered per function and input signature.
+longer vectors and other reductions, Python control flow on a traced value
+and host reads; ``Unsupported`` is raised, and the collection runs the
+vmapped path, which is remembered per function and input signature.
 """
 from __future__ import annotations
 
@@ -338,6 +342,29 @@ class SymKey:
             lb = tr.tmp("uint32_t", f"abm::bits_at({b}, {per}LL, {j}LL, {tr.mode})")
             expr = f"(int32_t)((uint32_t)({lo}) + abm::randint_offset({hb}, {lb}, {span}u))"
             return Sym(tr, tr.tmp("int32_t", expr), "i32", True)
+
+        return self._shaped(shape, one)
+
+    def gamma(self, a, b, kind: int, shape=()):
+        """gamma / loggamma (b None) or beta draws: the device sampler the
+        library's kernel runs, with float32 parameters per element."""
+        from ._lib import BETA, LOGGAMMA
+
+        tr = self.tr
+        shape = tuple(shape)
+        params = [tr.cast(tr.voperand(v), "f32") for v in ((a,) if b is None else (a, b))]
+        for p in params:
+            if isinstance(p, SymVec) and (shape == () or len(p) != shape[0]):
+                raise Unsupported("a draw parameter that does not match the draw's shape")
+
+        def one(per, j):
+            ps = [tr._elem(p, j).name for p in params]
+            if kind == BETA:
+                expr = f"abm::beta_draw({self.name}, {per}LL, {j}LL, {ps[0]}, {ps[1]}, {tr.mode})"
+            else:
+                log = "true" if kind == LOGGAMMA else "false"
+                expr = f"abm::gamma_draw({self.name}, {per}LL, {j}LL, {ps[0]}, {log}, {tr.mode})"
+            return Sym(tr, tr.tmp("float", expr), "f32", True)
 
         return self._shaped(shape, one)
 
@@ -941,6 +968,30 @@ class _Trace:
                       kind, x.varying)
         return out
 
+    def nan_to_num(self, x, nan, posinf, neginf):
+        """torch.nan_to_num / jnp.nan_to_num: NaN, +inf and -inf replaced by
+        the Python values given (infinities default to the dtype's largest
+        finite magnitude), converted to the dtype; other dtypes unchanged."""
+        x = self.voperand(x)
+        if isinstance(x, SymVec):
+            return SymVec(self, [self.nan_to_num(v, nan, posinf, neginf) for v in x.items])
+        if x.weak:
+            raise Unsupported("nan_to_num of a Python scalar")
+        if x.kind not in ("f32", "f64"):
+            return x
+        big = float(np.finfo(np.float32 if x.kind == "f32" else np.float64).max)
+        vals = []
+        for v, default in ((nan, 0.0), (posinf, big), (neginf, -big)):
+            if v is None:
+                v = default
+            if isinstance(v, bool) or not isinstance(v, (int, float)):
+                raise Unsupported("nan_to_num with a replacement that is not a Python number")
+            vals.append(_lit(float(v), x.kind))
+        c = x.name
+        expr = (f"({c} != {c} ? {vals[0]} : ({c} == __builtin_inf() ? {vals[1]} : "
+                f"({c} == -__builtin_inf() ? {vals[2]} : {c})))")
+        return Sym(self, self.tmp(_CT[x.kind], expr), x.kind, x.varying)
+
     # -- torch functions called with traced values
     def torch_call(self, func, args, kwargs):
         name = getattr(func, "__name__", "")
@@ -991,6 +1042,13 @@ class _Trace:
             return self.clamp(x, lo, hi)
         if name == "pow" and len(args) == 2 and not kwargs:
             return self.power(args[0], args[1])
+        if name == "nan_to_num" and 1 <= len(args) <= 4:
+            opts = dict(zip(("nan", "posinf", "neginf"), args[1:]))
+            opts.update(kwargs)
+            if set(opts) - {"nan", "posinf", "neginf"}:
+                raise Unsupported("nan_to_num options")
+            return self.nan_to_num(args[0], opts.get("nan", 0.0), opts.get("posinf"),
+                                   opts.get("neginf"))
         if name in ("sum", "mean") and len(args) == 1 and isinstance(args[0], SymVec):
             dim = kwargs.get("dim")
             dt = kwargs.get("dtype")

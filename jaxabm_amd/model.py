@@ -20,8 +20,9 @@ from typing import Any, Callable, Dict, List, Optional
 import numpy as np
 import torch
 
+from . import numpy as jnp
 from . import random as jr
-from .agent import AgentCollection, vmap_agents
+from .agent import AgentCollection, _fn_id, vmap_agents
 from .core import ModelConfig
 
 
@@ -41,8 +42,9 @@ class Model:
         self._env_state: Dict[str, Any] = {}
         self._state: Optional[Dict[str, Any]] = None
         self._params = params or {}
-        self._update_state_fn = update_state_fn
-        self._metrics_fn = metrics_fn
+        # the callbacks see JAX's promotion of booleans in subtraction
+        self._update_state_fn = jnp.with_bool_arithmetic(update_state_fn)
+        self._metrics_fn = jnp.with_bool_arithmetic(metrics_fn)
         self._time_step = 0
         self._history: List[Dict[str, Any]] = []
         self._is_initialized = False
@@ -159,7 +161,8 @@ class Model:
                 n = next(iter(states.values())).shape[0]
                 upd = coll.agent_type.update
                 updated[name] = vmap_agents(
-                    lambda s, k: upd(s, model_state, self.config, k), states, jr.split(sub, n), n=n)
+                    lambda s, k: upd(s, model_state, self.config, k), states, jr.split(sub, n), n=n,
+                    ident=_fn_id(upd))
             if self._update_state_fn:
                 key, uk = jr.split(key)
                 env_state = self._update_state_fn(env_state, updated, params, uk)

@@ -176,6 +176,57 @@ def clip(x, a_min=None, a_max=None):
     return torch.clamp(_t(x), min=a_min, max=a_max)
 
 
+# ---------------------------------------------------------------- bool arithmetic
+_SUBTRACTIONS = frozenset({torch.sub, torch.subtract, torch.rsub, torch.Tensor.sub,
+                           torch.Tensor.subtract, torch.Tensor.__sub__, torch.Tensor.__rsub__})
+BOOL_SUB_ERROR = "with a bool tensor is not supported"   # torch's message
+
+
+def _is_bool(x) -> bool:
+    return isinstance(x, torch.Tensor) and x.dtype == torch.bool
+
+
+class BoolArithmetic(torch.overrides.TorchFunctionMode):
+    """JAX promotes a boolean operand of a subtraction (``1.0 - employed``
+    in the reference's examples/models/advanced_economic_model.py:184,
+    ``x - mask``, ``mask - 1``); torch refuses any bool operand of ``-``.
+    Inside this mode such a bool tensor is first cast to torch's result type
+    of the pair.  bool - bool and the negation of a bool still raise, as
+    they do in JAX."""
+
+    def __torch_function__(self, func, types, args=(), kwargs=None):
+        kwargs = kwargs or {}
+        if func in _SUBTRACTIONS and len(args) >= 2 and _is_bool(args[0]) != _is_bool(args[1]):
+            dt = torch.result_type(args[0], args[1])
+            if dt != torch.bool:
+                args = tuple(a.to(dt) if _is_bool(a) else a for a in args[:2]) + tuple(args[2:])
+        return func(*args, **kwargs)
+
+
+def with_bool_arithmetic(fn):
+    """``fn`` run inside :class:`BoolArithmetic` (None stays None)."""
+    if fn is None:
+        return None
+
+    def wrapped(*args, **kwargs):
+        with BoolArithmetic():
+            return fn(*args, **kwargs)
+
+    wrapped.__wrapped__ = fn
+    return wrapped
+
+
+def nan_to_num(x, copy=True, nan=0.0, posinf=None, neginf=None):
+    """``jnp.nan_to_num``: NaN -> ``nan``, +inf -> ``posinf`` and -inf ->
+    ``neginf`` (the infinities default to the dtype's largest finite
+    magnitude); integer and boolean arrays come back unchanged (used by the
+    reference's examples/models/advanced_economic_model.py:1860-1869)."""
+    t = _t(x)
+    if not t.is_floating_point():
+        return t
+    return torch.nan_to_num(t, nan=nan, posinf=posinf, neginf=neginf)
+
+
 def _unary(fn):
     return lambda x: fn(_t(x))
 

@@ -31,6 +31,7 @@ __all__ = [
     "bits", "uniform", "normal", "bernoulli", "randint", "choice", "permutation", "permutation_each",
     "split_each", "uniform_each", "normal_each", "bernoulli_each",
     "randint_each", "bits_each", "set_threefry_partitionable", "default_mode",
+    "gamma", "loggamma", "beta", "lognormal", "exponential",
 ]
 
 _default_mode = MODE_PARTITIONABLE
@@ -225,6 +226,41 @@ def _(info, in_dims, key, shape, lo, hi, mode):
     return randint_each(_batched_keys(info, in_dims[0], key), tuple(shape), lo, hi, mode), 0
 
 
+def _gamma_each_impl(keys: torch.Tensor, a: torch.Tensor, b: torch.Tensor, kind: int,
+                     mode: int) -> torch.Tensor:
+    """keys (n, 2); a, b (n, *shape) float32 -> (n, *shape)."""
+    k, a, b = keys.contiguous(), a.contiguous(), b.contiguous()
+    out = torch.empty(a.shape, dtype=torch.float32, device=k.device)
+    n = k.shape[0]
+    per = a.numel() // n if n else 0
+    check(lib().abm_gamma_each(ptr(k), n, per, ptr(a), ptr(b), kind, ptr(out), mode,
+                               stream_handle()), "gamma")
+    return out
+
+
+@torch.library.custom_op("jaxabm::gamma", mutates_args=(), device_types="cuda")
+def _op_gamma(key: torch.Tensor, a: torch.Tensor, b: torch.Tensor, kind: int,
+              mode: int) -> torch.Tensor:
+    out = _gamma_each_impl(key.reshape(1, 2), a.reshape(1, *a.shape), b.reshape(1, *b.shape),
+                           kind, mode)
+    return out.reshape(a.shape)
+
+
+@_op_gamma.register_fake
+def _(key, a, b, kind, mode):
+    return a.new_empty(a.shape)
+
+
+@_op_gamma.register_vmap
+def _(info, in_dims, key, a, b, kind, mode):
+    k = _batched_keys(info, in_dims[0], key)
+
+    def batched(t, d):
+        return t.movedim(d, 0) if d is not None else t.unsqueeze(0).expand(info.batch_size, *t.shape)
+
+    return _gamma_each_impl(k, batched(a, in_dims[1]), batched(b, in_dims[2]), kind, mode), 0
+
+
 def _key_arg(key) -> torch.Tensor:
     if isinstance(key, torch.Tensor):
         if key.shape[-1:] != (2,):
@@ -314,6 +350,85 @@ def randint(key, shape, minval, maxval, dtype=torch.int32, mode=None) -> torch.T
         _traced_scalar_shape(key, shape, mode)
         return key.randint(lo, hi, _shape(shape))
     return torch.ops.jaxabm.randint(_key_arg(key), list(_shape(shape)), lo, hi, _mode(mode))
+
+
+def _param_shape(params, shape) -> tuple:
+    if shape is not None:
+        return _shape(shape)
+    return tuple(torch.broadcast_shapes(*(tuple(getattr(v, "shape", ())) for v in params)))
+
+
+def _param(v, shape: tuple, device) -> torch.Tensor:
+    """A draw parameter as float32 (jax.random converts it to the draw's
+    dtype) broadcast to the draw's shape."""
+    if isinstance(v, torch.Tensor):
+        t = v.to(device=device, dtype=torch.float32)
+    else:
+        t = torch.full((), float(v), dtype=torch.float32, device=device)
+    return t.broadcast_to(shape)
+
+
+def _gamma_family(key, a, b, shape, kind: int, dtype, mode):
+    if dtype not in (torch.float32, None):
+        raise NotImplementedError("gamma/beta: float32 only (the reference runs with "
+                                  "jax_enable_x64 off)")
+    params = (a,) if b is None else (a, b)
+    if _traced_key(key):
+        from .fuse import SymVec
+
+        shp = () if shape is None else _shape(shape)
+        if shape is None:
+            lens = {len(v) for v in params if isinstance(v, SymVec)}
+            shp = (lens.pop(),) if len(lens) == 1 else shp
+        _traced_scalar_shape(key, shp, mode)
+        return key.gamma(a, b, kind, shp)
+    k = _key_arg(key)
+    shp = _param_shape(params, shape)
+    at = _param(a, shp, k.device)
+    bt = _param(b, shp, k.device) if b is not None else at
+    return torch.ops.jaxabm.gamma(k, at, bt, kind, _mode(mode))
+
+
+def gamma(key, a, shape=None, dtype=torch.float32, mode=None) -> torch.Tensor:
+    """``jax.random.gamma(key, a, shape, float32)``: Marsaglia-Tsang rejection
+    sampling with jax.random's key layout (element j of a draw of k elements
+    from key ``split(key, k)[j]``); ``shape`` defaults to ``a``'s."""
+    return _gamma_family(key, a, None, shape, _lib.GAMMA, dtype, mode)
+
+
+def loggamma(key, a, shape=None, dtype=torch.float32, mode=None) -> torch.Tensor:
+    """``jax.random.loggamma(key, a, shape, float32)``: log of a gamma draw,
+    computed in log space (accurate for small ``a``)."""
+    return _gamma_family(key, a, None, shape, _lib.LOGGAMMA, dtype, mode)
+
+
+def beta(key, a, b, shape=None, dtype=torch.float32, mode=None) -> torch.Tensor:
+    """``jax.random.beta(key, a, b, shape, float32)`` (called by the reference's
+    examples/models/advanced_economic_model.py:106,351,635,958,962): loggamma
+    draws of a and b from the two halves of ``split(key)``, combined as
+    ``exp(la - m) / (exp(la - m) + exp(lb - m))``, ``m = max(la, lb)``."""
+    return _gamma_family(key, a, b, shape, _lib.BETA, dtype, mode)
+
+
+def lognormal(key, sigma=1.0, shape=None, dtype=torch.float32, mode=None) -> torch.Tensor:
+    """``jax.random.lognormal(key, sigma, shape, float32)`` =
+    ``exp(normal(key, shape) * sigma)`` (examples/models/
+    advanced_economic_model.py:98,343,627,950)."""
+    if dtype not in (torch.float32, None):
+        raise NotImplementedError("lognormal: float32 only")
+    shp = _param_shape((sigma,), shape) if not _traced_key(key) or shape is not None else ()
+    z = normal(key, shp, mode=mode)
+    if isinstance(sigma, torch.Tensor):
+        sigma = sigma.to(torch.float32)
+    return torch.exp(z * sigma)
+
+
+def exponential(key, shape=(), dtype=torch.float32, mode=None) -> torch.Tensor:
+    """``jax.random.exponential(key, shape, float32)`` = ``-log1p(-uniform)``."""
+    if dtype not in (torch.float32, None):
+        raise NotImplementedError("exponential: float32 only")
+    u = uniform(key, shape, mode=mode)
+    return torch.neg(torch.log1p(torch.neg(u)))
 
 
 def choice(key, a, shape=(), mode=None):

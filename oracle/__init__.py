@@ -72,6 +72,8 @@ def _declare(L):
         "orc_randint_i32": (None, [_u32p, _i64, _i32, _i32, _i32p, C.c_int]),
         "orc_erfinv_f32": (C.c_float, [C.c_float]),
         "orc_normal_f32": (None, [_u32p, _i64, _f32p, C.c_int]),
+        "orc_gamma_f32": (None, [_u32p, _i64, _f32p, C.c_int, _f32p, C.c_int]),
+        "orc_beta_f32": (None, [_u32p, _i64, _f32p, _f32p, _f32p, C.c_int]),
         "orc_permutation_rounds": (_i32, [_i64]),
         "orc_permutation": (None, [_u32p, _i64, _i32p, C.c_int]),
         "orc_ba_num_edges": (_i64, [_i64, _i32]),
@@ -155,6 +157,24 @@ def erfinv(x: float) -> float:
 def normal(key, n: int, mode: int = MODE_PARTITIONABLE) -> np.ndarray:
     out = np.zeros(max(n, 1), np.float32)
     lib().orc_normal_f32(_key(key), n, out, mode)
+    return out[:n]
+
+
+def _params(v, n: int) -> np.ndarray:
+    return np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float32), (n,)))
+
+
+def gamma(key, a, n: int, log_space: bool = False,
+          mode: int = MODE_PARTITIONABLE) -> np.ndarray:
+    """gamma(key, a, (n,)) (loggamma with log_space), a scalar or (n,)."""
+    out = np.zeros(max(n, 1), np.float32)
+    lib().orc_gamma_f32(_key(key), n, _params(a, max(n, 1)), int(log_space), out, mode)
+    return out[:n]
+
+
+def beta(key, a, b, n: int, mode: int = MODE_PARTITIONABLE) -> np.ndarray:
+    out = np.zeros(max(n, 1), np.float32)
+    lib().orc_beta_f32(_key(key), n, _params(a, max(n, 1)), _params(b, max(n, 1)), out, mode)
     return out[:n]
 
 

@@ -218,6 +218,95 @@ void orc_normal_f32(const uint32_t key[2], int64_t n, float *out, int mode) {
   for (int64_t i = 0; i < n; ++i) out[i] = sqrt2 * orc_erfinv_f32(out[i]);
 }
 
+/* ---- gamma, log-gamma and beta: jax.random.gamma / loggamma / beta (the
+ * reference's examples/models/advanced_economic_model.py:106,351,635,958,962
+ * call beta).  Marsaglia & Tsang, "A simple method for generating gamma
+ * variables", ACM TOMS 26(3), 2000, in JAX's key layout (JAX is an
+ * un-vendored dependency, jax>=0.4.1 unpinned, requirements.txt:9-10):
+ *   element j of a draw of `count` values uses k = split(key, count)[j];
+ *   (k_loop, k_boost) = split(k);
+ *   a rejection round: (k_loop, k_x, k_u) = split(k_loop, 3); then, while
+ *   v <= 0: (k_x, k_d) = split(k_x), x = normal(k_d), v = 1 + x c;
+ *   X = x^2, V = v^3, U = uniform(k_u); the round repeats while
+ *   U >= 1 - 0.0331 X^2 and log U >= X/2 + d (1 - V + log V), starting from
+ *   (X, V, U) = (0, 1, 2);
+ *   alpha < 1 runs with alpha + 1 and multiplies by (1 - uniform(k_boost))^
+ *   (1/alpha), or in log space adds log1p(-uniform(k_boost)) * (1/alpha)
+ *   unless that logarithm is 0;
+ *   beta(a, b): (ka, kb) = split(key), la = loggamma(ka, a), lb =
+ *   loggamma(kb, b), m = max(la, lb), exp(la - m) / (exp(la - m) +
+ *   exp(lb - m)).
+ * Bit parity with JAX is unpinned (JAX cannot run here); the tests pin the
+ * sampled distributions against scipy.stats. */
+static float orc_gamma_one(const uint32_t key[2], float alpha, int log_space, int mode) {
+  const int boost = alpha >= 1.0f;
+  const float a = boost ? alpha : alpha + 1.0f;
+  const float third = (float)(1.0 / 3.0);
+  const float d = a - third;
+  const float c = third / sqrtf(d);
+  uint32_t ks[4], k[2], sub[2];
+  orc_split(key, 2, ks, mode);
+  k[0] = ks[0]; k[1] = ks[1];
+  sub[0] = ks[2]; sub[1] = ks[3];
+  float X = 0.0f, V = 1.0f, U = 2.0f;
+  for (;;) {
+    const float squeeze = 1.0f - 0.0331f * (X * X);
+    const float rhs = X * 0.5f + d * ((1.0f - V) + logf(V));
+    if (!(U >= squeeze && logf(U) >= rhs)) break;
+    uint32_t k3[6], xk[2], uk[2];
+    orc_split(k, 3, k3, mode);
+    k[0] = k3[0]; k[1] = k3[1];
+    xk[0] = k3[2]; xk[1] = k3[3];
+    uk[0] = k3[4]; uk[1] = k3[5];
+    float x = 0.0f, v = -1.0f;
+    while (v <= 0.0f) {
+      uint32_t k2[4];
+      orc_split(xk, 2, k2, mode);
+      xk[0] = k2[0]; xk[1] = k2[1];
+      orc_normal_f32(k2 + 2, 1, &x, mode);
+      v = 1.0f + x * c;
+    }
+    X = x * x;
+    V = (v * v) * v;
+    orc_uniform_f32(uk, 1, 0.0f, 1.0f, &U, mode);
+  }
+  float u;
+  orc_uniform_f32(sub, 1, 0.0f, 1.0f, &u, mode);
+  if (log_space) {
+    const float ls = log1pf(-u);
+    const float lb = (boost || ls == 0.0f) ? 0.0f : ls * (1.0f / alpha);
+    return (logf(d) + logf(V)) + lb;
+  }
+  const float s = 1.0f - u;
+  const float b = boost ? 1.0f : powf(s, 1.0f / alpha);
+  return (d * V) * b;
+}
+
+void orc_gamma_f32(const uint32_t key[2], int64_t count, const float *alpha, int log_space,
+                   float *out, int mode) {
+  uint32_t *ks = (uint32_t *)malloc(sizeof(uint32_t) * 2 * (size_t)(count > 0 ? count : 1));
+  orc_split(key, count, ks, mode);
+  for (int64_t j = 0; j < count; ++j) out[j] = orc_gamma_one(ks + 2 * j, alpha[j], log_space, mode);
+  free(ks);
+}
+
+void orc_beta_f32(const uint32_t key[2], int64_t count, const float *a, const float *b,
+                  float *out, int mode) {
+  uint32_t kab[4];
+  orc_split(key, 2, kab, mode);
+  float *la = (float *)malloc(sizeof(float) * (size_t)(count > 0 ? count : 1));
+  float *lb = (float *)malloc(sizeof(float) * (size_t)(count > 0 ? count : 1));
+  orc_gamma_f32(kab, count, a, 1, la, mode);
+  orc_gamma_f32(kab + 2, count, b, 1, lb, mode);
+  for (int64_t j = 0; j < count; ++j) {
+    const float m = (isnan(la[j]) || isnan(lb[j])) ? la[j] + lb[j] : (la[j] > lb[j] ? la[j] : lb[j]);
+    const float ea = expf(la[j] - m), eb = expf(lb[j] - m);
+    out[j] = ea / (ea + eb);
+  }
+  free(la);
+  free(lb);
+}
+
 /* jax.random.permutation(key, n) via _shuffle: rounds = ceil(3 ln(max(1,n)) /
  * ln(2^32-1)); each round key, sub = split(key); sort_key_val(bits(sub), x)
  * stable by key. */

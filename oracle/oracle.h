@@ -45,6 +45,12 @@ void orc_randint_i32(const uint32_t key[2], int64_t n, int32_t lo, int32_t hi,
                      int32_t *out, int mode);
 float orc_erfinv_f32(float x);
 void orc_normal_f32(const uint32_t key[2], int64_t n, float *out, int mode);
+/* gamma(key, alpha[j], (count,)) (log_space: loggamma) and
+ * beta(key, a[j], b[j], (count,)), float32. */
+void orc_gamma_f32(const uint32_t key[2], int64_t count, const float *alpha, int log_space,
+                   float *out, int mode);
+void orc_beta_f32(const uint32_t key[2], int64_t count, const float *a, const float *b,
+                  float *out, int mode);
 int32_t orc_permutation_rounds(int64_t n);
 void orc_permutation(const uint32_t key[2], int64_t n, int32_t *out, int mode);
 

@@ -150,3 +150,58 @@ def test_integer_sums_are_int32_as_jax_x32():
     assert jnp.prod(torch.tensor([3, 5], dtype=torch.int32)).dtype == torch.int32
     assert jnp.cumsum(torch.tensor([1, 2, 3], dtype=torch.int32)).tolist() == [1, 3, 6]
     assert jnp.sum(torch.tensor([0.5, 0.25])).dtype == torch.float32
+
+
+def test_gamma_family_and_nan_to_num_trace_and_compile():
+    """beta / gamma / loggamma / lognormal / exponential and nan_to_num as
+    the reference's economic example uses them
+    (examples/models/advanced_economic_model.py:98-106, 1860-1869)."""
+    def init(s, m, k):
+        k1, k2, k3, k4 = jr.split(k, 4)
+        a = jr.lognormal(k1, sigma=0.5) * 1000.0
+        return {"a": a, "b": jr.beta(k2, a=5, b=2) * 0.4 + 0.6,
+                "g": jr.gamma(k3, 2.5) + jr.loggamma(k3, 0.3),
+                "v": jr.beta(k4, 2.0, 5.0, (3,)), "e": jr.exponential(k4),
+                "n": jnp.nan_to_num(jnp.log(a - 1000.0), nan=0.1, posinf=5.0)}
+
+    name, src, tr, outs = _trace(init)
+    assert {k: (v.dtype, tuple(v.shape)) for k, v in outs.items()} == {
+        "a": (torch.float32, (N,)), "b": (torch.float32, (N,)), "g": (torch.float32, (N,)),
+        "v": (torch.float32, (N, 3)), "e": (torch.float32, (N,)), "n": (torch.float32, (N,))}
+    assert "abm::beta_draw" in src and "abm::gamma_draw" in src
+    fuse.compile_source(src)
+
+    def upd(s, m, k):    # a per-agent shape parameter
+        return {"x": jr.gamma(k, s["x"] + 0.5), "w": jnp.nan_to_num(s["w"])}
+
+    name, src, tr, outs = _trace(upd, _states(), {"env": {}})
+    assert outs["x"].dtype == torch.float32 and outs["w"].dtype == torch.int32
+    fuse.compile_source(src)
+
+
+def test_bool_subtraction_follows_jax():
+    """JAX promotes a boolean operand of a subtraction; torch refuses it.
+    The traced kernel and the vmapped path (jnp.BoolArithmetic) both give
+    JAX's values; bool - bool and -bool still raise, as in JAX."""
+    from jaxabm_amd.agent import _BOOL_SUB_FNS, vmap_agents
+
+    m = torch.tensor([True, False, True])
+    x = torch.tensor([0.5, 1.5, 2.5])
+
+    def fn(a, b):
+        return {"y": 1.0 - a, "z": b - a, "w": a - 2}
+
+    out = vmap_agents(fn, m, x, n=3)
+    assert out["y"].tolist() == [0.0, 1.0, 0.0] and out["z"].tolist() == [-0.5, 1.5, 1.5]
+    assert out["w"].dtype == torch.int64 and out["w"].tolist() == [-1, -2, -1]
+    assert fn.__code__ in _BOOL_SUB_FNS
+    with jnp.BoolArithmetic():
+        with pytest.raises(RuntimeError):
+            m - m
+        with pytest.raises(RuntimeError):
+            -m
+    name, src, tr, outs = _trace(lambda s, m_, k: {"y": 1.0 - s["alive"], "w": s["alive"] - 2},
+                                 _states(), {"env": {}})
+    assert outs["y"].dtype == torch.float32 and outs["w"].dtype == torch.int64
+    with pytest.raises(fuse.Unsupported):
+        _trace(lambda s, m_, k: {"y": -s["alive"]}, _states(), {"env": {}})
