@@ -156,7 +156,9 @@ class Sym:
         return self.tr.cast(self, _KIND[dt])
 
     def astype(self, dt):
-        return self.to(dt)
+        from .numpy import jax_dtype
+
+        return self.to(jax_dtype(dt))
 
     def float(self):
         return self.to(torch.float32)
@@ -457,7 +459,7 @@ class SymVec:
         return SymVec(self.tr, [x.to(*args, **kwargs) for x in self.items])
 
     def astype(self, dt):
-        return self.to(dt)
+        return SymVec(self.tr, [x.astype(dt) for x in self.items])
 
     def float(self):
         return self.to(torch.float32)

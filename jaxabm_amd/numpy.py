@@ -50,8 +50,41 @@ def _default_dtype(value):
     return None
 
 
+def jax_dtype(dt) -> torch.dtype:
+    """A dtype argument read as JAX reads it with jax_enable_x64 off (the
+    reference's setting): Python ``bool``/``int``/``float`` -> bool, int32,
+    float32; 64-bit numpy dtypes -> their 32-bit forms; unsigned 32-bit ->
+    int32 storage (as ``uint32`` here); torch dtypes as they are."""
+    if isinstance(dt, torch.dtype):
+        return dt
+    if dt is bool:
+        return torch.bool
+    if dt is int:
+        return torch.int32
+    if dt is float:
+        return torch.float32
+    nd = _np.dtype(dt)
+    nd = {_np.dtype(_np.float64): _np.float32, _np.dtype(_np.int64): _np.int32,
+          _np.dtype(_np.uint64): _np.int32, _np.dtype(_np.uint32): _np.int32}.get(nd, nd)
+    return torch.from_numpy(_np.zeros(0, nd)).dtype
+
+
+def _astype(self, dtype):
+    """``jax.Array.astype``: a converted copy, dtype read as JAX does."""
+    return self.to(jax_dtype(dtype))
+
+
+# jax.Array.astype on torch tensors (reference model code calls it on state
+# arrays, e.g. examples/models/advanced_economic_model.py:1493); torch has no
+# method of that name, so nothing of torch's is replaced
+if not hasattr(torch.Tensor, "astype"):
+    torch.Tensor.astype = _astype
+
+
 def asarray(a, dtype=None) -> torch.Tensor:
     """Array from a tensor, numpy array, Python scalar or (nested) sequence."""
+    if dtype is not None:
+        dtype = jax_dtype(dtype)
     if getattr(a, "_abm_sym", False) is True:
         return a.as_array() if dtype is None else a.as_array().to(dtype)
     tr = _trace()

@@ -205,3 +205,21 @@ def test_bool_subtraction_follows_jax():
     assert outs["y"].dtype == torch.float32 and outs["w"].dtype == torch.int64
     with pytest.raises(fuse.Unsupported):
         _trace(lambda s, m_, k: {"y": -s["alive"]}, _states(), {"env": {}})
+
+
+def test_astype_reads_dtypes_as_jax_x32():
+    """jax.Array.astype on state tensors and traced values: Python int /
+    float / bool and 64-bit numpy dtypes read as JAX reads them with x64 off
+    (examples/models/advanced_economic_model.py:1493,
+    examples/sensitivity/sensitivity_calibration_example.py:137-143)."""
+    t = torch.tensor([1.7, -2.5])
+    assert t.astype(int).tolist() == [1, -2] and t.astype(int).dtype == torch.int32
+    assert t.astype(float).dtype == torch.float32 and t.astype(np.float64).dtype == torch.float32
+    assert torch.tensor([True, False]).astype(float).tolist() == [1.0, 0.0]
+    assert torch.func.vmap(lambda x: x.astype(int))(t).dtype == torch.int32
+    name, src, tr, outs = _trace(lambda s, m, k: {"i": s["x"].astype(int),
+                                                  "f": s["alive"].astype(float),
+                                                  "v": s["p"].astype(np.int64)},
+                                 {**_states(), "p": torch.rand(N, 2)}, {"env": {}})
+    assert {k: (v.dtype, tuple(v.shape)) for k, v in outs.items()} == {
+        "i": (torch.int32, (N,)), "f": (torch.float32, (N,)), "v": (torch.int32, (N, 2))}
