@@ -455,6 +455,10 @@ class SymVec:
             return SymVec(self.tr, self.items[i])
         return self.items[i]
 
+    @property
+    def at(self):
+        return _SymVecAt(self)
+
     def to(self, *args, **kwargs):
         return SymVec(self.tr, [x.to(*args, **kwargs) for x in self.items])
 
@@ -538,6 +542,42 @@ class SymVec:
 
 
 # ---------------------------------------------------------------- the trace
+class _SymVecAt:
+    """``v.at[j]`` on a per-agent vector with a constant index: ``set``,
+    ``add``, ``multiply`` and ``get``, as torch's index updates compute them
+    (the value is converted to the vector's dtype first)."""
+
+    def __init__(self, v: "SymVec", j=None):
+        self.v, self.j = v, j
+
+    def __getitem__(self, j):
+        if isinstance(j, bool) or not isinstance(j, int) or not -len(self.v) <= j < len(self.v):
+            raise Unsupported("a vector update at other than a constant in-range index")
+        return _SymVecAt(self.v, j % len(self.v))
+
+    def _update(self, fn):
+        items = list(self.v.items)
+        items[self.j] = fn(items[self.j])
+        return SymVec(self.v.tr, items)
+
+    def get(self):
+        return self.v.items[self.j]
+
+    def set(self, value):
+        tr = self.v.tr
+        return self._update(lambda e: tr.cast(tr.operand(value), e.kind))
+
+    def add(self, value):
+        tr = self.v.tr
+        return self._update(lambda e: tr.binary("add", e, tr.cast(tr.operand(value), e.kind)))
+
+    def multiply(self, value):
+        tr = self.v.tr
+        return self._update(lambda e: tr.binary("mul", e, tr.cast(tr.operand(value), e.kind)))
+
+    mul = multiply
+
+
 class _Trace:
     def __init__(self, n: int, mode: int, allow_host: bool = False):
         self.n = n

@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import builtins
 import math
+import warnings
 
 import numpy as _np
 import torch
@@ -515,13 +516,101 @@ def at_set(x, index, value):
 
 
 def at_add(x, index, value):
-    """``x.at[index].add(value)`` (out of place, repeated indices accumulate)."""
+    """``x.at[index].add(value)`` (out of place): the value is converted to
+    x's dtype; repeated integer indices accumulate."""
     y = _t(x).clone()
-    idx = index if not isinstance(index, torch.Tensor) else index.long()
-    y.index_put_((idx,) if not isinstance(idx, tuple) else idx,
-                  torch.as_tensor(value, dtype=y.dtype, device=y.device).expand_as(y[idx]),
-                  accumulate=True)
+    v = value.to(y.dtype) if isinstance(value, torch.Tensor) else \
+        torch.as_tensor(value, dtype=y.dtype, device=y.device)
+    if isinstance(index, list):
+        index = torch.as_tensor(index, device=y.device)
+    parts = index if isinstance(index, tuple) else (index,)
+    if builtins.all(_int_index(i) for i in parts):
+        idx = tuple(i.long() for i in parts)
+        y.index_put_(idx, v.expand_as(y[idx]), accumulate=True)
+        return y
+    y[index] = y[index] + v
     return y
+
+
+def _int_index(index) -> bool:
+    return isinstance(index, torch.Tensor) and not index.is_floating_point() and \
+        index.dtype != torch.bool
+
+
+class _AtRef:
+    """``x.at[index]``: out-of-place updates with JAX's meaning
+    (jax.numpy.ndarray.at).  Repeated integer indices accumulate in ``add``,
+    ``multiply``, ``min`` and ``max``; ``set`` keeps one of the values."""
+
+    __slots__ = ("x", "index")
+
+    def __init__(self, x, index):
+        self.x, self.index = x, index
+
+    def get(self):
+        return self.x[self.index]
+
+    def set(self, value):
+        return at_set(self.x, self.index, value)
+
+    def add(self, value):
+        return at_add(self.x, self.index, value)
+
+    def _reduce(self, value, how: str, elementwise):
+        y = self.x.clone()
+        v = value.to(y.dtype) if isinstance(value, torch.Tensor) else \
+            torch.as_tensor(value, dtype=y.dtype, device=y.device)
+        if _int_index(self.index) and self.index.dim() == 1:
+            idx = self.index.long()
+            idx = torch.where(idx < 0, idx + y.shape[0], idx)
+            src = v.expand((idx.shape[0],) + tuple(y.shape[1:]))
+            with warnings.catch_warnings():   # torch marks index_reduce as beta
+                warnings.simplefilter("ignore", UserWarning)
+                return y.index_reduce_(0, idx, src.contiguous(), how, include_self=True)
+        y[self.index] = elementwise(y[self.index], v)
+        return y
+
+    def multiply(self, value):
+        return self._reduce(value, "prod", torch.mul)
+
+    mul = multiply
+
+    def divide(self, value):
+        """x / (product of the values at each index), as JAX computes it."""
+        ones = torch.ones_like(self.x)
+        den = _AtRef(ones, self.index).multiply(value)
+        return self.x / den
+
+    def power(self, value):
+        y = self.x.clone()
+        y[self.index] = y[self.index] ** value
+        return y
+
+    def min(self, value):
+        return self._reduce(value, "amin", torch.minimum)
+
+    def max(self, value):
+        return self._reduce(value, "amax", torch.maximum)
+
+    def apply(self, fn):
+        y = self.x.clone()
+        y[self.index] = fn(y[self.index])
+        return y
+
+
+class _AtIndexer:
+    __slots__ = ("x",)
+
+    def __init__(self, x):
+        self.x = x
+
+    def __getitem__(self, index):
+        return _AtRef(self.x, index)
+
+
+# jax.Array.at on torch tensors (torch has no attribute of that name)
+if not hasattr(torch.Tensor, "at"):
+    torch.Tensor.at = property(lambda self: _AtIndexer(self))
 
 
 class linalg:  # noqa: N801 -- mirrors jnp.linalg

@@ -223,3 +223,38 @@ def test_astype_reads_dtypes_as_jax_x32():
                                  {**_states(), "p": torch.rand(N, 2)}, {"env": {}})
     assert {k: (v.dtype, tuple(v.shape)) for k, v in outs.items()} == {
         "i": (torch.int32, (N,)), "f": (torch.float32, (N,)), "v": (torch.int32, (N, 2))}
+
+
+def test_at_updates_follow_jax():
+    """x.at[i].set/add/multiply/min/max/divide/get with JAX's meaning: out of
+    place, values converted to x's dtype, repeated integer indices
+    accumulating (numpy's ufunc.at is the same accumulation)."""
+    x = torch.arange(6, dtype=torch.float32)
+    i = [1, 1, 3, -1]
+    ti = torch.tensor(i)
+    ref = np.arange(6, dtype=np.float32)
+    for name, ufunc, v in (("add", np.add, 1.5), ("multiply", np.multiply, 2.0),
+                           ("min", np.minimum, -1.0), ("max", np.maximum, 10.0)):
+        want = ref.copy()
+        ufunc.at(want, i, np.float32(v))
+        np.testing.assert_array_equal(getattr(x.at[ti], name)(v).numpy(), want, err_msg=name)
+    np.testing.assert_array_equal(x.at[ti].divide(2.0).numpy(),
+                                  ref / np.array([1, 4, 1, 2, 1, 2], np.float32))
+    assert x.at[2].set(9.0).tolist() == [0, 1, 9, 3, 4, 5] and x.tolist() == list(range(6))
+    assert x.at[1:3].add(1.0).tolist() == [0, 2, 3, 3, 4, 5]
+    assert x.at[[1, 1]].add(1.0).tolist() == [0, 3, 2, 3, 4, 5]
+    assert x.at[4].get().item() == 4.0
+    assert torch.arange(3, dtype=torch.int32).at[0].add(0.7).tolist() == [0, 1, 2]
+    got = torch.func.vmap(lambda v: v.at[0].add(v[1] * 2))(torch.arange(6.).reshape(3, 2))
+    assert got.tolist() == [[2, 1], [8, 3], [14, 5]]
+
+    def upd(s, m, k):
+        return {"p": s["p"].at[0].set(s["p"][2] * 2).at[-1].add(1.5),
+                "q": s["q"].at[1].add(0.7).at[0].multiply(3)}
+
+    name, src, tr, outs = _trace(upd, {"p": torch.rand(N, 3),
+                                       "q": torch.randint(0, 5, (N, 2), dtype=torch.int32)},
+                                 {"env": {}})
+    assert {k: (v.dtype, tuple(v.shape)) for k, v in outs.items()} == {
+        "p": (torch.float32, (N, 3)), "q": (torch.int32, (N, 2))}
+    fuse.compile_source(src)

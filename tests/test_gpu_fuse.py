@@ -402,3 +402,48 @@ def test_vector_reductions_fused_equal_vmapped(J, k):
     for kk in a:
         assert a[kk].dtype == b[kk].dtype, kk
         np.testing.assert_array_equal(bits(a[kk]), bits(b[kk]), err_msg=kk)
+
+
+def test_vector_at_updates_and_bool_subtraction_fused_equal_vmapped(J):
+    """x.at[j].set / add / multiply on per-agent vectors, .astype, and the
+    subtraction of a boolean (JAX promotes it; the vmapped path runs it under
+    jnp.BoolArithmetic), fused and vmapped alike."""
+    import torch
+
+    from jaxabm_amd import fuse
+    from jaxabm_amd import numpy as jnp
+    from jaxabm_amd import random as jr
+
+    class Updater:
+        def init_state(self, cfg, key):
+            k1, k2 = jr.split(key)
+            return {"p": jr.uniform(k1, (3,), minval=-1.0, maxval=1.0),
+                    "q": jr.randint(k2, (2,), 0, 9), "on": jr.bernoulli(k2, 0.5)}
+
+        def update(self, s, ms, cfg, key):
+            p, q, on = s["p"], s["q"], s["on"]
+            p = p.at[0].set(p[2] * 2.0).at[-1].add(0.25).at[1].multiply(1.0 - on)
+            q = q.at[1].add((p[0] > 0).astype(int)).at[0].set(p[1] * 10.0)
+            return {"p": p, "q": q, "on": ~on, "f": 1.0 - on, "g": on - 2}
+
+    n = 5000
+    res = []
+    for fused in (False, True):
+        fuse.enable(fused)
+        try:
+            before = fuse.stats()["launches"]
+            m = J.JaxModel(config=J.ModelConfig(seed=4))
+            m.add_agent_collection("u", J.AgentCollection(Updater(), n))
+            m.initialize()
+            for _ in range(3):
+                m.step()
+            assert fuse.stats()["launches"] - before == (4 if fused else 0), fuse.refused()
+            res.append({k: v.clone() for k, v in m.agent_collections["u"].states.items()})
+        finally:
+            fuse.enable(True)
+    a, b = res
+    assert set(a) == set(b)
+    for k in a:
+        assert a[k].dtype == b[k].dtype and a[k].shape == b[k].shape, k
+        np.testing.assert_array_equal(bits(a[k]), bits(b[k]), err_msg=k)
+    assert a["f"].dtype == torch.float32 and a["g"].dtype == torch.int64
