@@ -8,8 +8,10 @@ jax_enable_x64 off), so such code runs with ``import jaxabm_amd.numpy as jnp``.
 Every function is a torch op, so per-agent functions run under
 ``torch.func.vmap`` exactly as under ``jax.vmap``.
 
-Functional updates: ``x.at[i].set(v)`` has no torch form; use
-:func:`at_set` / :func:`at_add` (out-of-place) instead.
+JAX behaviours torch lacks are added where torch has no name for them:
+``x.at[i].set(v)`` (and ``add``, ``multiply``, ...), ``x.astype(float)``,
+and, inside :class:`BoolArithmetic`, the promotion of a boolean operand of a
+subtraction.
 """
 from __future__ import annotations
 
