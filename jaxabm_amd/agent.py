@@ -62,7 +62,7 @@ def _tensor_dict(out, what: str) -> Dict[str, torch.Tensor]:
     return {k: as_state_value(v) for k, v in out.items()}
 
 
-_BOOL_SUB_FNS: set = set()   # agent functions that subtract booleans
+_JAX_SEMANTICS_FNS: set = set()   # agent functions that need jnp.JaxSemantics
 
 
 def _fn_id(fn):
@@ -74,23 +74,24 @@ def vmap_agents(fn: Callable, *args, n: int, ident: Any = None) -> Dict[str, tor
     """``vmap(fn)(*args)`` over the leading (agent) axis; outputs that do not
     depend on the agent are broadcast to n agents.
 
-    A function that subtracts booleans (JAX promotes them, torch refuses) is
-    run again under ``jnp.BoolArithmetic`` and remembered by ``ident`` (agent
-    functions are pure, as under ``jax.vmap``, so the second run is the
-    first's)."""
+    A function that torch refuses where JAX has a meaning -- a boolean
+    operand of a subtraction, or an agents array indexed by a per-agent
+    integer -- is run again under ``jnp.JaxSemantics`` and remembered by
+    ``ident`` (agent functions are pure, as under ``jax.vmap``, so the second
+    run is the first's)."""
     mapped = torch.func.vmap(lambda *a: _tensor_dict(fn(*a), "agent function"))
     key = ident if ident is not None else _fn_id(fn)
-    if key in _BOOL_SUB_FNS:
-        with jnp.BoolArithmetic():
+    if key in _JAX_SEMANTICS_FNS:
+        with jnp.JaxSemantics():
             out = mapped(*args)
     else:
         try:
             out = mapped(*args)
         except RuntimeError as e:
-            if jnp.BOOL_SUB_ERROR not in str(e):
+            if not any(m in str(e) for m in jnp.JAX_SEMANTICS_ERRORS):
                 raise
-            _BOOL_SUB_FNS.add(key)
-            with jnp.BoolArithmetic():
+            _JAX_SEMANTICS_FNS.add(key)
+            with jnp.JaxSemantics():
                 out = mapped(*args)
     return {k: (v if v.dim() and v.shape[0] == n else v.expand(n, *v.shape)).contiguous()
             for k, v in out.items()}

@@ -37,8 +37,13 @@ operand's sign.
 * Per-agent vectors of up to 16 values are unrolled.  float32 sums and means
   over 2 to 7 of them follow torch's summation order; integer sums are exact.
 
-Anything else is not traced.  That includes reading other agents' arrays,
-longer vectors and other reductions, Python control flow on a traced value
+* ``model_state['agents_<name>'][var][j]`` (another agent's value, j a
+  per-agent or constant integer) is a gather; out-of-range indices follow
+  JAX's clamp.
+
+Anything else is not traced.  That includes whole-array use of the agents'
+arrays (``jnp.mean(model_state['agents_a']['x'])``), longer vectors and
+other reductions, Python control flow on a traced value
 and host reads; ``Unsupported`` is raised, and the collection runs the
 vmapped path, which is remembered per function and input signature.
 """
@@ -542,6 +547,73 @@ class SymVec:
 
 
 # ---------------------------------------------------------------- the trace
+def _refuse_op(name: str):
+    def op(self, *a, **k):
+        raise Unsupported(f"{name} of a whole agents array inside a per-agent function")
+    return op
+
+
+class SymArray:
+    """A collection's whole state array, ``model_state['agents_<name>'][var]``
+    of shape (M,) or (M, k), read inside a per-agent function
+    (jaxabm/model.py:101-116 puts every collection's states there).  Indexing
+    it by a per-agent or constant integer is a gather with JAX's meaning: a
+    negative index counts from the end, and an index still out of range is
+    clamped to the nearest end (where torch's indexing would fault).  Any
+    other use of the array is not traced."""
+
+    _abm_sym = True
+    weak = False
+    varying = False
+
+    def __init__(self, tr: "_Trace", t: torch.Tensor):
+        self.tr, self.t = tr, t
+
+    @property
+    def shape(self):
+        return self.t.shape
+
+    @property
+    def dtype(self):
+        return self.t.dtype
+
+    def dim(self):
+        return self.t.dim()
+
+    def size(self, *a):
+        return self.t.size(*a)
+
+    def __len__(self):
+        return self.t.shape[0]
+
+    def as_array(self):
+        raise Unsupported("a whole agents array inside a per-agent function")
+
+    def __getitem__(self, i):
+        col = None
+        if isinstance(i, tuple):
+            if len(i) != 2 or self.t.dim() != 2 or isinstance(i[1], bool) or \
+                    not isinstance(i[1], int):
+                raise Unsupported("an agents-array index other than [j] or [j, constant]")
+            i, col = i
+        return self.tr.gather(self.t, i, col)
+
+    @classmethod
+    def __torch_function__(cls, func, types, args=(), kwargs=None):
+        raise Unsupported(f"{getattr(func, '__name__', func)} of a whole agents array")
+
+
+for _name in ("__add__", "__radd__", "__sub__", "__rsub__", "__mul__", "__rmul__",
+              "__truediv__", "__rtruediv__", "__floordiv__", "__rfloordiv__", "__mod__",
+              "__rmod__", "__pow__", "__rpow__", "__lt__", "__le__", "__gt__", "__ge__",
+              "__eq__", "__ne__", "__and__", "__rand__", "__or__", "__ror__", "__xor__",
+              "__rxor__", "__neg__", "__abs__", "__invert__", "__bool__", "__iter__",
+              "__float__", "__int__", "__index__", "sum", "mean", "max", "min", "to",
+              "astype", "float", "int", "long", "bool", "item", "tolist"):
+    setattr(SymArray, _name, _refuse_op(_name))
+SymArray.__hash__ = object.__hash__
+
+
 class _SymVecAt:
     """``v.at[j]`` on a per-agent vector with a constant index: ``set``,
     ``add``, ``multiply`` and ``get``, as torch's index updates compute them
@@ -1010,6 +1082,34 @@ class _Trace:
                       kind, x.varying)
         return out
 
+    def gather(self, t: torch.Tensor, i, col=None):
+        """t[i] (or t[i, col]) for an agents array t of shape (M,) or (M, k)."""
+        kind = _KIND.get(t.dtype)
+        if kind is None or not t.is_contiguous() or t.dim() not in (1, 2) or t.shape[0] == 0 or \
+                (t.dim() == 2 and not 0 < t.shape[1] <= _MAX_VEC):
+            raise Unsupported("a gather from an array that is not (M,) or (M, k <= 16)")
+        j = self.operand(i)
+        if j.kind not in ("i32", "i64", "pyi"):
+            raise Unsupported("a gather index that is not an int32/int64 value")
+        m = t.shape[0]
+        w = self.word("ptr", t)
+        idx = self.tmp("int64_t", f"abm_gather_index((int64_t)({j.name}), {m}LL)")
+        ct = _CT[kind]
+        base = f"((const {ct} *)a.w[{w}])"
+        if t.dim() == 1:
+            return Sym(self, self.tmp(ct, f"{base}[{idx}]"), kind, j.varying)
+        k = t.shape[1]
+
+        def elem(c):
+            return Sym(self, self.tmp(ct, f"{base}[{idx} * {k}LL + {c}]"), kind, j.varying)
+
+        if col is not None:
+            c = col + k if col < 0 else col
+            if not 0 <= c < k:
+                raise Unsupported("a gather column out of range")
+            return elem(c)
+        return SymVec(self, [elem(c) for c in range(k)])
+
     def nan_to_num(self, x, nan, posinf, neginf):
         """torch.nan_to_num / jnp.nan_to_num: NaN, +inf and -inf replaced by
         the Python values given (infinities default to the dtype's largest
@@ -1162,6 +1262,11 @@ template <typename T> __device__ __forceinline__ T abm_floordiv_fs(T a, T b) {
   }
   return f;
 }
+// a gather index as JAX reads it: negative counts from the end, then clamped
+__device__ __forceinline__ int64_t abm_gather_index(int64_t j, int64_t m) {
+  if (j < 0) j += m;
+  return j < 0 ? 0 : (j >= m ? m - 1 : j);
+}
 template <typename T> __device__ __forceinline__ bool abm_isnan(T a) { return a != a; }
 template <typename T> __device__ __forceinline__ T abm_lo2(T a, T b) { return b < a ? b : a; }
 template <typename T> __device__ __forceinline__ T abm_hi2(T a, T b) { return a < b ? b : a; }
@@ -1288,12 +1393,20 @@ def _wrap_model_state(tr: _Trace, model_state):
             return tr.uniform_vector(v)
         return v
 
+    def agents_array(v):
+        if isinstance(v, torch.Tensor) and (v.is_cuda or tr.allow_host) and v.dtype in _KIND \
+                and v.dim() in (1, 2):
+            return SymArray(tr, v)
+        return v
+
     out = {}
     for k, v in model_state.items():
         if k == "env" and isinstance(v, dict):
             out[k] = {ek: wrap(ev) for ek, ev in v.items()}
         elif k == "time_step":
             out[k] = wrap(v)
+        elif k.startswith("agents_") and isinstance(v, dict):
+            out[k] = {vk: agents_array(vv) for vk, vv in v.items()}
         else:
             out[k] = v
     return out

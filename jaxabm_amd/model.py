@@ -42,9 +42,10 @@ class Model:
         self._env_state: Dict[str, Any] = {}
         self._state: Optional[Dict[str, Any]] = None
         self._params = params or {}
-        # the callbacks see JAX's promotion of booleans in subtraction
-        self._update_state_fn = jnp.with_bool_arithmetic(update_state_fn)
-        self._metrics_fn = jnp.with_bool_arithmetic(metrics_fn)
+        # the callbacks see JAX's promotion of booleans in subtraction and
+        # JAX's gather indexing
+        self._update_state_fn = jnp.with_jax_semantics(update_state_fn)
+        self._metrics_fn = jnp.with_jax_semantics(metrics_fn)
         self._time_step = 0
         self._history: List[Dict[str, Any]] = []
         self._is_initialized = False

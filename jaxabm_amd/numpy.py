@@ -10,8 +10,8 @@ Every function is a torch op, so per-agent functions run under
 
 JAX behaviours torch lacks are added where torch has no name for them:
 ``x.at[i].set(v)`` (and ``add``, ``multiply``, ...), ``x.astype(float)``,
-and, inside :class:`BoolArithmetic`, the promotion of a boolean operand of a
-subtraction.
+and, inside :class:`JaxSemantics`, the promotion of a boolean operand of a
+subtraction and JAX's gather indexing by integer tensors.
 """
 from __future__ import annotations
 
@@ -212,23 +212,46 @@ def clip(x, a_min=None, a_max=None):
     return torch.clamp(_t(x), min=a_min, max=a_max)
 
 
-# ---------------------------------------------------------------- bool arithmetic
+# ---------------------------------------------------------------- JAX semantics
 _SUBTRACTIONS = frozenset({torch.sub, torch.subtract, torch.rsub, torch.Tensor.sub,
                            torch.Tensor.subtract, torch.Tensor.__sub__, torch.Tensor.__rsub__})
-BOOL_SUB_ERROR = "with a bool tensor is not supported"   # torch's message
+# torch's messages for what JaxSemantics does differently
+JAX_SEMANTICS_ERRORS = ("with a bool tensor is not supported",
+                        "calling .item() on a Tensor")
 
 
 def _is_bool(x) -> bool:
     return isinstance(x, torch.Tensor) and x.dtype == torch.bool
 
 
-class BoolArithmetic(torch.overrides.TorchFunctionMode):
-    """JAX promotes a boolean operand of a subtraction (``1.0 - employed``
-    in the reference's examples/models/advanced_economic_model.py:184,
-    ``x - mask``, ``mask - 1``); torch refuses any bool operand of ``-``.
-    Inside this mode such a bool tensor is first cast to torch's result type
-    of the pair.  bool - bool and the negation of a bool still raise, as
-    they do in JAX."""
+def _int_tensor(x) -> bool:
+    return isinstance(x, torch.Tensor) and not x.is_floating_point() and \
+        not x.is_complex() and x.dtype != torch.bool
+
+
+def _gather_rows(t: torch.Tensor, j: torch.Tensor) -> torch.Tensor:
+    """t[j] along dim 0 as JAX's gather reads j: negative counts from the
+    end, then clamped to [0, M-1]; a 0-d j selects one row without a host
+    read (so it works under vmap and in captured graphs)."""
+    m = t.shape[0]
+    j = j.long()
+    j = torch.where(j < 0, j + m, j).clamp(0, m - 1)
+    if j.dim() == 0:
+        return t.index_select(0, j.reshape(1)).squeeze(0)
+    return t[j]
+
+
+class JaxSemantics(torch.overrides.TorchFunctionMode):
+    """Two JAX behaviours torch lacks, for model code written against JAX:
+
+    * a boolean operand of a subtraction is promoted (``1.0 - employed`` in
+      the reference's examples/models/advanced_economic_model.py:184,
+      ``x - mask``, ``mask - 1``) -- it is cast to torch's result type of the
+      pair; bool - bool and -bool still raise, as they do in JAX;
+    * indexing dim 0 with an integer tensor (``others[j]``, ``others[j, c]``,
+      j per agent under vmap, where torch would read a 0-d j on the host)
+      gathers with JAX's index rule: negative counts from the end, out of
+      range clamps."""
 
     def __torch_function__(self, func, types, args=(), kwargs=None):
         kwargs = kwargs or {}
@@ -236,16 +259,29 @@ class BoolArithmetic(torch.overrides.TorchFunctionMode):
             dt = torch.result_type(args[0], args[1])
             if dt != torch.bool:
                 args = tuple(a.to(dt) if _is_bool(a) else a for a in args[:2]) + tuple(args[2:])
+        elif func is torch.Tensor.__getitem__ and len(args) == 2 and \
+                isinstance(args[0], torch.Tensor) and args[0].dim() >= 1:
+            t, idx = args
+            if _int_tensor(idx):
+                return _gather_rows(t, idx)
+            if isinstance(idx, tuple) and idx and _int_tensor(idx[0]) and \
+                    builtins.all(isinstance(i, (int, slice)) and not isinstance(i, bool)
+                                 for i in idx[1:]):
+                row = _gather_rows(t, idx[0])
+                rest = idx[1:]
+                if idx[0].dim() == 0:
+                    return row[rest]
+                return row[(slice(None),) * idx[0].dim() + rest]
         return func(*args, **kwargs)
 
 
-def with_bool_arithmetic(fn):
-    """``fn`` run inside :class:`BoolArithmetic` (None stays None)."""
+def with_jax_semantics(fn):
+    """``fn`` run inside :class:`JaxSemantics` (None stays None)."""
     if fn is None:
         return None
 
     def wrapped(*args, **kwargs):
-        with BoolArithmetic():
+        with JaxSemantics():
             return fn(*args, **kwargs)
 
     wrapped.__wrapped__ = fn

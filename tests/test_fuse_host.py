@@ -181,9 +181,9 @@ def test_gamma_family_and_nan_to_num_trace_and_compile():
 
 def test_bool_subtraction_follows_jax():
     """JAX promotes a boolean operand of a subtraction; torch refuses it.
-    The traced kernel and the vmapped path (jnp.BoolArithmetic) both give
+    The traced kernel and the vmapped path (jnp.JaxSemantics) both give
     JAX's values; bool - bool and -bool still raise, as in JAX."""
-    from jaxabm_amd.agent import _BOOL_SUB_FNS, vmap_agents
+    from jaxabm_amd.agent import _JAX_SEMANTICS_FNS, vmap_agents
 
     m = torch.tensor([True, False, True])
     x = torch.tensor([0.5, 1.5, 2.5])
@@ -194,8 +194,8 @@ def test_bool_subtraction_follows_jax():
     out = vmap_agents(fn, m, x, n=3)
     assert out["y"].tolist() == [0.0, 1.0, 0.0] and out["z"].tolist() == [-0.5, 1.5, 1.5]
     assert out["w"].dtype == torch.int64 and out["w"].tolist() == [-1, -2, -1]
-    assert fn.__code__ in _BOOL_SUB_FNS
-    with jnp.BoolArithmetic():
+    assert fn.__code__ in _JAX_SEMANTICS_FNS
+    with jnp.JaxSemantics():
         with pytest.raises(RuntimeError):
             m - m
         with pytest.raises(RuntimeError):
@@ -258,3 +258,58 @@ def test_at_updates_follow_jax():
     assert {k: (v.dtype, tuple(v.shape)) for k, v in outs.items()} == {
         "p": (torch.float32, (N, 3)), "q": (torch.int32, (N, 2))}
     fuse.compile_source(src)
+
+
+def test_gathers_from_agents_arrays_trace():
+    """model_state['agents_<name>'][var][j] with a per-agent or constant
+    integer j is a gather (jaxabm/model.py:101-116 puts every collection's
+    arrays in model_state); whole-array use is refused."""
+    ms = {"env": {}, "time_step": 0,
+          "agents_v": {"op": torch.rand(N), "pos": torch.rand(N, 2),
+                       "n": torch.randint(0, 3, (N,), dtype=torch.int32)}}
+
+    def upd(s, m, k):
+        a = m["agents_v"]
+        j = jr.randint(k, (), 0, len(a["op"]))
+        return {"x": jnp.where(jr.uniform(k) < 0.5, a["op"][j], s["x"]), "pos": a["pos"][j - N],
+                "c": a["pos"][0, -1] + a["n"][j]}
+
+    name, src, tr, outs = _trace(upd, _states(), ms)
+    assert {k: (v.dtype, tuple(v.shape)) for k, v in outs.items()} == {
+        "x": (torch.float32, (N,)), "pos": (torch.float32, (N, 2)), "c": (torch.float32, (N,))}
+    assert "abm_gather_index" in src
+    fuse.compile_source(src)
+    for bad in (lambda s, m, k: {"x": jnp.mean(m["agents_v"]["op"])},
+                lambda s, m, k: {"x": m["agents_v"]["op"] * 2},
+                lambda s, m, k: {"x": 2 * m["agents_v"]["op"]},
+                lambda s, m, k: {"x": m["agents_v"]["op"][s["x"]]}):
+        with pytest.raises(fuse.Unsupported):
+            _trace(bad, _states(), ms)
+
+
+def test_jax_gather_indexing_under_vmap():
+    """others[j] with a per-agent integer j under vmap (torch would read a
+    0-d j on the host, which vmap refuses): vmap_agents reruns the function
+    under jnp.JaxSemantics, which gathers with JAX's rule -- negative counts
+    from the end, out of range clamps.  Plain tensors follow the same rule
+    inside the mode."""
+    from jaxabm_amd.agent import _JAX_SEMANTICS_FNS, vmap_agents
+
+    others = torch.arange(10, dtype=torch.float32) * 2
+    pos = torch.arange(20, dtype=torch.float32).reshape(10, 2)
+    j = torch.tensor([0, 3, -1, -12, 15, 9], dtype=torch.int32)
+
+    def fn(jj):
+        return {"o": others[jj], "p": pos[jj], "c": pos[jj, -1], "k": others[jj] + pos[jj, 0]}
+
+    out = vmap_agents(fn, j, n=6)
+    jn = np.clip(np.where(j.numpy() < 0, j.numpy() + 10, j.numpy()), 0, 9)
+    np.testing.assert_array_equal(out["o"].numpy(), others.numpy()[jn])
+    np.testing.assert_array_equal(out["p"].numpy(), pos.numpy()[jn])
+    np.testing.assert_array_equal(out["c"].numpy(), pos.numpy()[jn, -1])
+    assert fn.__code__ in _JAX_SEMANTICS_FNS
+    with jnp.JaxSemantics():
+        np.testing.assert_array_equal(others[j].numpy(), others.numpy()[jn])
+        assert others[torch.tensor(-3)].item() == 14.0 and others[torch.tensor(40)].item() == 18.0
+        assert others[2].item() == 4.0 and others[1:3].tolist() == [2.0, 4.0]
+        assert others[others > 15].tolist() == [16.0, 18.0]   # masks are not indices

@@ -407,7 +407,7 @@ def test_vector_reductions_fused_equal_vmapped(J, k):
 def test_vector_at_updates_and_bool_subtraction_fused_equal_vmapped(J):
     """x.at[j].set / add / multiply on per-agent vectors, .astype, and the
     subtraction of a boolean (JAX promotes it; the vmapped path runs it under
-    jnp.BoolArithmetic), fused and vmapped alike."""
+    jnp.JaxSemantics), fused and vmapped alike."""
     import torch
 
     from jaxabm_amd import fuse
@@ -447,3 +447,89 @@ def test_vector_at_updates_and_bool_subtraction_fused_equal_vmapped(J):
         assert a[k].dtype == b[k].dtype and a[k].shape == b[k].shape, k
         np.testing.assert_array_equal(bits(a[k]), bits(b[k]), err_msg=k)
     assert a["f"].dtype == torch.float32 and a["g"].dtype == torch.int64
+
+
+def test_gathers_from_agents_arrays_fused_equal_vmapped(J):
+    """A voter-style update reading a random other agent's opinion and
+    position (model_state['agents_<name>'], jaxabm/model.py:101-116), with
+    negative and constant indices, fused and vmapped alike; two
+    collections read each other."""
+    import torch
+
+    from jaxabm_amd import fuse
+    from jaxabm_amd import numpy as jnp
+    from jaxabm_amd import random as jr
+
+    n_v, n_w = 3000, 700
+
+    class Voter:
+        def init_state(self, cfg, key):
+            k1, k2 = jr.split(key)
+            return {"op": jr.uniform(k1), "pos": jr.uniform(k2, (2,)),
+                    "tag": jr.randint(k2, (), 0, 4)}
+
+        def update(self, s, ms, cfg, key):
+            k1, k2, k3 = jr.split(key, 3)
+            a, w = ms["agents_v"], ms["agents_w"]
+            j = jr.randint(k1, (), 0, n_v)
+            other = a["op"][j]
+            near = jnp.abs(other - s["op"]) < ms["env"]["eps"]
+            op = jnp.where(near, s["op"] + 0.5 * (other - s["op"]), s["op"])
+            return {"op": op, "pos": (s["pos"] + a["pos"][j - n_v]) * 0.5,
+                    "tag": a["tag"][j] + w["k"][jr.randint(k2, (), 0, n_w)],
+                    "first": a["op"][0] + a["pos"][n_v - 1, -1]}
+
+    class Watcher:
+        def init_state(self, cfg, key):
+            return {"k": jr.randint(key, (), 0, 9), "seen": 0.0}
+
+        def update(self, s, ms, cfg, key):
+            return {"k": s["k"], "seen": ms["agents_v"]["op"][jr.randint(key, (), 0, n_v)]}
+
+    res = []
+    for fused in (False, True):
+        fuse.enable(fused)
+        try:
+            before = fuse.stats()["launches"]
+            m = J.JaxModel(config=J.ModelConfig(seed=9))
+            m.add_agent_collection("v", J.AgentCollection(Voter(), n_v))
+            m.add_agent_collection("w", J.AgentCollection(Watcher(), n_w))
+            m.add_env_state("eps", 0.3)
+            m.initialize()
+            for _ in range(6):
+                m.step()
+            assert fuse.stats()["launches"] - before == (14 if fused else 0), fuse.refused()
+            res.append({f"{c}.{k}": v.clone() for c in ("v", "w")
+                        for k, v in m.agent_collections[c].states.items()})
+        finally:
+            fuse.enable(True)
+    a, b = res
+    assert set(a) == set(b)
+    for k in a:
+        assert a[k].dtype == b[k].dtype and a[k].shape == b[k].shape, k
+        np.testing.assert_array_equal(bits(a[k]), bits(b[k]), err_msg=k)
+
+
+def test_gather_out_of_range_clamps_as_jax(J):
+    """Out-of-range gather indices: negative ones count from the end, the
+    rest clamp to the nearest end (JAX's gather).  Run through the fused
+    path only (torch's indexing would fault on them)."""
+    import torch
+
+    from jaxabm_amd import fuse
+    from jaxabm_amd import random as jr
+
+    n, m = 4000, 37
+    src = torch.arange(m, dtype=torch.float32, device="cuda") * 1.5
+    key = jr.PRNGKey(12)
+
+    def fn(s, ms, k):
+        return {"y": ms["agents_o"]["x"][jr.randint(k, (), -3 * m, 3 * m)]}
+
+    out = fuse.run_agents(fn, n, key, jr.default_mode(), None,
+                          {"env": {}, "agents_o": {"x": src}}, fn)
+    assert out is not None, fuse.refused()
+    j = jr.randint_each(jr.split(key, n), (), -3 * m, 3 * m).cpu().numpy().astype(np.int64)
+    jj = np.where(j < 0, j + m, j).clip(0, m - 1)
+    np.testing.assert_array_equal(out["y"].cpu().numpy(), src.cpu().numpy()[jj])
+    assert (j < -m).any() and (j >= m).any()
