@@ -38,12 +38,13 @@ operand's sign.
   over 2 to 7 of them follow torch's summation order; integer sums are exact.
 
 * ``model_state['agents_<name>'][var][j]`` (another agent's value, j a
-  per-agent or constant integer) is a gather; out-of-range indices follow
-  JAX's clamp.
+  per-agent integer) is a gather; out-of-range indices follow JAX's clamp.
+  Whole-array use of those arrays (``jnp.mean(...)``, a constant index)
+  gives the same value to every agent and runs as torch ops before the
+  kernel, as it does once under vmap.
 
-Anything else is not traced.  That includes whole-array use of the agents'
-arrays (``jnp.mean(model_state['agents_a']['x'])``), longer vectors and
-other reductions, Python control flow on a traced value
+Anything else is not traced.  That includes longer vectors and other
+reductions, Python control flow on a traced value
 and host reads; ``Unsupported`` is raised, and the collection runs the
 vmapped path, which is remembered per function and input signature.
 """
@@ -547,20 +548,28 @@ class SymVec:
 
 
 # ---------------------------------------------------------------- the trace
-def _refuse_op(name: str):
-    def op(self, *a, **k):
-        raise Unsupported(f"{name} of a whole agents array inside a per-agent function")
-    return op
+def _unwrap(x):
+    if isinstance(x, SymArray):
+        return x.t
+    if isinstance(x, (list, tuple)):
+        return type(x)(_unwrap(v) for v in x)
+    if isinstance(x, dict):
+        return {k: _unwrap(v) for k, v in x.items()}
+    return x
 
 
 class SymArray:
     """A collection's whole state array, ``model_state['agents_<name>'][var]``
     of shape (M,) or (M, k), read inside a per-agent function
-    (jaxabm/model.py:101-116 puts every collection's states there).  Indexing
-    it by a per-agent or constant integer is a gather with JAX's meaning: a
-    negative index counts from the end, and an index still out of range is
-    clamped to the nearest end (where torch's indexing would fault).  Any
-    other use of the array is not traced."""
+    (jaxabm/model.py:101-116 puts every collection's states there).
+
+    * Indexing it by a traced (per-agent) integer is a gather with JAX's
+      meaning: a negative index counts from the end, and an index still out of
+      range is clamped to the nearest end.
+    * Anything else -- a constant index, ``jnp.mean(arr)``, ``arr > 0.5`` --
+      is the same for every agent, as under vmap where the array is not
+      batched: it runs as the torch op on the real array, before the kernel,
+      and its result enters the kernel as a uniform value."""
 
     _abm_sym = True
     weak = False
@@ -569,48 +578,50 @@ class SymArray:
     def __init__(self, tr: "_Trace", t: torch.Tensor):
         self.tr, self.t = tr, t
 
-    @property
-    def shape(self):
-        return self.t.shape
-
-    @property
-    def dtype(self):
-        return self.t.dtype
-
-    def dim(self):
-        return self.t.dim()
-
-    def size(self, *a):
-        return self.t.size(*a)
-
-    def __len__(self):
-        return self.t.shape[0]
-
     def as_array(self):
-        raise Unsupported("a whole agents array inside a per-agent function")
+        return self.t
 
     def __getitem__(self, i):
+        first = i[0] if isinstance(i, tuple) and i else i
+        if not isinstance(first, (Sym, SymVec)):
+            return self.t[_unwrap(i)]
         col = None
         if isinstance(i, tuple):
             if len(i) != 2 or self.t.dim() != 2 or isinstance(i[1], bool) or \
                     not isinstance(i[1], int):
                 raise Unsupported("an agents-array index other than [j] or [j, constant]")
             i, col = i
+        if isinstance(i, SymVec):
+            return SymVec(self.tr, [self.tr.gather(self.t, v, col) for v in i.items])
         return self.tr.gather(self.t, i, col)
+
+    def __getattr__(self, name):
+        return getattr(self.t, name)
+
+    def __len__(self):
+        return len(self.t)
+
+    def __iter__(self):
+        return iter(self.t)
 
     @classmethod
     def __torch_function__(cls, func, types, args=(), kwargs=None):
-        raise Unsupported(f"{getattr(func, '__name__', func)} of a whole agents array")
+        return func(*_unwrap(args), **_unwrap(kwargs or {}))
+
+
+def _delegate(name: str):
+    def op(self, *a):
+        return getattr(self.t, name)(*_unwrap(a))
+    return op
 
 
 for _name in ("__add__", "__radd__", "__sub__", "__rsub__", "__mul__", "__rmul__",
               "__truediv__", "__rtruediv__", "__floordiv__", "__rfloordiv__", "__mod__",
               "__rmod__", "__pow__", "__rpow__", "__lt__", "__le__", "__gt__", "__ge__",
               "__eq__", "__ne__", "__and__", "__rand__", "__or__", "__ror__", "__xor__",
-              "__rxor__", "__neg__", "__abs__", "__invert__", "__bool__", "__iter__",
-              "__float__", "__int__", "__index__", "sum", "mean", "max", "min", "to",
-              "astype", "float", "int", "long", "bool", "item", "tolist"):
-    setattr(SymArray, _name, _refuse_op(_name))
+              "__rxor__", "__neg__", "__abs__", "__invert__", "__bool__", "__float__",
+              "__int__", "__index__"):
+    setattr(SymArray, _name, _delegate(_name))
 SymArray.__hash__ = object.__hash__
 
 

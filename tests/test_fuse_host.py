@@ -261,9 +261,11 @@ def test_at_updates_follow_jax():
 
 
 def test_gathers_from_agents_arrays_trace():
-    """model_state['agents_<name>'][var][j] with a per-agent or constant
-    integer j is a gather (jaxabm/model.py:101-116 puts every collection's
-    arrays in model_state); whole-array use is refused."""
+    """model_state['agents_<name>'][var][j] with a per-agent integer j is a
+    gather (jaxabm/model.py:101-116 puts every collection's arrays in
+    model_state); whole-array uses are uniform values computed by torch
+    before the kernel; an agents array combined with a per-agent value, or
+    indexed by a float, is refused."""
     ms = {"env": {}, "time_step": 0,
           "agents_v": {"op": torch.rand(N), "pos": torch.rand(N, 2),
                        "n": torch.randint(0, 3, (N,), dtype=torch.int32)}}
@@ -279,9 +281,15 @@ def test_gathers_from_agents_arrays_trace():
         "x": (torch.float32, (N,)), "pos": (torch.float32, (N, 2)), "c": (torch.float32, (N,))}
     assert "abm_gather_index" in src
     fuse.compile_source(src)
-    for bad in (lambda s, m, k: {"x": jnp.mean(m["agents_v"]["op"])},
-                lambda s, m, k: {"x": m["agents_v"]["op"] * 2},
-                lambda s, m, k: {"x": 2 * m["agents_v"]["op"]},
+    def uniform(s, m, k):
+        a = m["agents_v"]
+        return {"x": s["x"] - jnp.mean(a["op"]) + jnp.sum(a["op"] > 0.5) + a["op"].max() +
+                (2 * a["op"])[1] + a["pos"][0, 1]}
+
+    name, src, tr, outs = _trace(uniform, _states(), ms)
+    assert outs["x"].dtype == torch.float32
+    fuse.compile_source(src)
+    for bad in (lambda s, m, k: {"x": s["x"] + m["agents_v"]["op"]},
                 lambda s, m, k: {"x": m["agents_v"]["op"][s["x"]]}):
         with pytest.raises(fuse.Unsupported):
             _trace(bad, _states(), ms)

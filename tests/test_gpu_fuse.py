@@ -452,8 +452,8 @@ def test_vector_at_updates_and_bool_subtraction_fused_equal_vmapped(J):
 def test_gathers_from_agents_arrays_fused_equal_vmapped(J):
     """A voter-style update reading a random other agent's opinion and
     position (model_state['agents_<name>'], jaxabm/model.py:101-116), with
-    negative and constant indices, fused and vmapped alike; two
-    collections read each other."""
+    negative and constant indices and whole-array means and counts, fused
+    and vmapped alike; two collections read each other."""
     import torch
 
     from jaxabm_amd import fuse
@@ -475,9 +475,11 @@ def test_gathers_from_agents_arrays_fused_equal_vmapped(J):
             other = a["op"][j]
             near = jnp.abs(other - s["op"]) < ms["env"]["eps"]
             op = jnp.where(near, s["op"] + 0.5 * (other - s["op"]), s["op"])
-            return {"op": op, "pos": (s["pos"] + a["pos"][j - n_v]) * 0.5,
+            mean_op = jnp.mean(a["op"])            # the same for every agent
+            pull = jnp.where(jr.uniform(k3) < 0.1, mean_op - op, 0.0)
+            return {"op": op + pull, "pos": (s["pos"] + a["pos"][j - n_v]) * 0.5,
                     "tag": a["tag"][j] + w["k"][jr.randint(k2, (), 0, n_w)],
-                    "first": a["op"][0] + a["pos"][n_v - 1, -1]}
+                    "first": a["op"][0] + a["pos"][n_v - 1, -1] + jnp.sum(w["k"] > 4)}
 
     class Watcher:
         def init_state(self, cfg, key):
