@@ -392,3 +392,88 @@ def test_analysis_clients(J):
     assert set(best) == {"eps", "mu"}
     with pytest.raises(NotImplementedError):
         ModelCalibrator(dfactory, {"eps": 0.3}, {"opinion_var": 0.02}, method="dqn")
+
+
+def test_population_model_with_jax_idioms(J, capsys):
+    """An AgentPy-style population model in the style of the reference's
+    examples/sensitivity/sensitivity_calibration_example.py:15-175: agent
+    setups with vector positions, env-phase dynamics with
+    jnp.round(...).astype(int), jnp.maximum/minimum, env.add_state and
+    attribute reads, records, and compute_metrics with len() and jnp.mean.
+    Records equal a numpy restatement with the same dtype flow (float32
+    products of int32 counts)."""
+    from jaxabm_amd import fuse
+    from jaxabm_amd import numpy as jnp
+
+    class Prey(J.Agent):
+        def setup(self):
+            return {"alive": True, "age": 0, "energy": 1.0, "position": jnp.array([0.0, 0.0])}
+
+        def step(self, model_state):
+            gain = model_state.get("params", {}).get("prey_energy_gain", 0.13)
+            return {"alive": self._state["alive"], "age": self._state["age"] + 1,
+                    "energy": self._state["energy"] + gain, "position": self._state["position"]}
+
+    class Predator(J.Agent):
+        def setup(self):
+            return {"alive": True, "age": 0, "energy": 1.0, "position": jnp.array([0.0, 0.0])}
+
+        def step(self, model_state):
+            return {"alive": self._state["alive"], "age": self._state["age"],
+                    "energy": self._state["energy"] - 0.07, "position": self._state["position"]}
+
+    class Populations(J.Model):
+        def setup(self):
+            self.prey = self.add_agents(self.p.get("n_prey", 100), Prey)
+            self.predators = self.add_agents(self.p.get("n_predators", 20), Predator)
+            self.env.add_state("time", 0)
+            self.env.add_state("prey_count", self.p.get("n_prey", 100))
+            self.env.add_state("predator_count", self.p.get("n_predators", 20))
+
+        def step(self):
+            self.env.add_state("time", self.env.time + 1)
+            birth, death = 0.13 * 0.5, 0.07 * 0.8
+            prey, pred = self.env.prey_count, self.env.predator_count
+            pb = jnp.round(prey * birth * 0.1).astype(int)
+            pd = jnp.round(prey * pred * 0.001).astype(int)
+            new_prey = jnp.maximum(1, prey + pb - pd)
+            qb = jnp.round(pred * pd * 0.1).astype(int)
+            qd = jnp.round(pred * death * 0.2).astype(int)
+            new_pred = jnp.maximum(1, pred + qb - qd)
+            self.env.add_state("prey_count", jnp.minimum(new_prey, 500))
+            self.env.add_state("predator_count", jnp.minimum(new_pred, 100))
+            self.record("prey_count", self.env.prey_count)
+            self.record("predator_count", self.env.predator_count)
+
+        def compute_metrics(self, env_state, agent_states, params):
+            # add_agents names a collection class name + "s" (jaxabm/agentpy.py:960-961)
+            e = agent_states.get("preys", {}).get("energy", jnp.array([]))
+            return {"mean_prey_energy": jnp.mean(e) if len(e) > 0 else jnp.array(0.0)}
+
+    before = fuse.stats()["launches"]
+    res = Populations({"steps": 30, "n_prey": 100, "n_predators": 20, "seed": 2}).run()
+    assert fuse.stats()["launches"] - before == 2 + 2 * 30, fuse.refused()   # setups + steps
+
+    def f32(v):
+        return np.float32(v)
+
+    birth, death = 0.13 * 0.5, 0.07 * 0.8
+    prey, pred, want_prey, want_pred = 100, 20, [], []
+    for t in range(30):
+        if t == 0:   # Python ints times Python floats, then a float32 array
+            pb, pd = np.round(f32(prey * birth * 0.1)), np.round(f32(prey * pred * 0.001))
+            qd = np.round(f32(pred * death * 0.2))
+        else:        # int32 counts times Python floats: float32 products
+            pb = np.round(f32(prey) * f32(birth) * f32(0.1))
+            pd = np.round(f32(prey * pred) * f32(0.001))
+            qd = np.round(f32(pred) * f32(death) * f32(0.2))
+        pb, pd, qd = int(pb), int(pd), int(qd)
+        new_prey = max(1, prey + pb - pd)
+        qb = int(np.round(f32(pred * pd) * f32(0.1)))    # pd is an int32 array from step 0 on
+        prey, pred = min(new_prey, 500), min(max(1, pred + qb - qd), 100)
+        want_prey.append(prey)
+        want_pred.append(pred)
+    assert [int(v) for v in res["prey_count"]] == want_prey
+    assert [int(v) for v in res["predator_count"]] == want_pred
+    np.testing.assert_allclose([float(v) for v in res["mean_prey_energy"]],
+                               1.0 + 0.13 * np.arange(1, 31), rtol=1e-5)
