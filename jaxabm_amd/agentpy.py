@@ -429,6 +429,7 @@ class Results:
                     return len(self._values)
 
     def __init__(self, data: Dict[str, Any]):
+        self._raw = data     # the device values as recorded (gradients flow through them)
         self._data = convert_to_numpy(data)
         self.variables = self.VariableContainer(self._data)
 

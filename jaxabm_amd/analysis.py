@@ -12,8 +12,10 @@
   at seed 42, jaxabm/analysis.py:489-531 and 1309-1394), "es", "pso" and
   "cem".  The reference differentiates the whole simulation with
   ``jax.grad``; here a model that provides ``loss_gradient`` (the Deffuant
-  program's hand-written adjoint) supplies the exact gradient, any other
-  model a central finite difference of its runs.  The reinforcement-learning
+  program's hand-written adjoint) supplies the exact gradient, a generic
+  model torch autograd through its eager run (parameters as float32
+  tensors), and a model whose run reads a parameter on the host a central
+  finite difference of its runs.  The reinforcement-learning
   calibrators of the reference are not part of this engine.
 """
 from __future__ import annotations
@@ -39,6 +41,27 @@ def _last(results, metric):
     if hasattr(v, "__len__") and not isinstance(v, str):
         v = v[-1]
     return float(v)
+
+
+class _Concrete(Exception):
+    """A host read of a tensor that carries a gradient."""
+
+
+class _NoConcretization(torch.overrides.TorchFunctionMode):
+    """Raises on a host read (float(), int(), bool(), .item(), .tolist(),
+    .numpy()) of a tensor that requires grad, as JAX refuses to concretize a
+    traced value under jax.grad: a value read on the host is cut from the
+    gradient, so the autograd gradient would be silently partial."""
+
+    _READS = frozenset({torch.Tensor.__float__, torch.Tensor.__int__, torch.Tensor.__bool__,
+                        torch.Tensor.__index__, torch.Tensor.item, torch.Tensor.tolist,
+                        torch.Tensor.numpy})
+
+    def __torch_function__(self, func, types, args=(), kwargs=None):
+        if func in self._READS and args and isinstance(args[0], torch.Tensor) and \
+                args[0].requires_grad:
+            raise _Concrete(getattr(func, "__name__", str(func)))
+        return func(*args, **(kwargs or {}))
 
 
 def _servable(model) -> bool:
@@ -210,6 +233,7 @@ class ModelCalibrator:
         self.best_params = dict(initial_params)
         self.best_loss = float("inf")
         self._names = list(self.params.keys())
+        self.last_gradient: Optional[str] = None
 
     # -- losses (jaxabm/analysis.py:401-487)
     def _compute_loss(self, metrics: Dict[str, float]) -> float:
@@ -291,14 +315,79 @@ class ModelCalibrator:
         """Loss of one run at the fixed gradient seed 42."""
         return self._compute_loss(self._metrics(params, 42))
 
+    def _loss_tensor(self, metrics: Dict[str, torch.Tensor]):
+        """_compute_loss on 0-d tensors, differentiably."""
+        loss = None
+        for m, target in self.target_metrics.items():
+            if m not in metrics:
+                continue
+            d, w = metrics[m] - target, self.metrics_weights.get(m, 1.0)
+            if self.loss_type == "mse":
+                ml = d * d
+            elif self.loss_type == "mae":
+                ml = d.abs()
+            elif self.loss_type == "huber":
+                r = d.abs()
+                ml = torch.where(r <= 1.0, 0.5 * r * r, r - 0.5)
+            elif self.loss_type == "relative":
+                ml = d.abs() / (abs(target) + 1e-8)
+            else:
+                raise ValueError(f"Unknown loss type: {self.loss_type}")
+            loss = w * ml if loss is None else loss + w * ml
+        return loss
+
+    def _autograd_grad(self, params: Dict[str, float]) -> Optional[np.ndarray]:
+        """d loss / d params by torch autograd through one eager run at seed
+        42, the counterpart of the reference's jax.grad of the unrolled run
+        (jaxabm/analysis.py:489-531).  The parameters enter the model as
+        float32 tensors.  None -- and the caller differentiates numerically
+        -- when the run does not keep them as tensors all the way to the
+        target metrics: a host read of a parameter (float(), .item(), a
+        branch on it; JAX would refuse it too), a metric that is not a
+        tensor, or a step that fails on tensor parameters."""
+        dev = jr._device()
+        pt = {k: torch.tensor(float(v), dtype=torch.float32, device=dev, requires_grad=True)
+              for k, v in params.items()}
+        try:
+            with torch.enable_grad(), _NoConcretization():
+                model = self.model_factory(params=pt, config=ModelConfig(seed=42))
+                if _servable(model):
+                    return None
+                res = model.run(steps=self.evaluation_steps)
+                data = getattr(res, "_raw", res)
+                metrics = {}
+                for m in self.target_metrics:
+                    v = data.get(m) if isinstance(data, dict) else None
+                    if v is None:
+                        continue
+                    last = v[-1] if isinstance(v, (list, tuple)) else v
+                    if not isinstance(last, torch.Tensor) or last.numel() != 1:
+                        return None
+                    metrics[m] = last.float().reshape(())
+                loss = self._loss_tensor(metrics)
+                if loss is None or not loss.requires_grad:
+                    return None
+                grads = torch.autograd.grad(loss, [pt[k] for k in self._names], allow_unused=True)
+        except (_Concrete, TypeError, RuntimeError, ValueError, AttributeError, KeyError):
+            return None
+        return np.array([0.0 if g is None else float(g) for g in grads], np.float64)
+
     def grad_fn(self, params: Dict[str, float]) -> np.ndarray:
         """d loss / d params at seed 42: the model's own adjoint when it has
-        ``loss_gradient``, else central differences."""
+        ``loss_gradient``, else torch autograd through the run, else central
+        differences (``last_gradient`` says which)."""
         model = self.model_factory(params=params, config=ModelConfig(seed=42))
         if hasattr(model, "loss_gradient"):
+            self.last_gradient = "adjoint"
             g = model.loss_gradient(self.target_metrics, self.metrics_weights,
                                     steps=self.evaluation_steps)
             return np.array([g[k] for k in self._names], np.float64)
+        if not _servable(model):
+            g = self._autograd_grad(params)
+            if g is not None:
+                self.last_gradient = "autograd"
+                return g
+        self.last_gradient = "finite_difference"
         g = np.zeros(len(self._names))
         hs, runs = [], []
         for k in self._names:

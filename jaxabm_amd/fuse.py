@@ -1488,6 +1488,19 @@ def trace(n: int, key: torch.Tensor, mode: int, states: Optional[Dict[str, Any]]
     return name, _source(tr, stores, name), tr, outs
 
 
+def _needs_grad(*trees) -> bool:
+    def any_grad(v):
+        if isinstance(v, torch.Tensor):
+            return v.requires_grad
+        if isinstance(v, dict):
+            return any(any_grad(x) for x in v.values())
+        if isinstance(v, (list, tuple)):
+            return any(any_grad(x) for x in v)
+        return False
+
+    return any(any_grad(t) for t in trees)
+
+
 def run_agents(fn, n: int, key: torch.Tensor, mode: int, states: Optional[Dict[str, Any]],
                model_state, call) -> Optional[Dict[str, torch.Tensor]]:
     """Run ``call(states_sym, model_state_sym, agent_key_sym)`` for all n
@@ -1495,6 +1508,8 @@ def run_agents(fn, n: int, key: torch.Tensor, mode: int, states: Optional[Dict[s
     (remembered for the function and input signature)."""
     if not _ENABLED or not isinstance(key, torch.Tensor) or not key.is_cuda:
         return None
+    if torch.is_grad_enabled() and _needs_grad(states, model_state):
+        return None      # the generated kernel is not differentiable: run vmapped
     sig = _signature(fn, states, model_state)
     if sig in _refused:
         return None

@@ -321,3 +321,22 @@ def test_jax_gather_indexing_under_vmap():
         assert others[torch.tensor(-3)].item() == 14.0 and others[torch.tensor(40)].item() == 18.0
         assert others[2].item() == 4.0 and others[1:3].tolist() == [2.0, 4.0]
         assert others[others > 15].tolist() == [16.0, 18.0]   # masks are not indices
+
+
+def test_gradient_run_refuses_host_reads_of_parameters():
+    """Inside the calibrator's autograd run a host read of a tensor that
+    carries a gradient raises (JAX refuses to concretize under jax.grad), so
+    the calibrator differentiates numerically instead of returning a partial
+    gradient.  Reads of values without a gradient pass; a branch on a
+    comparison follows the taken branch, as torch autograd does."""
+    from jaxabm_amd.analysis import _Concrete, _NoConcretization
+
+    p = torch.tensor(0.5, requires_grad=True)
+    with _NoConcretization():
+        for read in (lambda: float(p), lambda: p.item(), lambda: p.tolist(),
+                     lambda: int(p * 2), lambda: float(p * 3.0)):
+            with pytest.raises(_Concrete):
+                read()
+        assert bool(p > 0) is True
+        assert float(torch.tensor(2.0)) == 2.0
+        assert (p * 2).detach().item() == 1.0
