@@ -289,6 +289,13 @@ class _Runner:
                 m._history.append({"time_step": t, "metrics": {k: v[i] for k, v in cols.items()}})
 
 
+def _needs_grad(model) -> bool:
+    from .fuse import _needs_grad as any_grad
+
+    states = {name: c.states for name, c in model.agent_collections.items()}
+    return any_grad(states, model._env_state, model._params)
+
+
 def run_graph(model, n: int, between=None) -> Dict[str, Any]:
     """Run up to ``n`` steps of ``model``: steps 1 and 2 eagerly, steps
     3..n as replays of one captured step.  Returns the run's info
@@ -298,6 +305,10 @@ def run_graph(model, n: int, between=None) -> Dict[str, Any]:
     ``steps_done`` calls of ``model.step()``; the caller runs the remaining
     steps eagerly.  ``between`` (tests) is called after each replay but the
     last."""
+    if torch.is_grad_enabled() and _needs_grad(model):
+        # a replayed graph is cut from autograd: differentiable runs stay eager
+        return {"mode": "eager", "steps": n, "steps_done": 0,
+                "reason": "a parameter, env value or state requires grad"}
     r = _Runner(model, n)
     originals = {name: c.states for name, c in r.colls}
     try:

@@ -227,3 +227,21 @@ def test_python_value_changing_between_steps_runs_eagerly_after_step_1(J):
     assert (ca["n"], cb["n"]) == (5, 6)
     assert (a._env_state.pop("label"), b._env_state.pop("label")) == ("odd", "even")
     assert_same_run(a, b, ra, rb)
+
+
+def test_differentiable_run_stays_eager(J, capsys):
+    """A run whose env value requires grad is not replayed (a graph would cut
+    it from autograd): it runs eagerly and the gradient reaches the value."""
+    import torch
+
+    from jaxabm_amd import numpy as jnp
+
+    inc = torch.tensor(0.5, device="cuda", requires_grad=True)
+    m = J.JaxModel(config=J.ModelConfig(seed=1),
+                   metrics_fn=lambda env, agents, p: {"m": jnp.mean(agents["d"]["value"])})
+    m.add_agent_collection("d", J.AgentCollection(Dummy(), 1000))
+    m.add_env_state("increment", inc)
+    out = m.run(6, graph=True)
+    assert m.last_run_info["mode"] == "eager" and "grad" in m.last_run_info["reason"]
+    (g,) = torch.autograd.grad(out["m"][-1], [inc])
+    assert float(g) == pytest.approx(6.0, rel=1e-6)
