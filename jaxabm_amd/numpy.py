@@ -234,7 +234,7 @@ def _gather_rows(t: torch.Tensor, j: torch.Tensor) -> torch.Tensor:
     end, then clamped to [0, M-1]; a 0-d j selects one row without a host
     read (so it works under vmap and in captured graphs)."""
     m = t.shape[0]
-    j = j.long()
+    j = j.to(device=t.device, dtype=torch.long)
     j = torch.where(j < 0, j + m, j).clamp(0, m - 1)
     if j.dim() == 0:
         return t.index_select(0, j.reshape(1)).squeeze(0)
@@ -563,7 +563,7 @@ def at_add(x, index, value):
         index = torch.as_tensor(index, device=y.device)
     parts = index if isinstance(index, tuple) else (index,)
     if builtins.all(_int_index(i) for i in parts):
-        idx = tuple(i.long() for i in parts)
+        idx = tuple(i.to(device=y.device, dtype=torch.long) for i in parts)
         y.index_put_(idx, v.expand_as(y[idx]), accumulate=True)
         return y
     y[index] = y[index] + v
@@ -599,7 +599,7 @@ class _AtRef:
         v = value.to(y.dtype) if isinstance(value, torch.Tensor) else \
             torch.as_tensor(value, dtype=y.dtype, device=y.device)
         if _int_index(self.index) and self.index.dim() == 1:
-            idx = self.index.long()
+            idx = self.index.to(device=y.device, dtype=torch.long)
             idx = torch.where(idx < 0, idx + y.shape[0], idx)
             src = v.expand((idx.shape[0],) + tuple(y.shape[1:]))
             with warnings.catch_warnings():   # torch marks index_reduce as beta
