@@ -101,6 +101,25 @@ def cpu_baseline(row_ptr, col, runs: int) -> dict:
     }
 
 
+def cpu_sample(fn, units: float, unit: str, sample: str, runs: int) -> dict:
+    """``fn`` (one bounded run of the C oracle, a restatement of the
+    reference semantics, single-threaded) timed as the median of ``runs``
+    runs after a warm-up; ``units`` of work per run."""
+    import numpy as np
+
+    times = []
+    for i in range(runs + 1):
+        t0 = time.perf_counter()
+        fn()
+        if i:
+            times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    return {"value": units / t, "unit": unit, "cores": 1, "kind": "port",
+            "sample": f"{sample}; C oracle, not JAX; median of {runs} runs after a warm-up; "
+                      f"host CPU: {cpu_model()}, nproc={os.cpu_count()}",
+            "run_s": t}
+
+
 PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
 
 
@@ -236,6 +255,7 @@ def bench_deffuant(args) -> dict:
                    "instances": B, "sim_steps": S, "ckpt_every": K,
                    "bench_step": "one forward + adjoint of the batch"},
         "forward_ms": t_fwd * 1e3, "adjoint_ms": (t_both - t_fwd) * 1e3,
+        "cpu_baseline": _deffuant_cpu(n, eps, mu, target, args.cpu_runs),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
@@ -244,6 +264,56 @@ def bench_deffuant(args) -> dict:
                      "algorithmic_bytes_per_launch": fwd_bytes, "avg_kernel_us": step_s * 1e6,
                      "copy_reference": copy_reference_gbs(stream)},
     }
+
+
+def _deffuant_cpu(n, eps, mu, target, runs):
+    if runs <= 0:
+        return None
+    import numpy as np
+
+    import oracle as O
+
+    b, steps, mode = 32, 10, O.MODE_PARTITIONABLE
+    x0, rng = O.deffuant_init(n, b, 0, mode)
+    e, m = eps[:b].astype(np.float64), mu[:b].astype(np.float64)
+    return cpu_sample(lambda: O.deffuant_grad(x0, e, m, rng, steps, target, mode), n * b * steps,
+                      "agent-instance-steps/s",
+                      f"forward + float64 reverse sweep of {n:,} agents x {b} of the 256 "
+                      f"(eps, mu) instances x {steps} steps (the per-step pairing is shared by "
+                      f"the instances, as in the measured batch)", min(runs, 3))
+
+
+def _wealth_cpu(n, steps, runs):
+    if runs <= 0:
+        return None
+    import oracle as O
+
+    mode = O.MODE_PARTITIONABLE
+
+    def run():
+        x, y, w, rng = O.wealth_init(n, 0, mode)
+        O.wealth_run(x, y, w, rng, steps, mode)
+
+    return cpu_sample(run, steps, "steps/s",
+                      f"the full {steps}-step {n:,}-agent simulation (init included, as on the "
+                      f"GPU)", max(runs, 5))
+
+
+def _schelling_cpu(h, w, runs):
+    if runs <= 0:
+        return None
+    import oracle as O
+
+    mode, steps = O.MODE_PARTITIONABLE, 20
+    init = O.schelling_init(h, w, 0, mode)
+
+    def run():
+        cells, pos, typ, rng = (a.copy() for a in init)
+        O.schelling_run(h, w, cells, pos, typ, rng, steps, mode)
+
+    return cpu_sample(run, h * w * steps, "cell-steps/s",
+                      f"the first {steps} of the 200 steps (the busiest) of the {h}x{w} "
+                      f"simulation from the same seeded grid", min(runs, 3))
 
 
 def bench_wealth(args) -> dict:
@@ -284,6 +354,7 @@ def bench_wealth(args) -> dict:
         "config": {"workload": "wealth_1000agents_100steps", "n_agents": n, "sim_steps": S,
                    "bench_step": "one full simulation"},
         "run_latency_us": t * 1e6, "agent_steps_per_s": n * S / t,
+        "cpu_baseline": _wealth_cpu(n, S, args.cpu_runs),
         "roofline": {"bound": "latency", "achieved": None, "peak": None, "unit": None,
                      "frac": None, "traffic": None,
                      "note": "one workgroup holds the 32 KB/step working set; launch- and "
@@ -449,6 +520,7 @@ def bench_schelling(args) -> dict:
                    "bench_step": "one full simulation from the seeded grid (state restored "
                                  "by device copies, then 200 steps)"},
         "replays_identical": same,
+        "cpu_baseline": _schelling_cpu(H, W, args.cpu_runs),
         "final_happy_fraction": float(ref[0][-1]), "total_moves": int(ref[1].sum()),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
