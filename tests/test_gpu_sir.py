@@ -82,20 +82,26 @@ def test_sir_two_runs_continue(sir, mode):
     np.testing.assert_array_equal(np.concatenate([a, b]), want)
 
 
-def test_sir_full_size_invariants_and_prefix(sir):
-    """1M-agent BA(m=4) at full size: exact against the oracle on a prefix of
-    steps, then size-independent invariants over the full 365 steps."""
+@pytest.mark.parametrize("mode", MODES)
+def test_sir_full_size_matches_oracle(sir, mode):
+    """The measured workload itself -- 1M-agent BA(m=4), 365 steps, 100 seed
+    infections -- bit for bit against the oracle (its OpenMP path over the
+    agents of each step, the same results as its serial path): every step's
+    S/I/R counts and the final state; plus the invariants S + I + R = N, S
+    never grows, R never shrinks."""
+    import os
+
     n, m = 1_000_000, 4
     row_ptr, col = O.ba_csr(n, m, graph_seed=0)
     assert row_ptr[-1] == 2 * m * (n - m)
-    eng = sir.SIRNetworkEngine(n, csr=(row_ptr, col), seed=0)
-    st, rng = O.sir_init(n, 100, 0, O.MODE_PARTITIONABLE)
-    want = O.sir_run(row_ptr, col, st, rng, 0.05, 0.1, 12, O.MODE_PARTITIONABLE)
-    got = eng.run(12)
+    eng = sir.SIRNetworkEngine(n, csr=(row_ptr, col), seed=0, mode=mode)
+    st, rng = O.sir_init(n, 100, 0, mode)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    want = O.sir_run_threads(row_ptr, col, st, rng, 0.05, 0.1, 365, threads, mode)
+    got = np.concatenate([eng.run(12), eng.run(365 - 12)])   # a run continues the state
     np.testing.assert_array_equal(got, want)
     np.testing.assert_array_equal(eng.state_host(), st)
-    rest = eng.run(365 - 12)
-    allc = np.concatenate([got, rest])
-    assert (allc.sum(axis=1) == n).all()
-    assert (np.diff(allc[:, 0]) <= 0).all()   # S never grows
-    assert (np.diff(allc[:, 2]) >= 0).all()   # R never shrinks
+    assert (got.sum(axis=1) == n).all()
+    assert (np.diff(got[:, 0]) <= 0).all()   # S never grows
+    assert (np.diff(got[:, 2]) >= 0).all()   # R never shrinks
+    assert got[-1, 1] == 0 and got[-1, 2] > n // 2
