@@ -230,19 +230,19 @@ def test_full_size_batch(dm):
 
 
 def test_batch_size_does_not_change_an_instance(dm):
-    """An instance's loss and gradients are the same bits in a batch of 40
-    and of 96 (a multiple of the step kernel's 4-column vectors): instances
-    are independent columns of every kernel."""
+    """An instance's loss and gradients are the same bits in a batch of 41
+    (step and adjoint kernels on single columns) and of 96 (on 4-column
+    vectors): instances are independent columns of every kernel."""
     n, steps, k = 4096, 30, 7
     rs = np.random.default_rng(9)
     eps = rs.uniform(0.05, 0.5, 96).astype(np.float32)
     mu = rs.uniform(0.05, 0.5, 96).astype(np.float32)
     target = rs.uniform(0.0, 0.1, 34)
     out = []
-    for B in (40, 96):
+    for B in (41, 96):
         m = dm.DeffuantModel(n, B, steps, seed=2, ckpt_every=k)
         loss = m.forward(eps[:B], mu[:B], target).cpu().numpy()
         ge, gm = (g.cpu().numpy() for g in m.backward())
-        out.append((loss[:40], ge[:40], gm[:40]))
+        out.append((loss[:41], ge[:41], gm[:41]))
     for a, b in zip(*out):
         np.testing.assert_array_equal(a, b)
