@@ -359,6 +359,23 @@ int abm_graph_destroy(abm_graph *g);
  * nodes is refused (kErrInvalid) and left unchanged.  counts[4]: memsets
  * replaced, kernel nodes, memcpy nodes, empty nodes. */
 int abm_graph_sanitize(void *graph, int32_t *counts);
+/* One launch of up to ABM_MAX_COPY_ITEMS device copies: item i copies
+ * `bytes` from src to dst + row[0] * row_stride (row_stride 0: dst as is;
+ * rows at or past row_limit are skipped).  The items (a host array) travel
+ * in the kernel arguments, so the launch can be captured into a graph.  The
+ * per-step result copies and history writes of JaxModel.run(graph=True)
+ * (jaxabm/model.py:146-216: the new states, env and metrics of a step). */
+#define ABM_MAX_COPY_ITEMS 64
+typedef struct {
+  const void *src;
+  void *dst;
+  int64_t bytes;
+  int64_t row_stride;
+  int64_t row_limit;
+} abm_copy_item;
+int abm_copy_items(const abm_copy_item *items, int32_t n_items, const int64_t *row, void *stream);
+/* row[0] += 1 and ts[0] += 1 (the replayed step's history row and time step). */
+int abm_step_counters(int64_t *row, int32_t *ts, void *stream);
 /* Stream-ordered device-to-device copy (capturable). */
 int abm_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
 

@@ -18,6 +18,12 @@ MODE_ORIGINAL = 0
 MODE_PARTITIONABLE = 1
 
 GAMMA, LOGGAMMA, BETA = 0, 1, 2   # abm_gamma_each kinds
+MAX_COPY_ITEMS = 64                # abm_copy_items
+
+
+class CopyItem(C.Structure):
+    _fields_ = [("src", C.c_void_p), ("dst", C.c_void_p), ("bytes", C.c_int64),
+                ("row_stride", C.c_int64), ("row_limit", C.c_int64)]
 
 _lock = threading.Lock()
 _lib = None
@@ -183,6 +189,8 @@ SIGNATURES = {
     "abm_graph_launch": (_int, [_p, _p]),
     "abm_graph_destroy": (_int, [_p]),
     "abm_graph_sanitize": (_int, [_p, _p]),
+    "abm_copy_items": (_int, [_p, _i32, _p, _p]),
+    "abm_step_counters": (_int, [_p, _p, _p]),
     "abm_reduce_counts_u8": (_int, [_p, _i64, _i32, _p, _p]),
     "abm_reduce_workspace": (_sz, [_i64]),
     "abm_mean_f32": (_int, [_p, _i64, _p, _p, _sz, _p]),

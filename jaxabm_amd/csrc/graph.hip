@@ -11,6 +11,7 @@
 // refused.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "common.hpp"
@@ -96,9 +97,78 @@ int replace_memset(hipGraph_t g, hipGraphNode_t node) {
 
 }  // namespace abm
 
+namespace abm {
+
+// A step's result copies in one launch: item y copies items.it[y].bytes
+// from src to dst + row * row_stride (row read from device memory when
+// row_stride != 0; rows at or past row_limit are not written).  The list
+// travels by value in the kernel arguments, so a captured launch keeps it.
+struct CopyItems {
+  abm_copy_item it[ABM_MAX_COPY_ITEMS];
+};
+
+__global__ void __launch_bounds__(256)
+    k_copy_items(const CopyItems items, const int64_t *__restrict__ row) {
+  const abm_copy_item &c = items.it[blockIdx.y];
+  int64_t r = 0;
+  if (c.row_stride != 0) {
+    r = row[0];
+    if (r < 0 || r >= c.row_limit) return;
+  }
+  char *dst = (char *)c.dst + r * c.row_stride;
+  const char *src = (const char *)c.src;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if ((((uintptr_t)dst | (uintptr_t)src | (uintptr_t)c.bytes) & 15u) == 0) {
+    const int64_t n16 = c.bytes >> 4;
+    for (int64_t i = t0; i < n16; i += step)
+      ((uint4 *)dst)[i] = ((const uint4 *)src)[i];
+  } else {
+    for (int64_t i = t0; i < c.bytes; i += step) dst[i] = src[i];
+  }
+}
+
+__global__ void k_step_counters(int64_t *__restrict__ row, int32_t *__restrict__ ts) {
+  if (threadIdx.x == 0) {
+    row[0] += 1;
+    ts[0] += 1;
+  }
+}
+
+}  // namespace abm
+
 using namespace abm;
 
 extern "C" {
+
+int abm_copy_items(const abm_copy_item *items, int32_t n_items, const int64_t *row,
+                   void *stream) {
+  ABM_REQUIRE(items && n_items >= 0 && n_items <= ABM_MAX_COPY_ITEMS,
+              "abm_copy_items: bad item list");
+  if (n_items == 0) return kOk;
+  CopyItems list{};
+  int64_t max_bytes = 0;
+  for (int32_t i = 0; i < n_items; ++i) {
+    const abm_copy_item &c = items[i];
+    ABM_REQUIRE(c.src && c.dst && c.bytes >= 0 && (c.row_stride == 0 || (row && c.row_limit > 0)),
+                "abm_copy_items: bad item");
+    list.it[i] = c;
+    if (c.bytes > max_bytes) max_bytes = c.bytes;
+  }
+  const int64_t per_block = 256 * 16;
+  const unsigned bx = (unsigned)std::min<int64_t>(std::max<int64_t>(1, (max_bytes + per_block - 1) / per_block), 1024);
+  hipLaunchKernelGGL(k_copy_items, dim3(bx, (unsigned)n_items), dim3(256), 0, (hipStream_t)stream,
+                     list, row);
+  ABM_LAUNCH_CHECK("abm_copy_items");
+  return kOk;
+}
+
+int abm_step_counters(int64_t *row, int32_t *ts, void *stream) {
+  ABM_REQUIRE(row && ts, "abm_step_counters: null pointer");
+  hipLaunchKernelGGL(k_step_counters, dim3(1), dim3(64), 0, (hipStream_t)stream, row, ts);
+  ABM_LAUNCH_CHECK("abm_step_counters");
+  return kOk;
+}
 
 int abm_graph_begin(void *stream) {
   ABM_CHECK_HIP(hipStreamBeginCapture((hipStream_t)stream,

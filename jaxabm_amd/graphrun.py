@@ -44,6 +44,7 @@ import numpy as np
 import torch
 
 from . import random as jr
+from ._lib import MAX_COPY_ITEMS, CopyItem, check, lib, stream_handle
 
 _SCALARS = (bool, int, float)
 
@@ -166,8 +167,22 @@ class _Runner:
                 srcs.append(t.clone())
             else:
                 srcs.append(t)
+        # contiguous same-dtype copies and history rows go to one launch
+        items, keep = [], []
+
+        def batched(src, dst, row_stride=0, row_limit=0) -> bool:
+            if (len(items) >= MAX_COPY_ITEMS or not src.is_contiguous() or
+                    not dst.is_contiguous() or src.dtype != dst.dtype or not src.is_cuda):
+                return False
+            nbytes = src.numel() * src.element_size()
+            if nbytes != (row_stride or dst.numel() * dst.element_size()):
+                return False
+            items.append(CopyItem(src.data_ptr(), dst.data_ptr(), nbytes, row_stride, row_limit))
+            keep.append(src)
+            return True
+
         for (d, _), t in zip(pairs, srcs):
-            if t is not None:
+            if t is not None and not batched(t, d):
                 d.copy_(t)
         for slot, v in env_fill:
             slot.fill_(v)
@@ -176,11 +191,17 @@ class _Runner:
         for k, v in metrics.items():
             h = self.hist[k]
             if isinstance(v, torch.Tensor):
-                h.index_copy_(0, self.row, v.reshape(1, *h.shape[1:]))
+                row = v.reshape(h.shape[1:])
+                if not batched(row, h, h[0].numel() * h.element_size(), h.shape[0]):
+                    h.index_copy_(0, self.row, v.reshape(1, *h.shape[1:]))
             else:
                 h.index_fill_(0, self.row, v)
-        self.ts.add_(1)
-        self.row.add_(1)
+        if items:
+            arr = (CopyItem * len(items))(*items)
+            check(lib().abm_copy_items(arr, len(items), self.row.data_ptr(), stream_handle()),
+                  "copy_items")
+        check(lib().abm_step_counters(self.row.data_ptr(), self.ts.data_ptr(), stream_handle()),
+              "step_counters")
 
     def first_step(self):
         """Step 1, eagerly, on the model's own env values and Python time
