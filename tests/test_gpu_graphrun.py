@@ -245,3 +245,39 @@ def test_differentiable_run_stays_eager(J, capsys):
     assert m.last_run_info["mode"] == "eager" and "grad" in m.last_run_info["reason"]
     (g,) = torch.autograd.grad(out["m"][-1], [inc])
     assert float(g) == pytest.approx(6.0, rel=1e-6)
+
+
+def test_copy_items_paths(J):
+    """abm_copy_items: 16-byte and byte paths, history rows at row[0] *
+    row_stride, rows at or past row_limit skipped; abm_step_counters."""
+    import ctypes as C
+
+    import torch
+
+    from jaxabm_amd._lib import CopyItem, check, lib, stream_handle
+
+    a = torch.arange(1000, dtype=torch.float32, device="cuda")          # 4000 B, aligned
+    b = torch.arange(7, dtype=torch.uint8, device="cuda") + 1           # 7 B, byte path
+    da, db = torch.zeros_like(a), torch.zeros(7, dtype=torch.uint8, device="cuda")
+    hist = torch.zeros((5, 3), dtype=torch.float32, device="cuda")
+    v = torch.tensor([1.5, 2.5, 3.5], device="cuda")
+    row = torch.tensor([2], dtype=torch.int64, device="cuda")
+    ts = torch.zeros((), dtype=torch.int32, device="cuda")
+    items = (CopyItem * 3)(CopyItem(a.data_ptr(), da.data_ptr(), 4000, 0, 0),
+                           CopyItem(b.data_ptr(), db.data_ptr(), 7, 0, 0),
+                           CopyItem(v.data_ptr(), hist.data_ptr(), 12, 12, 5))
+    check(lib().abm_copy_items(items, 3, row.data_ptr(), stream_handle()), "copy_items")
+    check(lib().abm_step_counters(row.data_ptr(), ts.data_ptr(), stream_handle()), "counters")
+    torch.cuda.synchronize()
+    assert torch.equal(da, a) and torch.equal(db, b)
+    want = torch.zeros((5, 3))
+    want[2] = v.cpu()
+    assert torch.equal(hist.cpu(), want)
+    assert int(row.item()) == 3 and int(ts.item()) == 1
+    row.fill_(5)                                   # at row_limit: not written
+    check(lib().abm_copy_items(items, 3, row.data_ptr(), stream_handle()), "copy_items")
+    torch.cuda.synchronize()
+    assert torch.equal(hist.cpu(), want)
+    with pytest.raises(Exception):
+        lib_items = (CopyItem * 1)(CopyItem(v.data_ptr(), hist.data_ptr(), 12, 12, 0))
+        check(lib().abm_copy_items(lib_items, 1, row.data_ptr(), stream_handle()), "bad")
