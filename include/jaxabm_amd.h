@@ -376,6 +376,15 @@ typedef struct {
 int abm_copy_items(const abm_copy_item *items, int32_t n_items, const int64_t *row, void *stream);
 /* row[0] += 1 and ts[0] += 1 (the replayed step's history row and time step). */
 int abm_step_counters(int64_t *row, int32_t *ts, void *stream);
+/* abm_copy_items plus the step's tail in the same launch: the last block to
+ * finish advances row[0] and ts[0], returns *done (a device counter, 0
+ * between launches) to 0, and copies key_words words of row row[0] of
+ * key_table[key_rows][key_words] to key_dst (when the new row is below
+ * key_rows; key_dst may be NULL): the next step's keys.  One node per
+ * replayed step of JaxModel.run(graph=True) for all of its bookkeeping. */
+int abm_commit_step(const abm_copy_item *items, int32_t n_items, int64_t *row, int32_t *ts,
+                    uint32_t *done, const uint32_t *key_table, int64_t key_rows,
+                    int32_t key_words, uint32_t *key_dst, void *stream);
 /* Stream-ordered device-to-device copy (capturable). */
 int abm_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
 

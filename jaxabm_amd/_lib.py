@@ -191,6 +191,7 @@ SIGNATURES = {
     "abm_graph_sanitize": (_int, [_p, _p]),
     "abm_copy_items": (_int, [_p, _i32, _p, _p]),
     "abm_step_counters": (_int, [_p, _p, _p]),
+    "abm_commit_step": (_int, [_p, _i32, _p, _p, _p, _p, _i64, _i32, _p, _p]),
     "abm_reduce_counts_u8": (_int, [_p, _i64, _i32, _p, _p]),
     "abm_reduce_workspace": (_sz, [_i64]),
     "abm_mean_f32": (_int, [_p, _i64, _p, _p, _sz, _p]),

@@ -128,6 +128,54 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// k_copy_items plus the step's tail, done by the last block to finish (the
+// one whose arrival makes `done` reach the grid size; every block reads row
+// before it arrives): row and ts advance, `done` returns to 0, and row r + 1
+// of the key table is copied to key_dst for the next step (if r + 1 <
+// key_rows).
+__global__ void __launch_bounds__(256)
+    k_commit_step(const CopyItems items, int32_t n_items, int64_t *__restrict__ row,
+                  int32_t *__restrict__ ts, uint32_t *__restrict__ done,
+                  const uint32_t *__restrict__ key_table, int64_t key_rows, int32_t key_words,
+                  uint32_t *__restrict__ key_dst) {
+  if ((int32_t)blockIdx.y < n_items) {
+    const abm_copy_item &c = items.it[blockIdx.y];
+    int64_t r = 0;
+    bool write = true;
+    if (c.row_stride != 0) {
+      r = row[0];
+      write = r >= 0 && r < c.row_limit;
+    }
+    if (write) {
+      char *dst = (char *)c.dst + r * c.row_stride;
+      const char *src = (const char *)c.src;
+      const int64_t step = (int64_t)gridDim.x * blockDim.x;
+      const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+      if ((((uintptr_t)dst | (uintptr_t)src | (uintptr_t)c.bytes) & 15u) == 0) {
+        const int64_t n16 = c.bytes >> 4;
+        for (int64_t i = t0; i < n16; i += step) ((uint4 *)dst)[i] = ((const uint4 *)src)[i];
+      } else {
+        for (int64_t i = t0; i < c.bytes; i += step) dst[i] = src[i];
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned total = gridDim.x * gridDim.y;
+    if (atomicAdd(done, 1u) == total - 1) {
+      __threadfence();
+      const int64_t r = row[0] + 1;
+      row[0] = r;
+      ts[0] += 1;
+      *done = 0u;
+      if (key_dst && r < key_rows)
+        for (int32_t w = 0; w < key_words; ++w) key_dst[w] = key_table[r * key_words + w];
+      __threadfence();
+    }
+  }
+}
+
 __global__ void k_step_counters(int64_t *__restrict__ row, int32_t *__restrict__ ts) {
   if (threadIdx.x == 0) {
     row[0] += 1;
@@ -160,6 +208,31 @@ int abm_copy_items(const abm_copy_item *items, int32_t n_items, const int64_t *r
   hipLaunchKernelGGL(k_copy_items, dim3(bx, (unsigned)n_items), dim3(256), 0, (hipStream_t)stream,
                      list, row);
   ABM_LAUNCH_CHECK("abm_copy_items");
+  return kOk;
+}
+
+int abm_commit_step(const abm_copy_item *items, int32_t n_items, int64_t *row, int32_t *ts,
+                    uint32_t *done, const uint32_t *key_table, int64_t key_rows,
+                    int32_t key_words, uint32_t *key_dst, void *stream) {
+  ABM_REQUIRE((items || n_items == 0) && n_items >= 0 && n_items <= ABM_MAX_COPY_ITEMS && row &&
+                  ts && done && key_rows >= 0 && key_words >= 0 &&
+                  (!key_dst || (key_table && key_words > 0)),
+              "abm_commit_step: bad arguments");
+  CopyItems list{};
+  int64_t max_bytes = 0;
+  for (int32_t i = 0; i < n_items; ++i) {
+    const abm_copy_item &c = items[i];
+    ABM_REQUIRE(c.src && c.dst && c.bytes >= 0 && (c.row_stride == 0 || c.row_limit > 0),
+                "abm_commit_step: bad item");
+    list.it[i] = c;
+    if (c.bytes > max_bytes) max_bytes = c.bytes;
+  }
+  const int64_t per_block = 256 * 16;
+  const unsigned bx = (unsigned)std::min<int64_t>(std::max<int64_t>(1, (max_bytes + per_block - 1) / per_block), 1024);
+  hipLaunchKernelGGL(k_commit_step, dim3(bx, (unsigned)std::max<int32_t>(n_items, 1)), dim3(256), 0,
+                     (hipStream_t)stream, list, n_items, row, ts, done, key_table, key_rows,
+                     key_words, key_dst);
+  ABM_LAUNCH_CHECK("abm_commit_step");
   return kOk;
 }
 

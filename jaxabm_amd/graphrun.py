@@ -84,6 +84,10 @@ class _Runner:
         self.table = torch.from_numpy(
             np.ascontiguousarray(tab).view(np.int32).reshape(n, self.width, 2).copy()).to(self.dev)
         self.row = torch.zeros((1,), dtype=torch.int64, device=self.dev)
+        # the current step's keys: row 0 now, then written by each step's
+        # commit for the next (no gather node in the replayed step)
+        self.keys = self.table[0].clone()
+        self.done = torch.zeros((1,), dtype=torch.int32, device=self.dev)
         self.ts = torch.full((), int(model._time_step), dtype=torch.int32, device=self.dev)
         self.static = {name: {k: v.clone() for k, v in c.states.items()} for name, c in self.colls}
         self.py_types: Dict[str, type] = {}
@@ -95,7 +99,7 @@ class _Runner:
 
     # -------------------------------------------------------------- the body
     def body(self, time_step=None):
-        keys = self.table.index_select(0, self.row).reshape(self.width, 2)
+        keys = self.keys
         current = {"time_step": self.ts if time_step is None else time_step, "env": self.env}
         for name, _c in self.colls:
             current[f"agents_{name}"] = self.static[name]
@@ -196,12 +200,11 @@ class _Runner:
                     h.index_copy_(0, self.row, v.reshape(1, *h.shape[1:]))
             else:
                 h.index_fill_(0, self.row, v)
-        if items:
-            arr = (CopyItem * len(items))(*items)
-            check(lib().abm_copy_items(arr, len(items), self.row.data_ptr(), stream_handle()),
-                  "copy_items")
-        check(lib().abm_step_counters(self.row.data_ptr(), self.ts.data_ptr(), stream_handle()),
-              "step_counters")
+        arr = (CopyItem * max(len(items), 1))(*items)
+        check(lib().abm_commit_step(arr, len(items), self.row.data_ptr(), self.ts.data_ptr(),
+                                    self.done.data_ptr(), self.table.data_ptr(), self.n,
+                                    2 * self.width, self.keys.data_ptr(), stream_handle()),
+              "commit_step")
 
     def first_step(self):
         """Step 1, eagerly, on the model's own env values and Python time

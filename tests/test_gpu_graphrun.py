@@ -281,3 +281,35 @@ def test_copy_items_paths(J):
     with pytest.raises(Exception):
         lib_items = (CopyItem * 1)(CopyItem(v.data_ptr(), hist.data_ptr(), 12, 12, 0))
         check(lib().abm_copy_items(lib_items, 1, row.data_ptr(), stream_handle()), "bad")
+
+
+def test_commit_step_tail(J):
+    """abm_commit_step: the copies, then (once, by the last block) row and
+    time step advance, the counter returns to 0 and the next row of the key
+    table lands in the key buffer; past the table's last row no keys are
+    copied."""
+    import torch
+
+    from jaxabm_amd._lib import CopyItem, check, lib, stream_handle
+
+    a = torch.arange(300_000, dtype=torch.float32, device="cuda")     # many blocks
+    da = torch.zeros_like(a)
+    table = torch.arange(3 * 4, dtype=torch.int32, device="cuda").reshape(3, 2, 2)
+    keys = table[0].clone()
+    row = torch.zeros(1, dtype=torch.int64, device="cuda")
+    ts = torch.zeros((), dtype=torch.int32, device="cuda")
+    done = torch.zeros(1, dtype=torch.int32, device="cuda")
+    items = (CopyItem * 1)(CopyItem(a.data_ptr(), da.data_ptr(), a.numel() * 4, 0, 0))
+    for step in range(1, 4):
+        check(lib().abm_commit_step(items, 1, row.data_ptr(), ts.data_ptr(), done.data_ptr(),
+                                    table.data_ptr(), 3, 4, keys.data_ptr(), stream_handle()),
+              "commit_step")
+        torch.cuda.synchronize()
+        assert int(row.item()) == step and int(ts.item()) == step and int(done.item()) == 0
+        assert torch.equal(keys, table[min(step, 2)])
+    assert torch.equal(da, a)
+    empty = (CopyItem * 1)()
+    check(lib().abm_commit_step(empty, 0, row.data_ptr(), ts.data_ptr(), done.data_ptr(),
+                                table.data_ptr(), 3, 4, keys.data_ptr(), stream_handle()), "none")
+    torch.cuda.synchronize()
+    assert int(row.item()) == 4 and int(done.item()) == 0
