@@ -966,14 +966,38 @@ class _Trace:
         return f"(({_CT[kind]})({x.name}))"
 
     def power(self, x, e):
-        if not isinstance(e, int) or isinstance(e, bool) or e not in (2, 3):
-            raise Unsupported("power other than x**2 or x**3")
+        """x ** e for a Python number e, as torch's pow of a tensor by a
+        scalar: 2 and 3 multiply, 0.5 is sqrt, -1 and -2 reciprocals, any
+        other exponent of a float pow(x, e) with e in x's dtype (-0.5 is not
+        traced); integer bases take only 2 and 3."""
+        if isinstance(e, bool) or not isinstance(e, (int, float)):
+            raise Unsupported("a power with an exponent that is not a Python number")
         if isinstance(x, SymVec):
             return SymVec(self, [self.power(v, e) for v in x.items])
         x = self.operand(x)
+        if x.weak:
+            raise Unsupported("a power of a Python scalar")
         kind = self.promote(x, self.const(e))
         cx = self._as(x, kind)
-        expr = f"({cx} * {cx})" if e == 2 else f"({cx} * {cx} * {cx})"
+        if e in (2, 3):
+            expr = f"({cx} * {cx})" if e == 2 else f"({cx} * {cx} * {cx})"
+        elif kind not in ("f32", "f64"):
+            raise Unsupported("an integer power other than x**2 or x**3")
+        else:
+            f32 = kind == "f32"
+            one = "1.0f" if f32 else "1.0"
+            if e == 0.5:
+                expr = f"{'sqrtf' if f32 else 'sqrt'}({cx})"
+            elif e == -0.5:
+                # torch's own form of this case is not rsqrtf (one ulp apart
+                # on 5% of float32 values): left to the vmapped path
+                raise Unsupported("x ** -0.5")
+            elif e == -1:
+                expr = f"({one} / {cx})"
+            elif e == -2:
+                expr = f"({one} / ({cx} * {cx}))"
+            else:
+                expr = f"{'powf' if f32 else 'pow'}({cx}, {_lit(float(e), kind)})"
         return Sym(self, self.tmp(_CT[kind], expr), kind, x.varying)
 
     _UNARY_F = {"exp": "expf", "log": "logf", "log1p": "log1pf", "expm1": "expm1f",

@@ -84,7 +84,7 @@ def test_same_function_same_source_for_other_env_values():
     (lambda s, m, k: {"x": s["x"] + m["agents_a"]["x"]}, "operand"),
     (lambda s, m, k: {"x": jr.uniform(k, (3, 2))}, "two-dimensional draw"),
     (lambda s, m, k: {"x": float(s["x"])}, "float()"),
-    (lambda s, m, k: {"x": s["x"] ** 0.5}, "power"),
+    (lambda s, m, k: {"x": s["w"] ** 4}, "integer power"),
     (lambda s, m, k: [s["x"]], "dict"),
 ])
 def test_refused(fn, why):

@@ -554,3 +554,94 @@ def test_calibrator_autograd_gradient(J, capsys):
     g_fd = cal.grad_fn(p0)
     assert cal.last_gradient == "finite_difference"
     np.testing.assert_allclose(g_fd, closed_form(mean0), rtol=2e-2)
+
+
+def _economy(J, productivity=1.0, steps=20, graph=False):
+    """A consumer / producer economy in the style of the reference's
+    integration test (tests/integration/test_integration.py:19-200): incomes
+    and capital drawn at init, consumption priced by the env's price level,
+    production capital ** 0.7 with reinvested profits, and an env phase that
+    moves the price with the supply / demand ratio and records GDP."""
+    from jaxabm_amd import numpy as jnp
+    from jaxabm_amd import random as jr
+
+    class Consumer:
+        def init_state(self, cfg, key):
+            return {"income": 0.8 + 0.4 * jr.uniform(key), "savings": jnp.array(0.0),
+                    "consumption": jnp.array(0.0), "utility": jnp.array(0.0)}
+
+        def update(self, s, ms, cfg, key):
+            price = ms["env"].get("price_level", 1.0)
+            consumption = 0.8 * s["income"] / price
+            return {"income": s["income"], "consumption": consumption,
+                    "savings": s["savings"] + s["income"] - consumption * price,
+                    "utility": jnp.log(consumption + 1.0)}
+
+    class Producer:
+        def init_state(self, cfg, key):
+            return {"capital": 10.0 * (0.8 + 0.4 * jr.uniform(key)),
+                    "production": jnp.array(0.0), "profit": jnp.array(0.0)}
+
+        def update(self, s, ms, cfg, key):
+            production = productivity * s["capital"] ** 0.7
+            profit = production * ms["env"].get("price_level", 1.0) - (
+                0.1 * s["capital"] + 0.05 * production)
+            return {"capital": s["capital"] + 0.3 * profit, "production": production,
+                    "profit": profit}
+
+    def env_fn(env, agents, params, key):
+        demand = jnp.sum(agents["consumers"]["consumption"])
+        supply = jnp.sum(agents["producers"]["production"])
+        ratio = (supply + 1e-8) / (demand + 1e-8)
+        price = jnp.clip(env.get("price_level", 1.0) * (1.0 + params.get("rate", 0.1) *
+                                                         (1.0 - ratio)), 0.5, 2.0)
+        return {"price_level": price, "gdp": supply * price,
+                "unemployment": jnp.maximum(0.0, jnp.minimum(0.5, 1.0 - ratio))}
+
+    def metrics(env, agents, params):
+        return {"gdp": env["gdp"], "price_level": env["price_level"],
+                "unemployment": env["unemployment"],
+                "avg_utility": jnp.mean(agents["consumers"]["utility"]),
+                "avg_profit": jnp.mean(agents["producers"]["profit"])}
+
+    m = J.JaxModel(params={"rate": 0.1}, config=J.ModelConfig(seed=1, steps=steps),
+                   update_state_fn=env_fn, metrics_fn=metrics)
+    m.add_agent_collection("consumers", J.AgentCollection(Consumer(), 200))
+    m.add_agent_collection("producers", J.AgentCollection(Producer(), 50))
+    m.add_env_state("price_level", 1.0)
+    res = m.run(steps, graph=graph)
+    return m, res
+
+
+def test_economy_integration(J, capsys):
+    """The reference's integration assertions (keys, lengths, trajectories
+    that move, more output from more productive firms) on a model whose
+    agent functions run fused, against the vmapped and the graph-replayed
+    runs of the same model (identical)."""
+    import torch
+
+    from jaxabm_amd import fuse
+
+    m, res = _economy(J)
+    for k in ("gdp", "price_level", "unemployment", "avg_utility", "avg_profit"):
+        assert k in res and len(res[k]) == 20, k
+    assert float(res["gdp"][0]) != float(res["gdp"][-1])
+    assert float(res["price_level"][0]) != float(res["price_level"][-1])
+    _, hi = _economy(J, productivity=1.5)
+    assert np.mean([float(v) for v in hi["gdp"]]) > np.mean([float(v) for v in res["gdp"]])
+    fuse.enable(False)
+    try:
+        mv, rv = _economy(J)
+    finally:
+        fuse.enable(True)
+    mg, rg = _economy(J, graph=True)
+    assert mg.last_run_info["mode"] == "graph", mg.last_run_info
+    for other, r in ((mv, rv), (mg, rg)):
+        for k in res:
+            np.testing.assert_array_equal([np.asarray(torch.as_tensor(v).cpu()) for v in r[k]],
+                                          [np.asarray(torch.as_tensor(v).cpu()) for v in res[k]],
+                                          err_msg=k)
+        for c in ("consumers", "producers"):
+            for k, v in m.agent_collections[c].states.items():
+                assert torch.equal(v, other.agent_collections[c].states[k]), (c, k)
+    assert not any("Producer" in str(k) for k in fuse.refused()), fuse.refused()

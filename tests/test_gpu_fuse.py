@@ -212,6 +212,11 @@ OPS = {
     "div_t": lambda s, m: s["x"] / m["env"]["delta"],
     "div_xx": lambda s, m: s["x"] / (s["x"] + 1.0), "int_div3": lambda s, m: s["w"] / 3,
     "xsq7": lambda s, m: (s["x"] ** 2) / 7.0,
+    "pow07": lambda s, m: abs(s["x"]) ** 0.7, "pow_half": lambda s, m: s["x"] ** 0.5,
+    "pow_m1": lambda s, m: s["x"] ** -1,
+    "pow_m2": lambda s, m: s["x"] ** -2.0, "pow_25": lambda s, m: s["x"] ** 2.5,
+    "pow_n": lambda s, m: s["x"] ** -1.7, "pow_w": lambda s, m: s["w"] ** 0.5,
+    "pow_3f": lambda s, m: s["x"] ** 3.0,
     "fdiv_s": lambda s, m: s["x"] // 0.7, "fdiv_sn": lambda s, m: s["x"] // -1.3,
     "fdiv_xx": lambda s, m: s["x"] // (abs(s["x"]) * 0.3 + 0.1),
     "fdiv_env": lambda s, m: s["x"] // m["env"]["scale"],
