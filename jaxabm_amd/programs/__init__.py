@@ -1,2 +1,3 @@
-"""Library model programs: the BASELINE agent-step workloads as fused HIP
-step kernels behind the reference's Model/Agent API."""
+"""Library model programs -- network SIR, wealth exchange, Schelling and
+Deffuant -- as fused HIP step kernels behind the reference's Model/Agent
+API."""
