@@ -576,7 +576,7 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
+        print(f"bench.py: the engine is measured on one GPU only (asked for {args.gpus}); "
               "multi-GPU sharding is a design note, not a measured path.", file=sys.stderr)
         return 2
     if args.pmc_probe:
