@@ -316,6 +316,22 @@ def _schelling_cpu(h, w, runs):
                       f"simulation from the same seeded grid", min(runs, 3))
 
 
+def _trader_cpu(n, steps, runs):
+    if runs <= 0:
+        return None
+    import oracle as O
+
+    mode = O.MODE_PARTITIONABLE
+
+    def run():
+        x, w, rng = O.trader_init(n, 0, mode)
+        O.trader_run(x, w, rng, steps, mode)
+
+    return cpu_sample(run, steps, "steps/s",
+                      f"the full {steps}-step {n:,}-agent model (the oracle's restatement of its "
+                      f"agent code and key schedule)", min(runs, 3))
+
+
 def bench_wealth(args) -> dict:
     """Wealth exchange, 1,000 agents, 100 steps, seed 0: one bench step = one
     whole simulation (state init + one single-workgroup launch)."""
@@ -455,6 +471,7 @@ def bench_generic(args) -> dict:
         "config": {"workload": "generic_trader_10000agents_200steps", "n_agents": n,
                    "sim_steps": S, "bench_step": "one 200-step Model.run"},
         "steps_per_s": out, "speedup_vs_eager_vmap": out["graph_fused"] / out["eager_vmap"],
+        "cpu_baseline": _trader_cpu(n, S, args.cpu_runs),
         "identical_results": same, "graph_info": models["graph_fused"].last_run_info,
         "fuse": fuse.stats(),
     }

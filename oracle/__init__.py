@@ -86,6 +86,8 @@ def _declare(L):
                                        C.c_int32, C.c_int, _i32p, C.c_int]),
         "orc_sir_run": (None, [_i64, _i32p, _i32p, _u8p, _u32p, C.c_float,
                                C.c_float, _i32, C.c_int, _i32p]),
+        "orc_trader_init": (None, [_i64, _i64, C.c_int, _f32p, _i32p, _u32p]),
+        "orc_trader_run": (None, [_i64, _f32p, _i32p, _u32p, _i32, C.c_int]),
         "orc_wealth_init": (None, [_i64, _i64, C.c_int, _f32p, _f32p, _i32p, _u32p]),
         "orc_wealth_run": (None, [_i64, _f32p, _f32p, _i32p, _u32p, _i32,
                                   C.c_int, _f32p, _f32p]),
@@ -248,6 +250,20 @@ def sir_run_threads(row_ptr, col, state, rng, beta: float, gamma: float, steps: 
                               state, rng, beta, gamma, steps, mode, counts.reshape(-1),
                               int(threads))
     return counts[:steps]
+
+
+def trader_init(n: int, seed: int, mode: int = MODE_PARTITIONABLE):
+    """The generic trader model's initial x, w and model key (see oracle.c)."""
+    x = np.zeros(n, np.float32)
+    w = np.zeros(n, np.int32)
+    rng = np.zeros(2, np.uint32)
+    lib().orc_trader_init(n, seed, mode, x, w, rng)
+    return x, w, rng
+
+
+def trader_run(x, w, rng, steps: int, mode: int = MODE_PARTITIONABLE):
+    lib().orc_trader_run(len(x), x, w, rng, steps, mode)
+    return x, w
 
 
 def wealth_init(n: int, seed: int, mode: int = MODE_PARTITIONABLE):

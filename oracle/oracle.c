@@ -628,6 +628,49 @@ static void agent_xy_draw(const uint32_t akey[2], float *a, float *b,
   }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Generic trader model (the per-agent code of bench.py's generic workload,
+ * written against the reference API: one collection with an update and an
+ * env-phase callback, jaxabm/model.py:118-216).  Init with agent key a:
+ * k1, k2 = split(a); x = uniform(k1); w = randint(k2, (), 0, 6).  Step with
+ * agent key a: k1, k2 = split(a); x = mod(x + normal(k1) * 0.01, 1);
+ * w = (uniform(k2) < 0.5 and w > 0) ? w - 1 : w + 1. */
+void orc_trader_init(int64_t n, int64_t seed, int mode, float *x, int32_t *w,
+                     uint32_t rng[2]) {
+  uint32_t ik[2];
+  init_keys(seed, mode, rng, ik);
+  uint32_t *ak = (uint32_t *)malloc(sizeof(uint32_t) * 2 * (size_t)(n > 0 ? n : 1));
+  orc_split(ik, n, ak, mode);
+  for (int64_t i = 0; i < n; ++i) {
+    uint32_t k1[2], k2[2];
+    split2(ak + 2 * i, k1, k2, mode);
+    orc_uniform_f32(k1, 1, 0.0f, 1.0f, &x[i], mode);
+    orc_randint_i32(k2, 1, 0, 6, &w[i], mode);
+  }
+  free(ak);
+}
+
+void orc_trader_run(int64_t n, float *x, int32_t *w, uint32_t rng[2], int32_t steps,
+                    int mode) {
+  uint32_t *ak = (uint32_t *)malloc(sizeof(uint32_t) * 2 * (size_t)(n > 0 ? n : 1));
+  for (int32_t t = 0; t < steps; ++t) {
+    uint32_t ck[2], uk[2];
+    step_keys(rng, ck, uk, mode);
+    orc_split(ck, n, ak, mode);
+    for (int64_t i = 0; i < n; ++i) {
+      uint32_t k1[2], k2[2];
+      split2(ak + 2 * i, k1, k2, mode);
+      float z, u;
+      orc_normal_f32(k1, 1, &z, mode);
+      orc_uniform_f32(k2, 1, 0.0f, 1.0f, &u, mode);
+      volatile float d = z * 0.01f;
+      x[i] = mod1(x[i] + d);
+      w[i] = (u < 0.5f && w[i] > 0) ? w[i] - 1 : w[i] + 1;
+    }
+  }
+  free(ak);
+}
+
 void orc_wealth_init(int64_t n, int64_t seed, int mode, float *x, float *y,
                      int32_t *wealth, uint32_t rng[2]) {
   uint32_t ik[2];

@@ -75,6 +75,10 @@ void orc_sir_run_threads(int64_t n, const int32_t *row_ptr, const int32_t *col,
                          uint8_t *state, uint32_t rng[2], float beta, float gamma,
                          int32_t steps, int mode, int32_t *counts, int threads);
 
+void orc_trader_init(int64_t n, int64_t seed, int mode, float *x, int32_t *w,
+                     uint32_t rng[2]);
+void orc_trader_run(int64_t n, float *x, int32_t *w, uint32_t rng[2], int32_t steps,
+                    int mode);
 void orc_wealth_init(int64_t n, int64_t seed, int mode, float *x, float *y,
                      int32_t *wealth, uint32_t rng[2]);
 void orc_wealth_run(int64_t n, float *x, float *y, int32_t *wealth,

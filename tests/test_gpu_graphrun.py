@@ -313,3 +313,54 @@ def test_commit_step_tail(J):
                                 table.data_ptr(), 3, 4, keys.data_ptr(), stream_handle()), "none")
     torch.cuda.synchronize()
     assert int(row.item()) == 4 and int(done.item()) == 0
+
+
+class Trader:
+    """The generic workload's agent (a normal-draw walk of x, a coin-flip
+    wealth change), as the oracle restates it (orc_trader_*)."""
+
+    def init_state(self, cfg, key):
+        from jaxabm_amd import random as jr
+
+        k1, k2 = jr.split(key)
+        return {"x": jr.uniform(k1), "w": jr.randint(k2, (), 0, 6)}
+
+    def update(self, state, model_state, cfg, key):
+        from jaxabm_amd import numpy as jnp
+        from jaxabm_amd import random as jr
+
+        k1, k2 = jr.split(key)
+        x = jnp.mod(state["x"] + jr.normal(k1) * 0.01, 1.0)
+        give = jr.uniform(k2) < 0.5
+        return {"x": x, "w": jnp.where(give & (state["w"] > 0), state["w"] - 1, state["w"] + 1)}
+
+
+@pytest.mark.parametrize("fused,graph", [(True, True), (False, False)])
+def test_generic_trader_matches_oracle(J, capsys, fused, graph):
+    """The generic workload's model (10,000 agents, 200 steps, an env phase)
+    against the oracle's restatement: wealth exactly; positions to float32
+    rounding of the normal draw (distance on the unit circle, as mod 1
+    wraps), in the graph-replayed fused run and the eager vmapped one."""
+    import oracle as O
+    from jaxabm_amd import fuse
+    from jaxabm_amd import numpy as jnp
+
+    n, steps = 10_000, 200
+    fuse.enable(fused)
+    try:
+        m = J.JaxModel(params={}, config=J.ModelConfig(steps=steps, seed=0),
+                       update_state_fn=lambda env, a, p, k: {**env, "mean_x": jnp.mean(a["t"]["x"])},
+                       metrics_fn=lambda env, a, p: {"total_w": jnp.sum(a["t"]["w"])})
+        m.add_agent_collection("t", J.AgentCollection(Trader(), n))
+        res = m.run(steps, graph=graph)
+    finally:
+        fuse.enable(True)
+    assert m.last_run_info["mode"] == ("graph" if graph else "eager")
+    x, w, rng = O.trader_init(n, 0, O.MODE_PARTITIONABLE)
+    O.trader_run(x, w, rng, steps, O.MODE_PARTITIONABLE)
+    st = m.agent_collections["t"].states
+    np.testing.assert_array_equal(st["w"].cpu().numpy(), w)
+    d = np.abs(st["x"].cpu().numpy().astype(np.float64) - x)
+    assert np.minimum(d, 1.0 - d).max() < 1e-5
+    assert int(res["total_w"][-1]) == int(w.sum())
+    np.testing.assert_array_equal(np.asarray(m._rng, np.uint32), rng)
