@@ -12,9 +12,11 @@
  *     jaxabm/agent.py:115,156 (JAX itself is an un-vendored dependency,
  *     requirements.txt:9-10, jax>=0.4.1 unpinned).
  *   - The reference step schedule of JaxModel.initialize/step
- *     (jaxabm/model.py:118-216) applied to four model programs written
+ *     (jaxabm/model.py:118-216) applied to five model programs written
  *     against the reference API: wealth exchange, Schelling, network SIR,
- *     Deffuant (forward and adjoint).
+ *     Deffuant (forward and adjoint), and a generic trader model (the
+ *     per-agent code of the generic-model workload).
+ *   - jax.random's gamma / loggamma / beta samplers (Marsaglia-Tsang).
  *   - Record-time reductions: mean, Gini (examples/models/
  *     advanced_economic_model.py:1798-1811), S/I/R counts, soft histogram.
  *
