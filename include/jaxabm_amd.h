@@ -72,6 +72,19 @@ int abm_randint_each(const uint32_t *keys, int64_t n, int64_t per, int32_t lo, i
 #define ABM_BETA 2
 int abm_gamma_each(const uint32_t *keys, int64_t n, int64_t per, const float *a,
                    const float *b, int kind, float *out, int mode, void *stream);
+/* The float32 transcendental functions the draws and model kernels call
+ * (log1p inside erf_inv of jax.random.normal; log, log1p, pow and exp of
+ * jax.random.gamma / beta, examples/models/advanced_economic_model.py:106;
+ * exp of the Deffuant sigmoid gate, jaxabm/analysis.py:489-531), written with
+ * IEEE basic operations only (csrc/abm_math.h), elementwise:
+ * out[i] = f(x[i]) or pow(x[i], y[i]) (y NULL unless ABM_MATH_POW).
+ * abm_math_f64: the float64 exp and log they are built on. */
+#define ABM_MATH_EXP 0
+#define ABM_MATH_LOG 1
+#define ABM_MATH_LOG1P 2
+#define ABM_MATH_POW 3
+int abm_math_f32(int fn, const float *x, const float *y, float *out, int64_t n, void *stream);
+int abm_math_f64(int fn, const double *x, double *out, int64_t n, void *stream);
 /* jax.random.permutation(key, n) (jaxabm/analysis.py:87); workspace from
  * abm_permutation_workspace(n). */
 size_t abm_permutation_workspace(int64_t n);

@@ -135,6 +135,8 @@ SIGNATURES = {
     "abm_draw_each": (_int, [_p, _i64, _i64, _int, _f32, _f32, _p, _int, _p]),
     "abm_randint_each": (_int, [_p, _i64, _i64, _i32, _i32, _p, _int, _p]),
     "abm_gamma_each": (_int, [_p, _i64, _i64, _p, _p, _int, _p, _int, _p]),
+    "abm_math_f32": (_int, [_int, _p, _p, _p, _i64, _p]),
+    "abm_math_f64": (_int, [_int, _p, _p, _i64, _p]),
     "abm_permutation_workspace": (_sz, [_i64]),
     "abm_permutation_rounds": (_int, [_i64]),
     "abm_permutation": (_int, [_p, _i64, _p, _p, _sz, _int, _p]),

@@ -11,6 +11,8 @@
 #ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "abm_math.h"
 #endif
 
 #define ABM_HD __host__ __device__ __forceinline__
@@ -130,7 +132,7 @@ __device__ __forceinline__ float mod1(float x) {
 
 // XLA's float32 erf_inv: Giles (2010) single-precision polynomial pair.
 __device__ __forceinline__ float erfinv_f32(float x) {
-  float w = -log1pf(-x * x);
+  float w = -abm_log1pf(-x * x);
   float p;
   if (w < 5.0f) {
     w = w - 2.5f;
@@ -187,7 +189,7 @@ constexpr int kGammaMaxRounds = 4096;
 
 __device__ __forceinline__ bool gamma_reject(float X, float V, float U, float d) {
   const bool squeeze = U >= 1.0f - 0.0331f * (X * X);
-  const bool full = logf(U) >= X * 0.5f + d * ((1.0f - V) + logf(V));
+  const bool full = abm_logf(U) >= X * 0.5f + d * ((1.0f - V) + abm_logf(V));
   return squeeze && full;
 }
 
@@ -221,12 +223,12 @@ __device__ inline float gamma_one(Key key, float alpha, bool log_space, int mode
   }
   const float u = unit_float(bits_scalar(sub, mode));
   if (log_space) {
-    const float ls = log1pf(-u);
+    const float ls = abm_log1pf(-u);
     const float lb = (boost || ls == 0.0f) ? 0.0f : ls * (1.0f / alpha);
-    return (logf(d) + logf(V)) + lb;
+    return (abm_logf(d) + abm_logf(V)) + lb;
   }
   const float s = 1.0f - u;
-  const float b = boost ? 1.0f : powf(s, 1.0f / alpha);
+  const float b = boost ? 1.0f : abm_powf(s, 1.0f / alpha);
   return (d * V) * b;
 }
 
@@ -246,7 +248,7 @@ __device__ inline float beta_draw(Key key, int64_t count, int64_t j, float a, fl
   const float la = gamma_draw(ka, count, j, a, true, mode);
   const float lb = gamma_draw(kb, count, j, b, true, mode);
   const float m = (la != la || lb != lb) ? la + lb : (la > lb ? la : lb);
-  const float ea = expf(la - m), eb = expf(lb - m);
+  const float ea = abm_expf(la - m), eb = abm_expf(lb - m);
   return ea / (ea + eb);
 }
 

@@ -69,7 +69,7 @@ __device__ __forceinline__ void store_cols(float *p, const float (&v)[VEC]) {
 
 __device__ __forceinline__ float gate(float eps, float d) {
   const float z = (eps - fabsf(d)) / kGateScale;
-  const float e = expf(-z);
+  const float e = abm_expf(-z);
   const float den = 1.0f + e;
   return 1.0f / den;
 }

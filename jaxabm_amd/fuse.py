@@ -62,8 +62,10 @@ import torch
 
 _ENABLED = True
 _ARCH = os.environ.get("JAXABM_AMD_ARCH", "gfx950")
-_PRELUDE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc",
-                             "abm_device.hpp")
+_CSRC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc")
+# the generated kernels are compiled from the text of these files, in order
+# (abm_device.hpp includes abm_math.h only when not compiled by hiprtc)
+_PRELUDE_PATHS = (os.path.join(_CSRC, "abm_math.h"), os.path.join(_CSRC, "abm_device.hpp"))
 
 
 def enable(flag: bool = True) -> None:
@@ -1338,8 +1340,11 @@ _prelude_text: Optional[str] = None
 def _prelude() -> str:
     global _prelude_text
     if _prelude_text is None:
-        with open(_PRELUDE_PATH) as f:
-            _prelude_text = _RTC_TYPES + f.read() + _HELPERS
+        parts = []
+        for path in _PRELUDE_PATHS:
+            with open(path) as f:
+                parts.append(f.read())
+        _prelude_text = _RTC_TYPES + "\n".join(parts) + _HELPERS
     return _prelude_text
 
 

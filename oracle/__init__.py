@@ -70,6 +70,8 @@ def _declare(L):
         "orc_random_bits32": (None, [_u32p, _i64, _u32p, C.c_int]),
         "orc_uniform_f32": (None, [_u32p, _i64, C.c_float, C.c_float, _f32p, C.c_int]),
         "orc_randint_i32": (None, [_u32p, _i64, _i32, _i32, _i32p, C.c_int]),
+        "orc_math_f32": (None, [C.c_int, _f32p, C.c_void_p, _f32p, _i64]),
+        "orc_math_f64": (None, [C.c_int, _f64p, _f64p, _i64]),
         "orc_erfinv_f32": (C.c_float, [C.c_float]),
         "orc_normal_f32": (None, [_u32p, _i64, _f32p, C.c_int]),
         "orc_gamma_f32": (None, [_u32p, _i64, _f32p, C.c_int, _f32p, C.c_int]),
@@ -111,6 +113,31 @@ def _declare(L):
 
 def _key(key) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(key, dtype=np.uint32).reshape(2))
+
+
+# ---------------------------------------------------------------- shared math
+MATH_F32 = {"exp": 0, "log": 1, "log1p": 2, "pow": 3}
+MATH_F64 = {"exp": 0, "log": 1}
+
+
+def math_f32(fn: str, x, y=None) -> np.ndarray:
+    """Elementwise float32 exp / log / log1p / pow of abm_math.h."""
+    x = np.ascontiguousarray(x, dtype=np.float32).ravel()
+    out = np.empty_like(x)
+    yp = None
+    if fn == "pow":
+        y = np.ascontiguousarray(np.broadcast_to(np.asarray(y, np.float32), x.shape))
+        yp = y.ctypes.data
+    lib().orc_math_f32(MATH_F32[fn], x, yp, out, x.size)
+    return out
+
+
+def math_f64(fn: str, x) -> np.ndarray:
+    """Elementwise float64 exp / log of abm_math.h."""
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    out = np.empty_like(x)
+    lib().orc_math_f64(MATH_F64[fn], x, out, x.size)
+    return out
 
 
 # ---------------------------------------------------------------- PRNG

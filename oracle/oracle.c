@@ -22,13 +22,22 @@
  *                          h = 1e-3 (the gate's curvature separates the
  *                          secant from the tangent at h = 1e-3)
  *
+ *   exp / log / log1p / pow -- the shared IEEE-basic-operation functions of
+ *                          jaxabm_amd/csrc/abm_math.h (also compiled into the
+ *                          device kernels, so both compute the same bits),
+ *                          pinned against float64 numpy to 1 ulp and by
+ *                          identities in tests/test_oracle_math.py
+ *
  * Parity unpinned (no JAX in this environment to produce vectors): XLA:CPU's
- * rounding of log1p / sqrt inside erf_inv and of exp in the sigmoid gate
- * (tolerance-level: float32 results compared at 1e-5 relative), and
- * end-to-end trajectories under an installed JAX version beyond the
- * documented key and draw values above.
+ * own rounding of log1p inside erf_inv, of exp in the sigmoid gate and of
+ * log / pow / exp in the gamma sampler (this restatement and the device use
+ * abm_math.h, accurate to about one ulp, where XLA uses its own
+ * approximations), and end-to-end trajectories under an installed JAX
+ * version beyond the documented key and draw values above.
  */
 #include "oracle.h"
+
+#include "abm_math.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -176,11 +185,28 @@ void orc_randint_i32(const uint32_t key[2], int64_t n, int32_t lo, int32_t hi,
   free(lb);
 }
 
+/* The transcendental functions the float32 programs use, from abm_math.h,
+ * elementwise (pinned against float64 numpy in tests/test_oracle_math.py). */
+void orc_math_f32(int fn, const float *x, const float *y, float *out, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) {
+    switch (fn) {
+      case 0: out[i] = abm_expf(x[i]); break;
+      case 1: out[i] = abm_logf(x[i]); break;
+      case 2: out[i] = abm_log1pf(x[i]); break;
+      default: out[i] = abm_powf(x[i], y[i]); break;
+    }
+  }
+}
+
+void orc_math_f64(int fn, const double *x, double *out, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) out[i] = fn == 0 ? abm_exp(x[i]) : abm_log(x[i]);
+}
+
 /* erf_inv for float32: M. Giles, "Approximating the erfinv function", GPU
  * Computing Gems Jade Edition (2010), single-precision branch pair; the form
  * XLA lowers lax.erf_inv to for f32. */
 float orc_erfinv_f32(float x) {
-  float w = -log1pf(-x * x);
+  float w = -abm_log1pf(-x * x);
   float p;
   if (w < 5.0f) {
     w = w - 2.5f;
@@ -251,8 +277,8 @@ static float orc_gamma_one(const uint32_t key[2], float alpha, int log_space, in
   float X = 0.0f, V = 1.0f, U = 2.0f;
   for (;;) {
     const float squeeze = 1.0f - 0.0331f * (X * X);
-    const float rhs = X * 0.5f + d * ((1.0f - V) + logf(V));
-    if (!(U >= squeeze && logf(U) >= rhs)) break;
+    const float rhs = X * 0.5f + d * ((1.0f - V) + abm_logf(V));
+    if (!(U >= squeeze && abm_logf(U) >= rhs)) break;
     uint32_t k3[6], xk[2], uk[2];
     orc_split(k, 3, k3, mode);
     k[0] = k3[0]; k[1] = k3[1];
@@ -273,12 +299,12 @@ static float orc_gamma_one(const uint32_t key[2], float alpha, int log_space, in
   float u;
   orc_uniform_f32(sub, 1, 0.0f, 1.0f, &u, mode);
   if (log_space) {
-    const float ls = log1pf(-u);
+    const float ls = abm_log1pf(-u);
     const float lb = (boost || ls == 0.0f) ? 0.0f : ls * (1.0f / alpha);
-    return (logf(d) + logf(V)) + lb;
+    return (abm_logf(d) + abm_logf(V)) + lb;
   }
   const float s = 1.0f - u;
-  const float b = boost ? 1.0f : powf(s, 1.0f / alpha);
+  const float b = boost ? 1.0f : abm_powf(s, 1.0f / alpha);
   return (d * V) * b;
 }
 
@@ -300,7 +326,7 @@ void orc_beta_f32(const uint32_t key[2], int64_t count, const float *a, const fl
   orc_gamma_f32(kab + 2, count, b, 1, lb, mode);
   for (int64_t j = 0; j < count; ++j) {
     const float m = (isnan(la[j]) || isnan(lb[j])) ? la[j] + lb[j] : (la[j] > lb[j] ? la[j] : lb[j]);
-    const float ea = expf(la[j] - m), eb = expf(lb[j] - m);
+    const float ea = abm_expf(la[j] - m), eb = abm_expf(lb[j] - m);
     out[j] = ea / (ea + eb);
   }
   free(la);
@@ -848,7 +874,7 @@ void orc_deffuant_init(int64_t n, int32_t batch, int64_t seed, int mode,
 }
 
 static inline float sigmoidf_ref(float z) {
-  volatile float e = expf(-z);
+  volatile float e = abm_expf(-z);
   volatile float d = 1.0f + e;
   return 1.0f / d;
 }

@@ -45,6 +45,11 @@ void orc_uniform_f32(const uint32_t key[2], int64_t n, float lo, float hi,
                      float *out, int mode);
 void orc_randint_i32(const uint32_t key[2], int64_t n, int32_t lo, int32_t hi,
                      int32_t *out, int mode);
+/* Elementwise float32 exp (fn 0), log (1), log1p (2) and pow(x, y) (3), and
+ * float64 exp (0) and log (1), from the shared abm_math.h (y may be NULL
+ * unless fn is pow). */
+void orc_math_f32(int fn, const float *x, const float *y, float *out, int64_t n);
+void orc_math_f64(int fn, const double *x, double *out, int64_t n);
 float orc_erfinv_f32(float x);
 void orc_normal_f32(const uint32_t key[2], int64_t n, float *out, int mode);
 /* gamma(key, alpha[j], (count,)) (log_space: loggamma) and

@@ -338,9 +338,8 @@ class Trader:
 @pytest.mark.parametrize("fused,graph", [(True, True), (False, False)])
 def test_generic_trader_matches_oracle(J, capsys, fused, graph):
     """The generic workload's model (10,000 agents, 200 steps, an env phase)
-    against the oracle's restatement: wealth exactly; positions to float32
-    rounding of the normal draw (distance on the unit circle, as mod 1
-    wraps), in the graph-replayed fused run and the eager vmapped one."""
+    against the oracle's restatement, bit for bit (wealth, positions and the
+    final key), in the graph-replayed fused run and the eager vmapped one."""
     import oracle as O
     from jaxabm_amd import fuse
     from jaxabm_amd import numpy as jnp
@@ -360,7 +359,6 @@ def test_generic_trader_matches_oracle(J, capsys, fused, graph):
     O.trader_run(x, w, rng, steps, O.MODE_PARTITIONABLE)
     st = m.agent_collections["t"].states
     np.testing.assert_array_equal(st["w"].cpu().numpy(), w)
-    d = np.abs(st["x"].cpu().numpy().astype(np.float64) - x)
-    assert np.minimum(d, 1.0 - d).max() < 1e-5
+    np.testing.assert_array_equal(st["x"].cpu().numpy(), x)
     assert int(res["total_w"][-1]) == int(w.sum())
     np.testing.assert_array_equal(np.asarray(m._rng, np.uint32), rng)

@@ -248,6 +248,65 @@ def test_deffuant_dense_step_and_conservation():
     np.testing.assert_allclose(x.sum(0), x0.sum(0), rtol=1e-5)
 
 
+@pytest.mark.parametrize("mode", [O.MODE_ORIGINAL, O.MODE_PARTITIONABLE])
+@pytest.mark.parametrize("n,seed", [(1000, 0), (7, 3), (1, 5)])
+def test_deffuant_init_is_uniform_of_split_agent_keys(mode, n, seed):
+    """orc_deffuant_init composes pinned primitives as the reference does:
+    JaxModel.initialize splits PRNGKey(seed) into (rng, collection key)
+    (jaxabm/model.py:118-144), AgentCollection.init splits the collection key
+    into N agent keys (jaxabm/agent.py:92-130) and each opinion is
+    uniform(agent key); every instance column starts from the same opinions."""
+    x, rng = O.deffuant_init(n, 3, seed, mode)
+    rk, ck = O.split(O.prng_key(seed), 2, mode)
+    ak = O.split(ck, n, mode)
+    want = np.array([O.uniform(k, 1, mode=mode)[0] for k in ak], np.float32)
+    for b in range(3):
+        np.testing.assert_array_equal(x[:, b], want)
+    np.testing.assert_array_equal(rng, rk)
+
+
+def _expit(z):
+    return 0.5 * (1.0 + np.tanh(0.5 * z))   # logistic function, numerically stable
+
+
+def test_deffuant_stats_against_numpy_and_closed_forms():
+    """orc_deffuant_stats = [mean, population variance, 32-bin soft histogram
+    with slope 128]: mean and variance against numpy; the histogram against
+    its telescoping sum, the reflection x -> 1 - x (bins reversed), a single
+    agent at a bin centre (closed form tanh(1) in its bin), and a shift of
+    all opinions by exactly one bin width (bins move by one)."""
+    rs = np.random.default_rng(5)
+    x = rs.uniform(0, 1, (4001, 3)).astype(np.float32)
+    st = O.deffuant_stats(x)
+    x64 = x.astype(np.float64)
+    np.testing.assert_allclose(st[:, 0], x64.mean(0), rtol=1e-13)
+    np.testing.assert_allclose(st[:, 1], x64.var(0), rtol=1e-11)
+    h = st[:, 2:]
+    assert h.shape == (3, 32) and (h >= 0).all()
+    # sum_k [s(128(x - k/32)) - s(128(x - (k+1)/32))] telescopes
+    tel = (_expit(128.0 * x64) - _expit(128.0 * (x64 - 1.0))).mean(0)
+    np.testing.assert_allclose(h.sum(1), tel, rtol=1e-12)
+    # reflection: bin k of 1 - x is bin 31 - k of x (x and 1 - x exact in f32
+    # only for dyadic values: use multiples of 2^-12)
+    q = (rs.integers(0, 4097, (2048, 1)) / 4096.0).astype(np.float32)
+    hq = O.deffuant_stats(q)[0, 2:]
+    hr = O.deffuant_stats((1.0 - q).astype(np.float32))[0, 2:]
+    np.testing.assert_allclose(hr, hq[::-1], rtol=1e-12, atol=1e-15)
+    # one agent at the centre of bin k: sigma(2) - sigma(-2) = tanh(1)
+    for k in (0, 13, 31):
+        c = np.array([[(k + 0.5) / 32.0]], np.float32)
+        hk = O.deffuant_stats(c)[0]
+        assert hk[0] == np.float64(c[0, 0]) and hk[1] == 0.0
+        assert abs(hk[2 + k] - np.tanh(1.0)) < 1e-14
+        if k > 0:   # neighbouring bin: sigma(6) - sigma(2)
+            assert abs(hk[2 + k - 1] - (_expit(6.0) - _expit(2.0))) < 1e-14
+    # a shift by one bin width (dyadic, exact) moves every bin by one
+    base = (rs.integers(512, 3584, (1500, 1)) / 4096.0).astype(np.float32)
+    h0 = O.deffuant_stats(base)[0, 2:]
+    h1 = O.deffuant_stats((base + np.float32(1.0 / 32)).astype(np.float32))[0, 2:]
+    np.testing.assert_allclose(h1[1:], h0[:-1], rtol=1e-12, atol=1e-15)
+
+
 def test_deffuant_adjoint_matches_finite_differences():
     n, steps = 512, 40
     x0, rng0 = O.deffuant_init(n, 1, 0, O.MODE_PARTITIONABLE)
