@@ -421,6 +421,16 @@ int abm_mean_i32(const int32_t *x, int64_t n, float *out, void *ws, size_t ws_by
  * int32 (wrapping), G = f32(2S) / f32(nT) - f32((n+1)/n), 0 when T <= 1e-6. */
 int abm_gini_i32(const int32_t *x, int64_t n, float *out, void *ws, size_t ws_bytes,
                  void *stream);
+/* abm_gini_i32 without a workspace and without any allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Its scratch is a 5.25 MiB device array
+ * of the library, one copy per GPU held until the process ends, so n is at
+ * most ABM_GINI_NOALLOC_MAX_N (larger n is refused with an error).  Every
+ * call on a GPU uses that GPU's one copy: calls that overlap on one GPU (two
+ * streams, two threads) race and must be ordered by the caller. */
+#define ABM_GINI_NOALLOC_MAX_N 262144
+int abm_gini_i32_noalloc(const int32_t *x, int64_t n, float *out, void *stream);
 /* out[k] (float64, k < bins <= 256) = mean over i of
  * sigmoid(slope (x_i - k/bins)) - sigmoid(slope (x_i - (k+1)/bins)). */
 int abm_soft_hist_f32(const float *x, int64_t n, int32_t bins, double slope, double *out,

@@ -22,6 +22,8 @@
 //  hist    float64, per-block partials added in a fixed order
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "common.hpp"
 #include "jaxabm_amd.h"
 
@@ -194,6 +196,36 @@ __global__ void k_soft_hist_final(const double *__restrict__ partial, int nb, in
   out[k] = s / (double)n;
 }
 
+// ---- scratch of abm_gini_i32_noalloc: a file-scope device array, one copy
+// per GPU for the life of the process, sized for ABM_GINI_NOALLOC_MAX_N
+// values (abm_reduce_workspace of that n, checked at set-up).  Host code
+// never uses the array's name as an address: the address of each device's
+// copy comes from hipGetSymbolAddress, taken once per device with that
+// device current.
+constexpr size_t kNoAllocScratchBytes = 5505024;  // 5.25 MiB
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices];
+
+// The current device's copy of `scratch` (set up on its first use), or an
+// error status.
+int noalloc_scratch(void **out) {
+  int dev = -1;
+  ABM_CHECK_HIP(hipGetDevice(&dev));
+  ABM_REQUIRE(dev >= 0 && dev < kMaxDevices, "abm_gini_i32_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    ABM_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ABM_REQUIRE(p && ((uintptr_t)p & 255) == 0, "abm_gini_i32_noalloc: scratch misaligned");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return kOk;
+}
+
 }  // namespace abm
 
 using namespace abm;
@@ -271,6 +303,20 @@ int abm_gini_i32(const int32_t *x, int64_t n, float *out, void *ws, size_t ws_by
   hipLaunchKernelGGL(k_gini_final, dim3(1), dim3(1), 0, s, (const uint32_t *)st, n, out);
   ABM_LAUNCH_CHECK("abm_gini_i32");
   return kOk;
+}
+
+int abm_gini_i32_noalloc(const int32_t *x, int64_t n, float *out, void *stream) {
+  ABM_REQUIRE(x && out && n >= 1, "abm_gini_i32_noalloc: bad arguments");
+  ABM_REQUIRE(n <= ABM_GINI_NOALLOC_MAX_N,
+              "abm_gini_i32_noalloc: n exceeds ABM_GINI_NOALLOC_MAX_N (262144); use abm_gini_i32 "
+              "with a workspace");
+  const size_t need = abm_reduce_workspace(ABM_GINI_NOALLOC_MAX_N);
+  ABM_REQUIRE(need + 256 <= kNoAllocScratchBytes, "abm_gini_i32_noalloc: scratch too small");
+  void *ws = nullptr;
+  if (int rc = noalloc_scratch(&ws)) return rc;
+  // the same kernels as abm_gini_i32; its counters are cleared by a fill
+  // kernel inside, so nothing carries over from the previous call
+  return abm_gini_i32(x, n, out, ws, kNoAllocScratchBytes, stream);
 }
 
 int abm_soft_hist_f32(const float *x, int64_t n, int32_t bins, double slope, double *out,
