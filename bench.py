@@ -16,6 +16,9 @@ Also reported:
     HIP-event time of the timed graph runs on their stream divided by the
     number of step kernels, against MI355X's 8 TB/s;
   * an upper bound of the data-dependent bytes the skipping kernel touches;
+  * parity: the graph's CSR checksum and the recorded S/I/R rows must equal
+    the committed fixture (tests/golden/sir_ba1M_m4_365steps.json: the oracle's
+    run on the same graph), or nothing is reported and the exit status is 3;
   * cpu_baseline: the C oracle (a restatement of the reference semantics, one
     thread) running the same full simulation on the host;
   * serving: the batched-serving workload (seeded trace of wealth-exchange and
@@ -23,7 +26,11 @@ Also reported:
     Poisson arrivals at 200/s) through the continuous-batching server against
     the same server running one request at a time: saturated sims/s of each,
     arrival-to-result p50/p99 at 200 requests/s, and whether every request's
-    records are identical between the two.
+    records are identical between the two;
+  * wealth, schelling, deffuant: the other programs' lines (1,000-agent
+    100-step wealth exchange, 512 x 512 200-step Schelling, 65,536-agent x
+    256-instance 200-step Deffuant gradient evaluation), each with its
+    cpu_baseline and roofline.
 """
 from __future__ import annotations
 
@@ -572,55 +579,37 @@ def serving_summary(n_requests: int) -> dict:
     }
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser(description=__doc__)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20, help="timed full simulations")
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-runs", type=int, default=5,
-                    help="timed oracle runs per cpu_baseline variant (0 = skip)")
-    ap.add_argument("--no-graph", action="store_true", help="launch kernels directly instead of a hipGraph")
-    ap.add_argument("--serving-requests", type=int, default=400,
-                    help="requests in the serving trace (0 = skip the serving measurement)")
-    ap.add_argument("--workload", default="sir",
-                    choices=["sir", "deffuant", "wealth", "schelling", "generic"],
-                    help="sir (default): the headline line; the others measure the other "
-                         "programs at their configured sizes")
-    ap.add_argument("--pmc-probe", action="store_true",
-                    help="run a 256 MiB device copy and one un-captured SIR simulation, or with "
-                         "--workload deffuant one gradient evaluation (for rocprofv3 --pmc "
-                         "FETCH_SIZE / WRITE_SIZE) and exit")
-    args = ap.parse_args()
+def sir_fixture() -> dict:
+    """The committed checksum of the measured graph and the oracle's S/I/R
+    rows of the measured run (tests/golden/sir_ba1M_m4_365steps.json)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                        "sir_ba1M_m4_365steps.json")
+    with open(path) as f:
+        return json.load(f)
 
-    if args.gpus != 1:
-        print(f"bench.py: the engine is measured on one GPU only (asked for {args.gpus}); "
-              "multi-GPU sharding is a design note, not a measured path.", file=sys.stderr)
-        return 2
-    if args.pmc_probe:
-        return pmc_probe(args.workload)
-    if args.workload != "sir":
-        import torch
 
-        if not torch.cuda.is_available():
-            print("bench.py: no ROCm GPU visible", file=sys.stderr)
-            return 2
-        fn = {"deffuant": bench_deffuant, "wealth": bench_wealth,
-              "schelling": bench_schelling, "generic": bench_generic}[args.workload]
-        print(json.dumps(fn(args)))
-        return 0
+class ParityError(RuntimeError):
+    """The measured workload is not the parity-tested one."""
 
+
+def bench_sir(args) -> dict:
+    """The headline: 1M-agent BA SIR, 365 steps, one hipGraph replay per
+    bench step.  The graph must have the committed checksum and the recorded
+    S/I/R rows must equal the oracle's (both from the committed fixture), or
+    ParityError is raised before anything is reported."""
     import numpy as np
     import torch
 
     from jaxabm_amd._lib import check, lib, ptr
-    from jaxabm_amd.network import ba_csr
+    from jaxabm_amd.network import ba_csr, csr_checksum
     from jaxabm_amd.programs.sir import SIRNetworkEngine
 
-    if not torch.cuda.is_available():
-        print("bench.py: no ROCm GPU visible", file=sys.stderr)
-        return 2
-
+    gold = sir_fixture()
     row_ptr, col = ba_csr(N_AGENTS, BA_M, GRAPH_SEED)
+    digest = csr_checksum(row_ptr, col)
+    if digest != gold["csr_sha256"]:
+        raise ParityError(f"BA({N_AGENTS}, {BA_M}) CSR checksum {digest} differs from the "
+                          f"committed {gold['csr_sha256']}")
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
         eng = SIRNetworkEngine(N_AGENTS, csr=(row_ptr, col), beta=BETA, gamma=GAMMA,
@@ -661,6 +650,11 @@ def main() -> int:
         one_run()
     stream.synchronize()
     ref_counts = counts.cpu().numpy().copy()
+    want = np.array(gold["counts"]["partitionable"], np.int32)
+    if not np.array_equal(ref_counts, want):
+        bad = int(np.argmax((ref_counts != want).any(axis=1)))
+        raise ParityError(f"S/I/R rows differ from the oracle's from step {bad}: "
+                          f"{ref_counts[bad].tolist()} vs {want[bad].tolist()}")
 
     # ---- timed region: K complete simulations
     torch.cuda.synchronize()
@@ -677,16 +671,60 @@ def main() -> int:
     value = N_AGENTS * SIM_STEPS * args.steps / elapsed
 
     # every timed run recomputes the same simulation
-    assert np.array_equal(counts.cpu().numpy(), ref_counts)
-    assert (ref_counts.sum(axis=1) == N_AGENTS).all()
+    if not np.array_equal(counts.cpu().numpy(), ref_counts):
+        raise ParityError("a timed replay recorded different S/I/R rows")
 
-    # ---- step-kernel average: the event-timed graph runs hold one reset copy
-    # pair, one slot clear and one count reduction besides the 365 step
+    # ---- step-kernel average: the event-timed graph runs hold the reset
+    # copies, one slot clear and one count reduction besides the 365 step
     # kernels; all of it is charged to the step kernel (conservative).
     avg_kernel_s = gpu_s / (args.steps * SIM_STEPS)
+    nnz = int(row_ptr[-1])
+    algo = algorithmic_bytes(N_AGENTS, nnz)
+    achieved = algo / avg_kernel_s / 1e9
+    out = {
+        "metric": "agent-steps/s (1M-agent SIR)",
+        "value": value,
+        "unit": "agent-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded Barabasi-Albert graph and seeded infections)",
+        "config": {
+            "workload": "network_sir_ba1M_m4_365steps",
+            "n_agents": N_AGENTS, "ba_m": BA_M, "undirected_edges": nnz // 2, "csr_nnz": nnz,
+            "sim_steps": SIM_STEPS, "beta": BETA, "gamma": GAMMA, "seed_infected": N_SEED,
+            "rng": "jax threefry2x32, partitionable stream", "hipgraph": graph is not None,
+            "bench_step": "one full 365-step simulation",
+        },
+        "graph_checksum_ok": True,
+        "parity": {"csr_sha256": digest, "graph_checksum_ok": True,
+                   "counts_equal_oracle": True,
+                   "fixture": "tests/golden/sir_ba1M_m4_365steps.json"},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "k_sir_step",
+            "traffic_source": None,
+            "algorithmic_bytes_per_launch": algo,
+            "avg_kernel_us": avg_kernel_s * 1e6,
+        },
+        "final_counts": ref_counts[-1].tolist(),
+    }
+    if graph is not None:
+        lib().abm_graph_destroy(graph)
+    if args.profile_run:
+        return out
 
     # ---- data-dependent bytes: one instrumented step-by-step replay
-    nnz = int(row_ptr[-1])
     deg = torch.from_numpy(np.diff(row_ptr).astype(np.int64)).to("cuda")
     idx = torch.arange(N_AGENTS, device="cuda")
     with torch.cuda.stream(stream):
@@ -716,47 +754,12 @@ def main() -> int:
         c1 = torch.empty((SIM_STEPS, 3), dtype=torch.int32, device="cuda")
         check(lib().abm_sir_reduce_counts(ptr(sl1), SIM_STEPS, ptr(c1), sh), "reduce")
     stream.synchronize()
-    assert np.array_equal(c1.cpu().numpy(), ref_counts)
-    algo = algorithmic_bytes(N_AGENTS, nnz)
-    achieved = algo / avg_kernel_s / 1e9
+    if not np.array_equal(c1.cpu().numpy(), ref_counts):
+        raise ParityError("the step-by-step replay recorded different S/I/R rows")
     dd_avg = float(torch.stack(dd_bytes).double().mean().item())
-
-    out = {
-        "metric": "agent-steps/s (1M-agent SIR)",
-        "value": value,
-        "unit": "agent-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u8",
-        "data": "synthetic (seeded Barabasi-Albert graph and seeded infections)",
-        "config": {
-            "workload": "network_sir_ba1M_m4_365steps",
-            "n_agents": N_AGENTS, "ba_m": BA_M, "undirected_edges": nnz // 2, "csr_nnz": nnz,
-            "sim_steps": SIM_STEPS, "beta": BETA, "gamma": GAMMA, "seed_infected": N_SEED,
-            "rng": "jax threefry2x32, partitionable stream", "hipgraph": graph is not None,
-            "bench_step": "one full 365-step simulation",
-        },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "k_sir_step",
-            "traffic_source": None,
-            "algorithmic_bytes_per_launch": algo,
-            "avg_kernel_us": avg_kernel_s * 1e6,
-            "data_dependent_bytes_per_launch": dd_avg,
-            "data_dependent_GBs": dd_avg / avg_kernel_s / 1e9,
-        },
-        "final_counts": ref_counts[-1].tolist(),
-    }
+    out["roofline"]["data_dependent_bytes_per_launch"] = dd_avg
+    out["roofline"]["data_dependent_GBs"] = dd_avg / avg_kernel_s / 1e9
+    out["roofline"]["data_dependent_frac"] = dd_avg / avg_kernel_s / 1e9 / HBM_PEAK_GBS
     traffic = measured_traffic()
     if traffic is not None:
         out["roofline"]["traffic"] = traffic["bytes_per_launch"]
@@ -764,10 +767,68 @@ def main() -> int:
     out["roofline"]["copy_reference"] = copy_reference_gbs(torch.cuda.current_stream())
     if args.cpu_runs > 0:
         out["cpu_baseline"] = cpu_baseline(row_ptr, col, args.cpu_runs)
-    if args.serving_requests > 0:
-        out["serving"] = serving_summary(args.serving_requests)
-    if graph is not None:
-        lib().abm_graph_destroy(graph)
+    return out
+
+
+_SUB_DROP = ("n_gpus", "higher_is_better", "scaling", "vs_baseline", "steps", "warmup")
+
+
+def sub_line(fn, args) -> dict:
+    """Another program's bench line, embedded in the headline line."""
+    d = fn(args)
+    return {k: v for k, v in d.items() if k not in _SUB_DROP}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20, help="timed full simulations")
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-runs", type=int, default=5,
+                    help="timed oracle runs per cpu_baseline variant (0 = skip)")
+    ap.add_argument("--no-graph", action="store_true", help="launch kernels directly instead of a hipGraph")
+    ap.add_argument("--serving-requests", type=int, default=400,
+                    help="requests in the serving trace (0 = skip the serving measurement)")
+    ap.add_argument("--workload", default="sir",
+                    choices=["sir", "deffuant", "wealth", "schelling", "generic"],
+                    help="sir (default): the headline line, with the wealth, Schelling and "
+                         "Deffuant programs embedded; the others print one program's line")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="the SIR graph replays only (warm-up and timed), for rocprofv3: no "
+                         "step-by-step replay, no other programs, no CPU baseline, no serving")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="run a 256 MiB device copy and one un-captured SIR simulation, or with "
+                         "--workload deffuant one gradient evaluation (for rocprofv3 --pmc "
+                         "FETCH_SIZE / WRITE_SIZE) and exit")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: the engine is measured on one GPU only (asked for {args.gpus}); "
+              "multi-GPU sharding is a design note, not a measured path.", file=sys.stderr)
+        return 2
+    if args.pmc_probe:
+        return pmc_probe(args.workload)
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py: no ROCm GPU visible", file=sys.stderr)
+        return 2
+    if args.workload != "sir":
+        fn = {"deffuant": bench_deffuant, "wealth": bench_wealth,
+              "schelling": bench_schelling, "generic": bench_generic}[args.workload]
+        print(json.dumps(fn(args)))
+        return 0
+    try:
+        out = bench_sir(args)
+    except ParityError as e:
+        print(f"bench.py: parity check failed, no number reported: {e}", file=sys.stderr)
+        return 3
+    if not args.profile_run:
+        if args.serving_requests > 0:
+            out["serving"] = serving_summary(args.serving_requests)
+        out["wealth"] = sub_line(bench_wealth, args)
+        out["schelling"] = sub_line(bench_schelling, args)
+        out["deffuant"] = sub_line(bench_deffuant, args)
     print(json.dumps(out))
     return 0
 

@@ -86,6 +86,17 @@ def ba_csr(n: int, m: int, graph_seed: int) -> tuple[np.ndarray, np.ndarray]:
     return csr_undirected(n, src, dst)
 
 
+def csr_checksum(row_ptr: np.ndarray, col: np.ndarray) -> str:
+    """SHA-256 of the CSR arrays as little-endian int32 (row_ptr, then col):
+    names a graph without shipping its arrays."""
+    import hashlib
+
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(row_ptr, "<i4").tobytes())
+    h.update(np.ascontiguousarray(col, "<i4").tobytes())
+    return h.hexdigest()
+
+
 def csr_gather_sum(g: DeviceCSR, x: torch.Tensor) -> torch.Tensor:
     """out[v] = sum of x[u] over the neighbours u of v (float32 result,
     float64 accumulation)."""

@@ -57,6 +57,23 @@ def test_network_builders_match_oracle(n, m, seed):
     np.testing.assert_array_equal(col, col2)
 
 
+def test_network_builder_at_the_measured_size_matches_oracle_and_checksum():
+    """BA(1,000,000, m = 4, graph seed 0), the measured SIR graph: the
+    product's host builder equals the oracle's builder entry for entry, and
+    both match the committed checksum that bench.py asserts."""
+    import json
+
+    from jaxabm_amd import network as N
+
+    gold = json.load(open(os.path.join(REPO, "tests", "golden", "sir_ba1M_m4_365steps.json")))
+    rp, col = N.ba_csr(1_000_000, 4, 0)
+    orp, ocol = O.ba_csr(1_000_000, 4, 0)
+    np.testing.assert_array_equal(rp, orp)
+    np.testing.assert_array_equal(col, ocol)
+    assert int(rp[-1]) == gold["csr_nnz"] == 2 * 4 * (1_000_000 - 4)
+    assert N.csr_checksum(rp, col) == gold["csr_sha256"]
+
+
 def test_csr_rejects_out_of_range_edges():
     from jaxabm_amd import network as N
     from jaxabm_amd._lib import AbmError

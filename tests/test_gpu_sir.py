@@ -89,17 +89,29 @@ def test_sir_full_size_matches_oracle(sir, mode):
     agents of each step, the same results as its serial path): every step's
     S/I/R counts and the final state; plus the invariants S + I + R = N, S
     never grows, R never shrinks."""
+    import json
     import os
 
+    from jaxabm_amd import network as N
+
     n, m = 1_000_000, 4
-    row_ptr, col = O.ba_csr(n, m, graph_seed=0)
+    # the graph the bench measures: the product's builder, checksummed
+    row_ptr, col = N.ba_csr(n, m, graph_seed=0)
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                       "sir_ba1M_m4_365steps.json")))
+    assert N.csr_checksum(row_ptr, col) == gold["csr_sha256"]
+    orp, ocol = O.ba_csr(n, m, graph_seed=0)
+    np.testing.assert_array_equal(row_ptr, orp)
+    np.testing.assert_array_equal(col, ocol)
     assert row_ptr[-1] == 2 * m * (n - m)
     eng = sir.SIRNetworkEngine(n, csr=(row_ptr, col), seed=0, mode=mode)
     st, rng = O.sir_init(n, 100, 0, mode)
     threads = max(1, min(16, os.cpu_count() or 1))
-    want = O.sir_run_threads(row_ptr, col, st, rng, 0.05, 0.1, 365, threads, mode)
+    want = O.sir_run_threads(orp, ocol, st, rng, 0.05, 0.1, 365, threads, mode)
     got = np.concatenate([eng.run(12), eng.run(365 - 12)])   # a run continues the state
     np.testing.assert_array_equal(got, want)
+    name = "partitionable" if mode == O.MODE_PARTITIONABLE else "original"
+    np.testing.assert_array_equal(got, np.array(gold["counts"][name], np.int32))
     np.testing.assert_array_equal(eng.state_host(), st)
     assert (got.sum(axis=1) == n).all()
     assert (np.diff(got[:, 0]) <= 0).all()   # S never grows

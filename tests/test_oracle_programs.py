@@ -336,6 +336,29 @@ def test_deffuant_adjoint_matches_finite_differences():
     assert l0[0] < 1e-10
 
 
+def test_sir_fixture_is_the_oracle_run():
+    """tests/golden/sir_ba1M_m4_365steps.json (written by make_sir_ba1M.py,
+    asserted by bench.py) is the oracle's 365-step run on BA(1M, m=4) in both
+    stream modes, and satisfies S + I + R = N, S non-increasing, R
+    non-decreasing."""
+    import json
+    import os
+
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                       "sir_ba1M_m4_365steps.json")))
+    n = gold["n"]
+    rp, col = O.ba_csr(n, gold["m"], gold["graph_seed"])
+    for mode, name in ((O.MODE_PARTITIONABLE, "partitionable"), (O.MODE_ORIGINAL, "original")):
+        want = np.array(gold["counts"][name], np.int32)
+        assert want.shape == (365, 3)
+        assert (want.sum(1) == n).all()
+        assert (np.diff(want[:, 0]) <= 0).all() and (np.diff(want[:, 2]) >= 0).all()
+        st, rng = O.sir_init(n, gold["seed_infected"], gold["seed"], mode)
+        got = O.sir_run_threads(rp, col, st, rng, np.float32(gold["beta"]),
+                                np.float32(gold["gamma"]), gold["steps"], os.cpu_count() or 1, mode)
+        np.testing.assert_array_equal(got, want)
+
+
 def test_sir_threaded_run_is_identical():
     """The all-cores CPU baseline computes the same simulation: each agent's
     next state depends only on the pre-step copy."""
