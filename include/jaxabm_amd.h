@@ -171,7 +171,11 @@ int64_t abm_sir_bitmap_words(int64_t n);
  * plus one flag slot; callers zero it and never read it directly). */
 int abm_sir_slots(void);
 /* Initial state at step t0: all S with heavy-row flags, agents perm[0..n0)
- * infected, marks and bitmap (buffer 0) built from it. */
+ * infected, marks and bitmap (buffer 0) built from it.  Also validates the
+ * chunk table (read back once, synchronising the stream): 0 .. n ascending,
+ * inner boundaries multiples of 64, chunks of at most 8192 agents; a graph
+ * whose table fails is refused, and the step entry points below assume a
+ * graph that abm_sir_seed accepted. */
 int abm_sir_seed(const abm_sir_graph *g, const int32_t *perm, int64_t n0, uint8_t *state,
                  uint16_t *mark, uint32_t *bitmap, uint32_t t0, void *stream);
 /* Runs `steps` synchronous steps.  keys: device [steps][4] = (k_inf, k_rec)

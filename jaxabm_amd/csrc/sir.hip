@@ -49,6 +49,9 @@
 // col int32[nnz] (rows ascending), chunk_ptr int32[n_chunks+1].
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <vector>
+
 #include "common.hpp"
 #include "jaxabm_amd.h"
 
@@ -547,6 +550,51 @@ int check_graph(const abm_sir_graph *g, const char *who) {
   return kOk;
 }
 
+// The chunk table the step kernel assumes: chunk_ptr[0] = 0, chunk_ptr
+// [n_chunks] = n, ascending, every inner boundary a multiple of kChunkAlign
+// (each bitmap byte and 8-agent state word then has one owning block) and no
+// chunk longer than one tile.  Read back once per instance set-up.
+int check_chunks(const abm_sir_graph *g, hipStream_t s) {
+  std::vector<int32_t> cp((size_t)g->n_chunks + 1);
+  ABM_CHECK_HIP(hipMemcpyAsync(cp.data(), g->chunk_ptr, cp.size() * sizeof(int32_t),
+                               hipMemcpyDeviceToHost, s));
+  ABM_CHECK_HIP(hipStreamSynchronize(s));
+  bool ok = cp.front() == 0 && cp.back() == g->n;
+  for (size_t i = 1; ok && i < cp.size(); ++i) {
+    ok = cp[i] > cp[i - 1] && cp[i] - cp[i - 1] <= kTile;
+    if (i + 1 < cp.size()) ok = ok && cp[i] % kChunkAlign == 0;
+  }
+  if (!ok) {
+    set_error("abm_sir_seed: chunk_ptr must run from 0 to n, ascending, with inner boundaries "
+              "multiples of 64 and chunks of at most 8192 agents (abm_node_chunks with "
+              "abm_sir_chunk_params)");
+    return kErrInvalid;
+  }
+  return kOk;
+}
+
+// The bitmap-staging kernels need more than 64 KB of dynamic LDS: the limit
+// is raised once per device (under a lock; each device keeps its own
+// function attributes).
+constexpr int kMaxDevices = 64;
+std::mutex g_lds_attr_mu;
+bool g_lds_attr_set[kMaxDevices];
+
+int sir_lds_attr() {
+  int dev = -1;
+  ABM_CHECK_HIP(hipGetDevice(&dev));
+  ABM_REQUIRE(dev >= 0 && dev < kMaxDevices, "sir: device index out of range");
+  std::lock_guard<std::mutex> lock(g_lds_attr_mu);
+  if (!g_lds_attr_set[dev]) {
+    for (const void *k : {(const void *)k_sir_step<true, kModePartitionable>,
+                          (const void *)k_sir_step<true, kModeOriginal>})
+      ABM_CHECK_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)(kLdsBitmapWords * 4)));
+    g_lds_attr_set[dev] = true;
+  }
+  return kOk;
+}
+
 int check_aligned(const void *p, uintptr_t bytes, const char *who, const char *what) {
   if (((uintptr_t)p & (bytes - 1)) != 0) {
     set_error(std::string(who) + ": " + what + " must be " + std::to_string(bytes) +
@@ -569,7 +617,6 @@ int launch_step(const abm_sir_graph *g, uint8_t *state, uint16_t *mark,
                 const uint32_t *keys4, uint32_t tb, uint32_t tg, int mode,
                 const int32_t *prev_slots, const int32_t *prev2_slots, int32_t *slots,
                 hipStream_t s) {
-  static bool lds_attr_set = false;
   SirStep a;
   a.n = g->n;
   a.nnz = g->nnz;
@@ -596,13 +643,7 @@ int launch_step(const abm_sir_graph *g, uint8_t *state, uint16_t *mark,
   if (a.words <= kLdsBitmapWords) {
     auto *kp = k_sir_step<true, kModePartitionable>;
     auto *ko = k_sir_step<true, kModeOriginal>;
-    if (!lds_attr_set) {
-      for (auto *k : {kp, ko})
-        ABM_CHECK_HIP(hipFuncSetAttribute((const void *)k,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize,
-                                          (int)(kLdsBitmapWords * 4)));
-      lds_attr_set = true;
-    }
+    if (int rc = sir_lds_attr()) return rc;
     hipLaunchKernelGGL(part ? kp : ko, dim3(blocks), dim3(kSirThreads), (size_t)a.words * 4,
                        s, a);
   } else {
@@ -653,6 +694,9 @@ int abm_sir_seed(const abm_sir_graph *g, const int32_t *perm, int64_t n0,
   ABM_REQUIRE(state && mark && bitmap && n0 >= 0 && n0 <= g->n && (n0 == 0 || perm),
               "abm_sir_seed: bad arguments");
   hipStream_t s = (hipStream_t)stream;
+  if (int rc = check_chunks(g, s)) return rc;
+  // the per-device launch set-up, here rather than inside a captured run
+  if (int rc = sir_lds_attr()) return rc;
   const int64_t n = g->n, words = bitmap_words(n);
   hipLaunchKernelGGL(k_sir_init_state, dim3(blocks_for(n, 256)), dim3(256), 0,
                      s, n, g->row_ptr, state);

@@ -117,3 +117,27 @@ def test_sir_full_size_matches_oracle(sir, mode):
     assert (np.diff(got[:, 0]) <= 0).all()   # S never grows
     assert (np.diff(got[:, 2]) >= 0).all()   # R never shrinks
     assert got[-1, 1] == 0 and got[-1, 2] > n // 2
+
+
+def test_seed_refuses_bad_chunk_tables(sir):
+    """abm_sir_seed validates the chunk table the step kernel assumes (0 to
+    n, ascending, 64-aligned inner boundaries, at most one tile per chunk)
+    before launching anything."""
+    import ctypes as C
+
+    import torch
+
+    from jaxabm_amd._lib import AbmError, SirGraph, check, lib, ptr, stream_handle
+
+    n = 1000
+    row_ptr, col = O.ba_csr(n, 4, graph_seed=2)
+    eng = sir.SIRNetworkEngine(n, csr=(row_ptr, col), n_seed_infected=5, seed=0)
+    perm = torch.arange(n, dtype=torch.int32, device="cuda")
+    for bad in ([0, 500, 1000], [0, 512, 999], [0, 1000, 512], [64, 1000]):
+        cp = torch.tensor(bad, dtype=torch.int32, device="cuda")
+        g = SirGraph(n, eng.graph.nnz, eng.graph.row_ptr.data_ptr(), eng.graph.col.data_ptr(),
+                     cp.data_ptr(), len(bad) - 1)
+        rc = lib().abm_sir_seed(C.byref(g), ptr(perm), 5, ptr(eng.state), ptr(eng.mark),
+                                ptr(eng.bitmaps), 0, stream_handle())
+        with pytest.raises(AbmError, match="chunk_ptr"):
+            check(rc, "abm_sir_seed")
