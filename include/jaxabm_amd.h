@@ -117,6 +117,23 @@ int64_t abm_ba_edges(int64_t n, int32_t m, const uint32_t key[2], int32_t *src,
 /* Undirected CSR (both directions, rows ascending). */
 int abm_csr_undirected(int64_t n, int64_t n_edges, const int32_t *src, const int32_t *dst,
                        int32_t *row_ptr, int32_t *col);
+/* CSR of a Network built on the device from m edge-list entries (replaces
+ * the edge list grown by Network.add_edge and the masked get_neighbors
+ * queries, jaxabm/agentpy.py:559-615).  src, dst, row_ptr[n+1] and col are
+ * device arrays; col holds up to m entries, 2m with ABM_CSR_SYMMETRIZE.
+ * Flags: ABM_CSR_SYMMETRIZE adds (dst, src) after every entry (both
+ * directions, as abm_csr_undirected); ABM_CSR_DEDUP drops repeated (row, col)
+ * entries; ABM_CSR_INSERTION_ORDER keeps each row in entry order (otherwise
+ * neighbours ascend; not with ABM_CSR_DEDUP).  *nnz (host) receives the entry
+ * count, also in row_ptr[n].  Synchronises the stream (set-up, not for graph
+ * capture); an endpoint outside [0, n) is an error. */
+#define ABM_CSR_SYMMETRIZE 1
+#define ABM_CSR_DEDUP 2
+#define ABM_CSR_INSERTION_ORDER 4
+size_t abm_csr_from_edges_workspace(int64_t m, int32_t flags);
+int abm_csr_from_edges(int64_t n, int64_t m, const int32_t *src, const int32_t *dst,
+                       int32_t flags, int32_t *row_ptr, int32_t *col, int64_t *nnz, void *ws,
+                       size_t ws_bytes, void *stream);
 /* Contiguous agent chunks of about `target` cost (cost(v) = node_cost +
  * degree(v), plus heavy_cost when degree(v) > heavy_deg) and at most
  * max_nodes agents, with boundaries that are multiples of `align` (except the
@@ -229,6 +246,21 @@ int abm_wealth_init(const uint32_t *init_key, int64_t n, float *x, float *y, int
  * state in registers.  Same results either way. */
 int abm_wealth_run_batch(const abm_wealth_inst *insts, int32_t count, int64_t max_n, int mode,
                          void *stream);
+
+/* ---- Moore-8 neighbourhood reductions over a (rows, cols) row-major Grid
+ *      array (the reference Grid stores only shape and periodic flag,
+ *      jaxabm/agentpy.py:465-527; its models reduce neighbourhoods with jnp
+ *      code).  periodic != 0 wraps both axes; otherwise offsets outside the
+ *      grid are left out. ---- */
+/* out[r][c] = sum of x over the 8 neighbours, float64 accumulation in the
+ * fixed order (-1,-1) (-1,0) (-1,1) (0,-1) (0,1) (1,-1) (1,0) (1,1), rounded
+ * once to float32. */
+int abm_grid_moore_sum_f32(int64_t rows, int64_t cols, int32_t periodic, const float *x,
+                           float *out, void *stream);
+/* same[r][c] = neighbours with cells != 0 and == cells[r][c]; occupied[r][c]
+ * = neighbours with cells != 0 (either output may be NULL). */
+int abm_grid_moore_counts_i8(int64_t rows, int64_t cols, int32_t periodic, const int8_t *cells,
+                             int32_t *same, int32_t *occupied, void *stream);
 
 /* ---- Schelling program on a periodic Grid (jaxabm/agentpy.py:465-527
  *      rebuilt with a Moore-8 step) ---- */

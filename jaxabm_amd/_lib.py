@@ -147,6 +147,9 @@ SIGNATURES = {
     "abm_ba_edges": (_i64, [_i64, _i32, _u32arr, _i32arr, _i32arr]),
     "abm_csr_undirected": (_int, [_i64, _i64, _i32arr, _i32arr, _i32arr, _i32arr]),
     "abm_node_chunks": (_i64, [_i64, _i32arr, _i32, _i64, _i32, _i64, _i64, _i32, _p]),
+    "abm_csr_from_edges_workspace": (_sz, [_i64, _i32]),
+    "abm_csr_from_edges": (_int, [_i64, _i64, _p, _p, _i32, _p, _p, C.POINTER(_i64), _p, _sz,
+                                  _p]),
     "abm_sir_chunk_params": (_int, [C.POINTER(_i32), C.POINTER(_i64), C.POINTER(_i32),
                                     C.POINTER(_i64), C.POINTER(_i32), C.POINTER(_i32),
                                     C.POINTER(_i32)]),
@@ -181,6 +184,8 @@ SIGNATURES = {
     "abm_csr_count_u8": (_int, [_i64, _p, _p, _p, _p, _p]),
     "abm_scatter_add_f32": (_int, [_i64, _p, _p, _p, _p]),
     "abm_scatter_add_i32": (_int, [_i64, _p, _p, _p, _p]),
+    "abm_grid_moore_sum_f32": (_int, [_i64, _i64, _i32, _p, _p, _p]),
+    "abm_grid_moore_counts_i8": (_int, [_i64, _i64, _i32, _p, _p, _p, _p]),
     "abm_schelling_block_count": (_i64, [_i64]),
     "abm_schelling_init": (_int, [C.POINTER(SchellingGrid), _p, _int, C.POINTER(_i32), _p]),
     "abm_schelling_run": (_int, [C.POINTER(SchellingGrid), _p, _i32, _int, _p, _p, _p, _p]),

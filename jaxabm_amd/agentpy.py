@@ -249,6 +249,39 @@ class Grid:
             agents.collection._states = new
 
 
+    def _grid_array(self, values: torch.Tensor, dtype) -> torch.Tensor:
+        t = torch.as_tensor(values, device=jr._device())
+        if t.dim() != 2:
+            raise ValueError("a Grid array is 2-D: (rows, cols)")
+        return t.to(dtype).contiguous()
+
+    def moore_sum(self, values) -> torch.Tensor:
+        """Sum of ``values`` (a (rows, cols) array) over each cell's 8 Moore
+        neighbours, wrapping when the grid is periodic (HIP, LDS-tiled;
+        float32 result of a float64 sum in a fixed order)."""
+        from ._lib import check, lib, ptr, stream_handle
+
+        x = self._grid_array(values, torch.float32)
+        out = torch.empty_like(x)
+        check(lib().abm_grid_moore_sum_f32(x.shape[0], x.shape[1], int(bool(self.periodic)),
+                                           ptr(x), ptr(out), stream_handle()), "grid_moore_sum")
+        return out
+
+    def moore_counts(self, cells) -> Tuple[torch.Tensor, torch.Tensor]:
+        """For a (rows, cols) array of cell types (0 = empty): per cell, the
+        number of Moore neighbours of its own non-zero type and the number of
+        occupied Moore neighbours (int32), wrapping when periodic."""
+        from ._lib import check, lib, ptr, stream_handle
+
+        c = self._grid_array(cells, torch.int8)
+        same = torch.empty(c.shape, dtype=torch.int32, device=c.device)
+        occ = torch.empty(c.shape, dtype=torch.int32, device=c.device)
+        check(lib().abm_grid_moore_counts_i8(c.shape[0], c.shape[1], int(bool(self.periodic)),
+                                             ptr(c), ptr(same), ptr(occ), stream_handle()),
+              "grid_moore_counts")
+        return same, occ
+
+
 class Network:
     """Agent network (jaxabm/agentpy.py:530-615)."""
 
@@ -337,36 +370,29 @@ class Network:
         return n
 
     def csr(self):
-        """Device CSR of the stored (directed) entries: rows ascending,
-        neighbours sorted, duplicates removed for undirected networks."""
+        """CSR of the stored entries, built on the device (abm_csr_from_edges):
+        rows ascending.  Undirected: row v holds the sorted unique ids of both
+        directions of v's entries (what get_neighbors returns); directed: the
+        targets of v's entries in insertion order, duplicates kept."""
         if self._csr is None:
-            from .network import DeviceCSR
+            from .network import csr_from_edges
 
             s, d = self._host_edges()
             n = self._n_nodes(s, d)
             if self.directed:
-                order = np.lexsort((np.arange(len(s)), s))
-                rs, cs = s[order], d[order]
+                self._csr = csr_from_edges(n, s, d, insertion_order=True, device=jr._device())
             else:
-                key = np.unique(s.astype(np.int64) * (1 << 32) + d.astype(np.int64))
-                rs, cs = (key >> 32).astype(np.int32), (key & 0xFFFFFFFF).astype(np.int32)
-            row_ptr = np.zeros(n + 1, np.int32)
-            np.add.at(row_ptr, rs + 1, 1)
-            row_ptr = np.cumsum(row_ptr).astype(np.int32)
-            self._csr = DeviceCSR.from_host(row_ptr, cs.astype(np.int32))
+                self._csr = csr_from_edges(n, s, d, symmetrize=True, dedup=True,
+                                           device=jr._device())
         return self._csr
 
     # -- queries
     def get_neighbors(self, agent: Union[Agent, int]) -> torch.Tensor:
         """Directed: targets of the agent's edges in insertion order.
-        Undirected: sorted unique ids over both directions."""
+        Undirected: sorted unique ids over both directions.  One row of the
+        device CSR (built once per change of the edge list)."""
         a = agent.id if isinstance(agent, Agent) else int(agent)
-        s, d = self._host_edges()
-        if self.directed:
-            out = d[s == a]
-        else:
-            out = np.unique(np.concatenate([d[s == a], s[d == a]]))
-        return torch.from_numpy(out.astype(np.int32)).to(jr._device())
+        return self.csr().row(a)
 
     def neighbor_sum(self, values: torch.Tensor) -> torch.Tensor:
         """out[v] = sum of values[u] over the CSR row of v (HIP segmented

@@ -9,8 +9,6 @@ networks (ascending unique ids).
 """
 from __future__ import annotations
 
-from dataclasses import dataclass
-
 import numpy as np
 import torch
 
@@ -59,16 +57,18 @@ def node_chunks(row_ptr: np.ndarray, align: int, node_cost: int, target: int,
     return out
 
 
-@dataclass
 class DeviceCSR:
-    """An undirected graph resident in HBM."""
+    """A graph resident in HBM as CSR (row_ptr int32[n+1], col int32[nnz]);
+    host copies are made on first use."""
 
-    n: int
-    nnz: int
-    row_ptr: torch.Tensor  # int32 [n+1]
-    col: torch.Tensor  # int32 [nnz]
-    host_row_ptr: np.ndarray
-    host_col: np.ndarray
+    def __init__(self, n: int, nnz: int, row_ptr: torch.Tensor, col: torch.Tensor,
+                 host_row_ptr: np.ndarray | None = None, host_col: np.ndarray | None = None):
+        self.n = int(n)
+        self.nnz = int(nnz)
+        self.row_ptr = row_ptr
+        self.col = col
+        self._host_row_ptr = host_row_ptr
+        self._host_col = host_col
 
     @classmethod
     def from_host(cls, row_ptr: np.ndarray, col: np.ndarray, device="cuda") -> "DeviceCSR":
@@ -77,8 +77,59 @@ class DeviceCSR:
                                                    np.int32)).to(device)
         return cls(len(row_ptr) - 1, int(row_ptr[-1]), rp, cl, row_ptr, col)
 
+    @property
+    def host_row_ptr(self) -> np.ndarray:
+        if self._host_row_ptr is None:
+            self._host_row_ptr = self.row_ptr.cpu().numpy()
+        return self._host_row_ptr
+
+    @property
+    def host_col(self) -> np.ndarray:
+        if self._host_col is None:
+            self._host_col = self.col[: self.nnz].cpu().numpy()
+        return self._host_col
+
     def neighbors(self, v: int) -> np.ndarray:
         return self.host_col[self.host_row_ptr[v]:self.host_row_ptr[v + 1]].copy()
+
+    def row(self, v: int) -> torch.Tensor:
+        """Row v's entries on the device (an int32 copy)."""
+        if not 0 <= v < self.n:
+            return torch.zeros(0, dtype=torch.int32, device=self.col.device)
+        lo, hi = (int(x) for x in self.row_ptr[v:v + 2].cpu())
+        return self.col[lo:hi].clone()
+
+
+CSR_SYMMETRIZE, CSR_DEDUP, CSR_INSERTION_ORDER = 1, 2, 4   # abm_csr_from_edges flags
+
+
+def csr_from_edges(n: int, src, dst, symmetrize: bool = False, dedup: bool = False,
+                   insertion_order: bool = False, device="cuda") -> DeviceCSR:
+    """CSR built on the device from the edge-list entries (src[i], dst[i]):
+    rows ascending; neighbours ascending, or in entry order with
+    ``insertion_order``; ``symmetrize`` adds (dst, src) after every entry;
+    ``dedup`` drops repeated entries.  src/dst: host arrays or device tensors."""
+    import ctypes as C
+
+    s = torch.as_tensor(np.asarray(src, np.int32) if not torch.is_tensor(src) else src)
+    d = torch.as_tensor(np.asarray(dst, np.int32) if not torch.is_tensor(dst) else dst)
+    s = s.to(device=device, dtype=torch.int32).contiguous().reshape(-1)
+    d = d.to(device=device, dtype=torch.int32).contiguous().reshape(-1)
+    if s.numel() != d.numel():
+        raise ValueError("src and dst need the same length")
+    m = s.numel()
+    flags = ((CSR_SYMMETRIZE if symmetrize else 0) | (CSR_DEDUP if dedup else 0)
+             | (CSR_INSERTION_ORDER if insertion_order else 0))
+    cap = 2 * m if symmetrize else m
+    row_ptr = torch.empty(n + 1, dtype=torch.int32, device=device)
+    col = torch.empty(max(cap, 1), dtype=torch.int32, device=device)
+    ws = torch.empty(max(int(lib().abm_csr_from_edges_workspace(m, flags)), 1), dtype=torch.uint8,
+                     device=device)
+    nnz = C.c_int64(0)
+    check(lib().abm_csr_from_edges(int(n), m, ptr(s), ptr(d), flags, ptr(row_ptr), ptr(col),
+                                   C.byref(nnz), ptr(ws), ws.numel(), stream_handle()),
+          "csr_from_edges")
+    return DeviceCSR(n, nnz.value, row_ptr, col)
 
 
 def ba_csr(n: int, m: int, graph_seed: int) -> tuple[np.ndarray, np.ndarray]:
