@@ -453,6 +453,17 @@ int abm_mean_f32(const float *x, int64_t n, float *out, void *ws, size_t ws_byte
                  void *stream);
 int abm_mean_i32(const int32_t *x, int64_t n, float *out, void *ws, size_t ws_bytes,
                  void *stream);
+/* Whole-array sums (jnp.sum with x64 off inside metrics / env callbacks):
+ * float32 summed in float64 in a fixed order and rounded once; int32 and
+ * uint8 (booleans) summed exactly into int32 with wrap-around.  Up to 65,536
+ * values one launch and no workspace (ws may be NULL); more need
+ * abm_reduce_workspace(n) bytes.  The means above take the same one-launch
+ * path up to 65,536 values. */
+int abm_sum_f32(const float *x, int64_t n, float *out, void *ws, size_t ws_bytes, void *stream);
+int abm_sum_i32(const int32_t *x, int64_t n, int32_t *out, void *ws, size_t ws_bytes,
+                void *stream);
+int abm_sum_u8(const uint8_t *x, int64_t n, int32_t *out, void *ws, size_t ws_bytes,
+               void *stream);
 /* *out = Gini of int32 values: sorted x, S = sum (i+1) x_(i), T = sum x in
  * int32 (wrapping), G = f32(2S) / f32(nT) - f32((n+1)/n), 0 when T <= 1e-6. */
 int abm_gini_i32(const int32_t *x, int64_t n, float *out, void *ws, size_t ws_bytes,

@@ -204,6 +204,9 @@ SIGNATURES = {
     "abm_mean_f32": (_int, [_p, _i64, _p, _p, _sz, _p]),
     "abm_mean_i32": (_int, [_p, _i64, _p, _p, _sz, _p]),
     "abm_gini_i32": (_int, [_p, _i64, _p, _p, _sz, _p]),
+    "abm_sum_f32": (_int, [_p, _i64, _p, _p, _sz, _p]),
+    "abm_sum_i32": (_int, [_p, _i64, _p, _p, _sz, _p]),
+    "abm_sum_u8": (_int, [_p, _i64, _p, _p, _sz, _p]),
     "abm_soft_hist_f32": (_int, [_p, _i64, _i32, C.c_double, _p, _p, _sz, _p]),
     "abm_gini_i32_noalloc": (_int, [_p, _i64, _p, _p]),
     "abm_rtc_compile": (_int, [C.c_char_p, C.c_char_p, C.POINTER(_p), C.POINTER(C.c_size_t)]),

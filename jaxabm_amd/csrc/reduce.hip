@@ -89,6 +89,43 @@ __global__ void __launch_bounds__(kRedThreads)
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// One workgroup of 1024 threads reduces the whole array (n up to
+// kSmallMax): thread t accumulates elements t, t + 1024, ... in order, then
+// the wavefront sums (fixed shuffle order) are added in wave order.  One
+// launch, no workspace.  OUT: 0 float32 sum, 1 float32 mean, 2 int32 sum
+// (wrapping), 3 float32 mean of int32 (f32(exact sum) / f32(n)).
+constexpr int64_t kSmallMax = 65536;
+
+template <typename In, typename Acc, int OUT>
+__global__ void __launch_bounds__(1024)
+    k_reduce_one_block(const In *__restrict__ x, int64_t n, void *__restrict__ out) {
+  __shared__ Acc red[16];
+  Acc v = 0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) v += (Acc)x[i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Acc s = 0;
+    for (int w = 0; w < 16; ++w) s += red[w];
+    if constexpr (OUT == 0) *(float *)out = (float)s;
+    else if constexpr (OUT == 1) *(float *)out = (float)((double)s / (double)n);
+    else if constexpr (OUT == 2) *(int32_t *)out = (int32_t)(uint32_t)(uint64_t)s;
+    else *(float *)out = (float)s / (float)n;
+  }
+}
+
+// Final pass of the multi-block sums: partial[b] added in block order.
+template <typename Acc, int OUT>
+__global__ void k_reduce_final(const Acc *__restrict__ partial, int nb, int64_t n,
+                               void *__restrict__ out) {
+  Acc s = 0;
+  for (int b = 0; b < nb; ++b) s += partial[b];
+  if constexpr (OUT == 0) *(float *)out = (float)s;
+  else if constexpr (OUT == 2) *(int32_t *)out = (int32_t)(uint32_t)(uint64_t)s;
+}
+
 __global__ void k_mean_f32_final(const double *__restrict__ partial, int nb, int64_t n,
                                  float *__restrict__ out) {
   double s = 0.0;
@@ -226,6 +263,32 @@ int noalloc_scratch(void **out) {
   return kOk;
 }
 
+// Whole-array sum: one block up to kSmallMax elements, else block partials
+// (k_partial_sum) and a final pass.
+template <typename In, typename Acc, int OUT>
+int sum_all(const In *x, int64_t n, void *out, void *ws, size_t ws_bytes, hipStream_t s,
+            const char *who) {
+  if (!(x && out && n >= 1)) {
+    set_error(std::string(who) + ": bad arguments");
+    return kErrInvalid;
+  }
+  if (n <= kSmallMax) {
+    hipLaunchKernelGGL((k_reduce_one_block<In, Acc, OUT>), dim3(1), dim3(1024), 0, s, x, n, out);
+  } else {
+    if (!ws || ws_bytes < (size_t)kRedMaxBlocks * 8) {
+      set_error(std::string(who) + ": workspace too small");
+      return kErrInvalid;
+    }
+    const unsigned nb = red_blocks(n);
+    hipLaunchKernelGGL((k_partial_sum<In, Acc>), dim3(nb), dim3(kRedThreads), 0, s, x, n,
+                       (Acc *)ws);
+    hipLaunchKernelGGL((k_reduce_final<Acc, OUT>), dim3(1), dim3(1), 0, s, (const Acc *)ws,
+                       (int)nb, n, out);
+  }
+  ABM_LAUNCH_CHECK(who);
+  return kOk;
+}
+
 }  // namespace abm
 
 using namespace abm;
@@ -256,9 +319,15 @@ size_t abm_reduce_workspace(int64_t n) {
 
 int abm_mean_f32(const float *x, int64_t n, float *out, void *ws, size_t ws_bytes,
                  void *stream) {
-  ABM_REQUIRE(x && out && ws && n >= 1 && ws_bytes >= (size_t)kRedMaxBlocks * 8,
-              "abm_mean_f32: bad arguments");
+  ABM_REQUIRE(x && out && n >= 1, "abm_mean_f32: bad arguments");
   hipStream_t s = (hipStream_t)stream;
+  if (n <= kSmallMax) {
+    hipLaunchKernelGGL((k_reduce_one_block<float, double, 1>), dim3(1), dim3(1024), 0, s, x, n,
+                       (void *)out);
+    ABM_LAUNCH_CHECK("abm_mean_f32");
+    return kOk;
+  }
+  ABM_REQUIRE(ws && ws_bytes >= (size_t)kRedMaxBlocks * 8, "abm_mean_f32: workspace too small");
   const unsigned nb = red_blocks(n);
   hipLaunchKernelGGL((k_partial_sum<float, double>), dim3(nb), dim3(kRedThreads), 0, s, x, n,
                      (double *)ws);
@@ -270,9 +339,15 @@ int abm_mean_f32(const float *x, int64_t n, float *out, void *ws, size_t ws_byte
 
 int abm_mean_i32(const int32_t *x, int64_t n, float *out, void *ws, size_t ws_bytes,
                  void *stream) {
-  ABM_REQUIRE(x && out && ws && n >= 1 && ws_bytes >= (size_t)kRedMaxBlocks * 8,
-              "abm_mean_i32: bad arguments");
+  ABM_REQUIRE(x && out && n >= 1, "abm_mean_i32: bad arguments");
   hipStream_t s = (hipStream_t)stream;
+  if (n <= kSmallMax) {
+    hipLaunchKernelGGL((k_reduce_one_block<int32_t, int64_t, 3>), dim3(1), dim3(1024), 0, s, x, n,
+                       (void *)out);
+    ABM_LAUNCH_CHECK("abm_mean_i32");
+    return kOk;
+  }
+  ABM_REQUIRE(ws && ws_bytes >= (size_t)kRedMaxBlocks * 8, "abm_mean_i32: workspace too small");
   const unsigned nb = red_blocks(n);
   hipLaunchKernelGGL((k_partial_sum<int32_t, int64_t>), dim3(nb), dim3(kRedThreads), 0, s, x, n,
                      (int64_t *)ws);
@@ -280,6 +355,22 @@ int abm_mean_i32(const int32_t *x, int64_t n, float *out, void *ws, size_t ws_by
                      out);
   ABM_LAUNCH_CHECK("abm_mean_i32");
   return kOk;
+}
+
+int abm_sum_f32(const float *x, int64_t n, float *out, void *ws, size_t ws_bytes, void *stream) {
+  return sum_all<float, double, 0>(x, n, out, ws, ws_bytes, (hipStream_t)stream, "abm_sum_f32");
+}
+
+int abm_sum_i32(const int32_t *x, int64_t n, int32_t *out, void *ws, size_t ws_bytes,
+                void *stream) {
+  return sum_all<int32_t, int64_t, 2>(x, n, out, ws, ws_bytes, (hipStream_t)stream,
+                                      "abm_sum_i32");
+}
+
+int abm_sum_u8(const uint8_t *x, int64_t n, int32_t *out, void *ws, size_t ws_bytes,
+               void *stream) {
+  return sum_all<uint8_t, int64_t, 2>(x, n, out, ws, ws_bytes, (hipStream_t)stream,
+                                      "abm_sum_u8");
 }
 
 int abm_gini_i32(const int32_t *x, int64_t n, float *out, void *ws, size_t ws_bytes,

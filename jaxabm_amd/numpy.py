@@ -368,6 +368,9 @@ def logical_xor(x, y):
 def _reduce(fn, x, axis, keepdims, **kw):
     x = _t(x)
     if axis is None:
+        if keepdims and isinstance(x, torch.Tensor) and x.dim() > 0:
+            # JAX keeps every reduced axis as size 1
+            return fn(x, dim=tuple(range(x.dim())), keepdim=True, **kw)
         return fn(x, **kw)
     return fn(x, dim=axis, keepdim=keepdims, **kw)
 
@@ -382,8 +385,41 @@ def _int_acc(x):
     return {"dtype": torch.int32} if x.dtype in _SMALL_INTS else {}
 
 
+_DEVICE_REDUCIBLE = (torch.float32, torch.int32, torch.int16, torch.int8, torch.uint8, torch.bool)
+_is_wrapped = torch._C._functorch.is_functorch_wrapped_tensor
+
+
+def _reductions():
+    from . import reductions
+
+    return reductions
+
+
+def _whole(x, axis) -> bool:
+    """True when ``jnp.sum`` / ``jnp.mean`` of x over ``axis`` is a
+    whole-array reduction of a plain GPU tensor that the HIP reduction
+    kernels take: not under vmap or another functorch transform, not
+    recording gradients."""
+    if not isinstance(x, torch.Tensor) or not x.is_cuda or x.numel() == 0:
+        return False
+    if axis is not None:
+        axes = axis if isinstance(axis, (tuple, list)) else (axis,)
+        rank = builtins.max(x.dim(), 1)
+        if sorted(a % rank for a in axes) != list(range(x.dim())):
+            return False
+    if x.dtype not in _DEVICE_REDUCIBLE or x.requires_grad:
+        return False
+    return not _is_wrapped(x)
+
+
 def sum(x, axis=None, keepdims=False):
+    """jnp.sum.  A whole-array sum of a plain GPU tensor (the usual metrics
+    and env-callback reduction) is one HIP launch (jaxabm_amd.reductions:
+    float32 summed in float64, integers and booleans exactly into int32);
+    other sums are torch reductions with JAX's x64-off dtypes."""
     x = _t(x)
+    if not keepdims and _whole(x, axis):
+        return _reductions().sum_all(x)
     return _reduce(torch.sum, x, axis, keepdims, **_int_acc(x))
 
 
@@ -395,7 +431,12 @@ def prod(x, axis=None, keepdims=False):
 
 
 def mean(x, axis=None, keepdims=False):
+    """jnp.mean.  A whole-array mean of a plain GPU tensor is one HIP launch
+    (float32 summed in float64; integers and booleans summed exactly, then
+    float32(sum) / float32(n)); others are torch means in float32."""
     x = _t(x)
+    if not keepdims and _whole(x, axis):
+        return _reductions().mean_all(x)
     if not x.is_floating_point():
         x = x.to(torch.float32)
     return _reduce(torch.mean, x, axis, keepdims)
@@ -452,7 +493,10 @@ def all(x, axis=None, keepdims=False):
 
 
 def count_nonzero(x, axis=None):
-    return torch.count_nonzero(_t(x), dim=axis).to(torch.int32)
+    x = _t(x)
+    if _whole(x, axis):
+        return _reductions().sum_all(x if x.dtype == torch.bool else x != 0)
+    return torch.count_nonzero(x, dim=axis).to(torch.int32)
 
 
 def cumsum(x, axis=None):

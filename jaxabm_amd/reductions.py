@@ -15,6 +15,9 @@ replayed as a graph (``Model.run(graph=True)``).
   then float32(sum) / float32(n)); float32 input summed in float64 in a fixed
   order and rounded once (deterministic; the reference's float32 summation
   order is XLA's, so float means agree to rounding).
+* ``sum_all(x)`` / ``mean_all(x)``: the whole-array jnp.sum / jnp.mean that
+  ``jaxabm_amd.numpy`` runs on plain GPU tensors (env and metrics callbacks):
+  one launch up to 65,536 values.
 * ``gini(x)``: int32 input, the reference's formula bit for bit (int32 sums
   with wrap-around, float32 quotient).
 * ``soft_histogram(x, bins, slope)``: float64 bin means of sigmoid
@@ -54,14 +57,80 @@ def state_counts(states: torch.Tensor, k: int) -> torch.Tensor:
 
 def mean(x: torch.Tensor) -> torch.Tensor:
     """0-d float32 mean of an int32 or float32 array."""
-    v = _check_input(x, (torch.int32, torch.float32))
+    _check_input(x, (torch.int32, torch.float32))
+    return mean_all(x)
+
+
+_SUM_SMALL = 65536   # abm_sum_* / abm_mean_* need no workspace up to this size
+
+# whole-array reductions are called from model callbacks every step: the
+# entry points and dtypes are looked up once, and the stream is read as a
+# raw handle
+_FNS: dict = {}
+
+
+def _fn(name: str):
+    f = _FNS.get(name)
+    if f is None:
+        f = _FNS[name] = getattr(lib(), name)
+    return f
+
+
+def _whole_call(name: str, v: torch.Tensor, out_dtype) -> torch.Tensor:
+    n = v.numel()
+    dev = v.device
+    out = torch.empty((), dtype=out_dtype, device=dev)
+    ws = None if n <= _SUM_SMALL else _workspace(n, dev)
+    stream = torch._C._cuda_getCurrentRawStream(dev.index if dev.index is not None
+                                                else torch.cuda.current_device())
+    rc = _fn(name)(v.data_ptr(), n, out.data_ptr(), None if ws is None else ws.data_ptr(),
+                   0 if ws is None else ws.numel(), stream)
+    if rc:
+        check(rc, name)
+    return out
+
+
+def sum_all(x: torch.Tensor) -> torch.Tensor:
+    """0-d sum of every element, as jnp.sum with x64 off: float32 (summed in
+    float64 in a fixed order, rounded once), or int32 with wrap-around for
+    int32, int16, int8, uint8 and bool input."""
+    if not isinstance(x, torch.Tensor) or not x.is_cuda:
+        raise TypeError("reductions take a GPU tensor")
+    dt = x.dtype
+    if dt in (torch.int8, torch.int16):
+        x, dt = x.to(torch.int32), torch.int32
+    if dt not in (torch.float32, torch.int32, torch.uint8, torch.bool):
+        raise TypeError(f"dtype {dt} not supported here")
+    v = x.reshape(-1)
+    if not v.is_contiguous():
+        v = v.contiguous()
+    if v.numel() == 0:
+        return torch.zeros((), dtype=torch.float32 if dt == torch.float32 else torch.int32,
+                           device=v.device)
+    if dt == torch.float32:
+        return _whole_call("abm_sum_f32", v, torch.float32)
+    if dt == torch.int32:
+        return _whole_call("abm_sum_i32", v, torch.int32)
+    return _whole_call("abm_sum_u8", v.view(torch.uint8) if dt == torch.bool else v, torch.int32)
+
+
+def mean_all(x: torch.Tensor) -> torch.Tensor:
+    """0-d float32 mean of every element: float32 input summed in float64
+    (fixed order), integer and boolean input summed exactly, then
+    float32(sum) / float32(n)."""
+    if not isinstance(x, torch.Tensor) or not x.is_cuda:
+        raise TypeError("reductions take a GPU tensor")
+    if x.dtype in (torch.int8, torch.int16, torch.uint8, torch.bool):
+        x = x.to(torch.int32)
+    if x.dtype not in (torch.int32, torch.float32):
+        raise TypeError(f"dtype {x.dtype} not supported here")
+    v = x.reshape(-1)
+    if not v.is_contiguous():
+        v = v.contiguous()
     if v.numel() == 0:
         raise ValueError("mean of an empty array")
-    out = torch.empty((), dtype=torch.float32, device=v.device)
-    ws = _workspace(v.numel(), v.device)
-    fn = lib().abm_mean_i32 if v.dtype == torch.int32 else lib().abm_mean_f32
-    check(fn(ptr(v), v.numel(), ptr(out), ptr(ws), ws.numel(), stream_handle()), "mean")
-    return out
+    return _whole_call("abm_mean_i32" if v.dtype == torch.int32 else "abm_mean_f32", v,
+                       torch.float32)
 
 
 def gini(x: torch.Tensor) -> torch.Tensor:

@@ -105,3 +105,57 @@ def test_reductions_in_a_replayed_step(R):
             np.testing.assert_array_equal(x.cpu().numpy(), y.cpu().numpy(), err_msg=f"{k} {i}")
     w = a.agent_collections["g"].states["w"].cpu().numpy()
     assert ra["gini"][-1].item() == O.gini_i32(w)
+
+
+@pytest.mark.parametrize("n", [1, 7, 10_000, 65_536, 65_537, 1_000_003])
+def test_whole_array_sums_and_means(R, n):
+    """jnp.sum / jnp.mean / jnp.count_nonzero of a plain GPU tensor run the
+    HIP kernels (one launch up to 65,536 values, block partials above):
+    int32 sums wrap exactly as JAX's x64-off int32, boolean sums count,
+    float32 sums and means are the float64 sum rounded once (within an ulp of
+    numpy's float64 pairwise sum, rounded)."""
+    import torch
+
+    from jaxabm_amd import numpy as jnp
+    from jaxabm_amd import reductions
+
+    rs = np.random.default_rng(n)
+    xi = rs.integers(-2**31, 2**31 - 1, n, dtype=np.int64).astype(np.int32)
+    ti = torch.from_numpy(xi).cuda()
+    assert jnp.sum(ti).dtype == torch.int32
+    wrapped = (int(xi.astype(np.int64).sum()) + 2**31) % 2**32 - 2**31
+    assert int(jnp.sum(ti)) == wrapped
+    xb = rs.random(n) < 0.3
+    tb = torch.from_numpy(xb).cuda()
+    assert int(jnp.sum(tb)) == int(xb.sum()) == int(jnp.count_nonzero(tb))
+    assert int(jnp.count_nonzero(ti)) == int((xi != 0).sum())
+    xf = (rs.standard_normal(n) * 100).astype(np.float32)
+    tf = torch.from_numpy(xf).cuda()
+    s = float(jnp.sum(tf))
+    want = np.float32(xf.astype(np.float64).sum())
+    assert abs(np.float32(s) - want) <= np.spacing(np.abs(want)) + 1e-30
+    m = float(jnp.mean(tf))
+    wm = np.float32(xf.astype(np.float64).sum() / n)
+    assert abs(np.float32(m) - wm) <= np.spacing(np.abs(wm)) + 1e-30
+    small = rs.integers(0, 1000, n).astype(np.int32)
+    np.testing.assert_equal(float(jnp.mean(torch.from_numpy(small).cuda())),
+                            float(np.float32(small.astype(np.int64).sum()) / np.float32(n)))
+    # the one-launch and multi-block paths are the module's entry points too
+    assert reductions.sum_all(ti).item() == jnp.sum(ti).item()
+
+
+def test_reduction_routing_rules(R):
+    """Axis reductions, keepdims, float64 and tensors that record gradients
+    stay torch reductions (with JAX's x64-off integer dtypes)."""
+    import torch
+
+    from jaxabm_amd import numpy as jnp
+
+    x = torch.arange(12, dtype=torch.int32, device="cuda").reshape(3, 4)
+    assert jnp.sum(x, axis=0).tolist() == [12, 15, 18, 21]
+    assert jnp.sum(x, axis=(0, 1)).item() == 66 and jnp.sum(x, axis=(0, 1)).dim() == 0
+    assert jnp.sum(x, keepdims=True).shape == (1, 1)
+    assert jnp.sum(x.double()).dtype == torch.float64
+    g = torch.ones(5, device="cuda", requires_grad=True)
+    jnp.mean(g * 2).backward()
+    assert torch.allclose(g.grad, torch.full((5,), 0.4, device="cuda"))
