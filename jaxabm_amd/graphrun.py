@@ -100,16 +100,20 @@ class _Runner:
     # -------------------------------------------------------------- the body
     def body(self, time_step=None):
         keys = self.keys
-        current = {"time_step": self.ts if time_step is None else time_step, "env": self.env}
+        # the callbacks get copies of the env-slot dict: a callback that
+        # writes into its env and returns it must not replace the fixed slot
+        # tensors the captured graph reads and commit() writes
+        current = {"time_step": self.ts if time_step is None else time_step,
+                   "env": dict(self.env)}
         for name, _c in self.colls:
             current[f"agents_{name}"] = self.static[name]
         for i, (name, c) in enumerate(self.colls):
             c._states = self.static[name]
             c.update(current, keys[i], self.m.config)
         updated = {name: c.states for name, c in self.colls}
-        env_new = self.env
+        env_new = dict(self.env)
         if self.has_env:
-            env_new = self.m._update_state_fn(self.env, updated, self.m._params,
+            env_new = self.m._update_state_fn(dict(self.env), updated, self.m._params,
                                               keys[len(self.colls)])
             if not isinstance(env_new, dict):
                 raise NotReplayable("update_state_fn did not return a dict")

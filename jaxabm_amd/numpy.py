@@ -560,13 +560,36 @@ def squeeze(x, axis=None):
     return _t(x).squeeze() if axis is None else _t(x).squeeze(axis)
 
 
+def _take_fill(dtype):
+    """jnp.take's default fill value for out-of-range indices: NaN for
+    floats, the most negative value for signed integers, the largest for
+    unsigned ones, True for booleans."""
+    if dtype.is_floating_point or dtype.is_complex:
+        return float("nan")
+    if dtype == torch.bool:
+        return True
+    return torch.iinfo(dtype).min if dtype.is_signed else torch.iinfo(dtype).max
+
+
 def take(x, indices, axis=None):
+    """jnp.take with its default mode "fill": negative indices count from the
+    end, and an index still outside the axis yields the fill value (NaN,
+    the most negative integer, the largest unsigned, True)."""
     x = _t(x)
-    idx = _t(indices).long()
+    idx = _t(indices).to(device=x.device, dtype=torch.long)
+    src = x.reshape(-1) if axis is None else x
+    ax = 0 if axis is None else axis % x.dim()
+    size = src.shape[ax]
+    idx = torch.where(idx < 0, idx + size, idx)
+    valid = (idx >= 0) & (idx < size)
+    safe = idx.clamp(0, builtins.max(size - 1, 0))
     if axis is None:
-        return x.reshape(-1)[idx]
-    return torch.index_select(x, axis, idx.reshape(-1)).reshape(
-        *x.shape[:axis], *idx.shape, *x.shape[axis + 1:])
+        out = src[safe] if size else torch.zeros(idx.shape, dtype=x.dtype, device=x.device)
+        return torch.where(valid, out, torch.full_like(out, _take_fill(x.dtype)))
+    out = torch.index_select(src, ax, safe.reshape(-1)).reshape(
+        *x.shape[:ax], *idx.shape, *x.shape[ax + 1:])
+    mask = valid.reshape((1,) * ax + tuple(idx.shape) + (1,) * (x.dim() - ax - 1))
+    return torch.where(mask, out, torch.full_like(out, _take_fill(x.dtype)))
 
 
 def dot(a, b):
@@ -590,26 +613,85 @@ def array_equal(a, b):
     return builtins.bool(a.shape == b.shape and torch.equal(a, b.to(a.dtype)))
 
 
+def _int_parts(y: torch.Tensor, index):
+    """An integer index of y's leading dims (an integer tensor, a Python int,
+    or a tuple of those), as JAX's scatter and gather read it: each part
+    normalised (a negative value counts from the end) and broadcast, with the
+    mask of the entries inside y.  None for other indices (slices, masks,
+    Ellipsis), which torch's indexing handles."""
+    parts = index if isinstance(index, tuple) else (index,)
+    if not parts or len(parts) > y.dim():
+        return None
+    if not builtins.all(_int_index(p) or (isinstance(p, int) and not isinstance(p, bool))
+                        for p in parts):
+        return None
+    idx, valid = [], None
+    for d, p in enumerate(parts):
+        t = torch.as_tensor(p, device=y.device).to(torch.long)
+        size = y.shape[d]
+        t = torch.where(t < 0, t + size, t)
+        ok = (t >= 0) & (t < size)
+        idx.append(t)
+        valid = ok if valid is None else valid & ok
+    idx = list(torch.broadcast_tensors(*idx))
+    return idx, valid.expand(idx[0].shape)
+
+
+def _scatter_dropping(y: torch.Tensor, index, update):
+    """Applies ``update(target, idx)`` (an in-place scatter) with JAX's rule
+    for updates outside the array: they are dropped.  Such entries are sent
+    to one spare row past the end of each indexed dim, which is cut off
+    afterwards (no host read of the indices, so vmap and graph capture keep
+    working).  Returns the updated copy, or None for an index this does not
+    cover."""
+    r = _int_parts(y, index)
+    if r is None:
+        return None
+    idx, valid = r
+    k = len(idx)
+    inner = tuple(slice(0, sz) for sz in y.shape[:k])
+    ext = y.new_zeros(tuple(sz + 1 for sz in y.shape[:k]) + tuple(y.shape[k:]))
+    ext[inner] = y
+    idx = tuple(torch.where(valid, t, torch.full_like(t, y.shape[d])) for d, t in enumerate(idx))
+    update(ext, idx)
+    return ext[inner].clone()
+
+
 def at_set(x, index, value):
-    """``x.at[index].set(value)`` (out of place)."""
-    y = _t(x).clone()
+    """``x.at[index].set(value)`` (out of place); an integer index outside
+    x is dropped, as JAX's scatter does."""
+    y = _t(x)
+    v = value.to(y.dtype) if isinstance(value, torch.Tensor) else \
+        torch.as_tensor(value, dtype=y.dtype, device=y.device)
+
+    def put(t, idx):
+        t[idx] = v
+
+    out = _scatter_dropping(y, index, put)
+    if out is not None:
+        return out
+    y = y.clone()
     y[index] = value
     return y
 
 
 def at_add(x, index, value):
     """``x.at[index].add(value)`` (out of place): the value is converted to
-    x's dtype; repeated integer indices accumulate."""
-    y = _t(x).clone()
+    x's dtype; repeated integer indices accumulate; indices outside x are
+    dropped."""
+    y = _t(x)
     v = value.to(y.dtype) if isinstance(value, torch.Tensor) else \
         torch.as_tensor(value, dtype=y.dtype, device=y.device)
     if isinstance(index, list):
         index = torch.as_tensor(index, device=y.device)
-    parts = index if isinstance(index, tuple) else (index,)
-    if builtins.all(_int_index(i) for i in parts):
-        idx = tuple(i.to(device=y.device, dtype=torch.long) for i in parts)
-        y.index_put_(idx, v.expand_as(y[idx]), accumulate=True)
-        return y
+
+    def acc(t, idx):
+        t.index_put_(idx, v.expand_as(t[idx]), accumulate=True)
+
+    out = _scatter_dropping(y, index, acc)
+    if out is not None:
+        return out
+    y = y.clone()
     y[index] = y[index] + v
     return y
 
@@ -630,7 +712,14 @@ class _AtRef:
         self.x, self.index = x, index
 
     def get(self):
-        return self.x[self.index]
+        """Gather with JAX's rule: negative indices count from the end and
+        an index still outside is clamped."""
+        r = _int_parts(self.x, self.index)
+        if r is None:
+            return self.x[self.index]
+        idx, _ = r
+        return self.x[tuple(t.clamp(0, builtins.max(self.x.shape[d] - 1, 0))
+                            for d, t in enumerate(idx))]
 
     def set(self, value):
         return at_set(self.x, self.index, value)
@@ -639,16 +728,22 @@ class _AtRef:
         return at_add(self.x, self.index, value)
 
     def _reduce(self, value, how: str, elementwise):
-        y = self.x.clone()
+        y = self.x
         v = value.to(y.dtype) if isinstance(value, torch.Tensor) else \
             torch.as_tensor(value, dtype=y.dtype, device=y.device)
         if _int_index(self.index) and self.index.dim() == 1:
-            idx = self.index.to(device=y.device, dtype=torch.long)
-            idx = torch.where(idx < 0, idx + y.shape[0], idx)
-            src = v.expand((idx.shape[0],) + tuple(y.shape[1:]))
-            with warnings.catch_warnings():   # torch marks index_reduce as beta
-                warnings.simplefilter("ignore", UserWarning)
-                return y.index_reduce_(0, idx, src.contiguous(), how, include_self=True)
+            def red(t, idx):
+                src = v.expand((idx[0].shape[0],) + tuple(t.shape[1:]))
+                with warnings.catch_warnings():   # torch marks index_reduce as beta
+                    warnings.simplefilter("ignore", UserWarning)
+                    t.index_reduce_(0, idx[0], src.contiguous(), how, include_self=True)
+        else:
+            def red(t, idx):
+                t[idx] = elementwise(t[idx], v)
+        out = _scatter_dropping(y, self.index, red)
+        if out is not None:
+            return out
+        y = y.clone()
         y[self.index] = elementwise(y[self.index], v)
         return y
 
@@ -664,6 +759,12 @@ class _AtRef:
         return self.x / den
 
     def power(self, value):
+        def pw(t, idx):
+            t[idx] = t[idx] ** value
+
+        out = _scatter_dropping(self.x, self.index, pw)
+        if out is not None:
+            return out
         y = self.x.clone()
         y[self.index] = y[self.index] ** value
         return y
@@ -675,6 +776,12 @@ class _AtRef:
         return self._reduce(value, "amax", torch.maximum)
 
     def apply(self, fn):
+        def ap(t, idx):
+            t[idx] = fn(t[idx])
+
+        out = _scatter_dropping(self.x, self.index, ap)
+        if out is not None:
+            return out
         y = self.x.clone()
         y[self.index] = fn(y[self.index])
         return y

@@ -340,3 +340,48 @@ def test_gradient_run_refuses_host_reads_of_parameters():
         assert bool(p > 0) is True
         assert float(torch.tensor(2.0)) == 2.0
         assert (p * 2).detach().item() == 1.0
+
+
+def test_at_updates_and_take_follow_jax_out_of_range_rules():
+    """x.at[i] scatters drop integer indices outside x after a negative index
+    is counted from the end; x.at[i].get() clamps; jnp.take fills (NaN for
+    floats, the most negative value for signed integers) -- JAX's defaults
+    (jax.numpy.ndarray.at, jax.numpy.take).  CPU tensors: no GPU fault is
+    possible, and the same torch ops run on the GPU."""
+    import math
+
+    import torch
+
+    from jaxabm_amd import numpy as jnp
+
+    x = torch.arange(5.0)
+    assert x.at[torch.tensor([0, 7, -1, -9])].set(torch.tensor([10.0, 11, 12, 13])).tolist() == \
+        [10.0, 1, 2, 3, 12]
+    assert x.at[torch.tensor([1, 1, 9])].add(1.0).tolist() == [0.0, 3, 2, 3, 4]
+    assert x.at[7].set(3.0).tolist() == x.tolist()
+    assert x.at[-1].set(3.0).tolist() == [0.0, 1, 2, 3, 3]
+    assert x.at[torch.tensor([1, 1, 9])].multiply(2.0).tolist() == [0.0, 4, 2, 3, 4]
+    assert x.at[torch.tensor([2, 8, -6])].max(100.0).tolist() == [0.0, 1, 100, 3, 4]
+    assert x.at[torch.tensor([3, 5])].min(-1.0).tolist() == [0.0, 1, 2, -1, 4]
+    assert x.at[torch.tensor([4, 6])].power(2).tolist() == [0.0, 1, 2, 3, 16]
+    assert x.at[torch.tensor([0, 10])].apply(lambda v: v + 7).tolist() == [7.0, 1, 2, 3, 4]
+    assert x.at[torch.tensor([9, -1, -7])].get().tolist() == [4.0, 4, 0]
+    m = torch.zeros(3, 4)
+    got = m.at[torch.tensor([0, 3, -1]), torch.tensor([1, 1, -2])].add(5.0)
+    want = torch.zeros(3, 4)
+    want[0, 1] = 5.0
+    want[2, 2] = 5.0
+    assert torch.equal(got, want)
+    # slices and boolean masks keep torch's (in-range) indexing
+    assert x.at[1:3].set(9.0).tolist() == [0.0, 9, 9, 3, 4]
+    assert x.at[torch.tensor([True, False, True, False, False])].set(1.0).tolist() == \
+        [1.0, 1, 1, 3, 4]
+    t = jnp.take(x, torch.tensor([0, 9, -1, -7]))
+    assert t[0] == 0 and math.isnan(t[1]) and t[2] == 4 and math.isnan(t[3])
+    ti = jnp.take(torch.arange(5, dtype=torch.int32), torch.tensor([9, 1]))
+    assert ti.tolist() == [torch.iinfo(torch.int32).min, 1]
+    t2 = jnp.take(torch.arange(12.0).reshape(3, 4), torch.tensor([0, 5, -1]), axis=1)
+    assert t2[:, 0].tolist() == [0.0, 4, 8] and t2[:, 2].tolist() == [3.0, 7, 11]
+    assert torch.isnan(t2[:, 1]).all()
+    # the input is never modified
+    assert x.tolist() == [0.0, 1, 2, 3, 4]

@@ -362,3 +362,34 @@ def test_generic_trader_matches_oracle(J, capsys, fused, graph):
     np.testing.assert_array_equal(st["x"].cpu().numpy(), x)
     assert int(res["total_w"][-1]) == int(w.sum())
     np.testing.assert_array_equal(np.asarray(m._rng, np.uint32), rng)
+
+
+def test_env_callback_that_updates_its_env_in_place(J):
+    """An env callback that writes into the dict it receives and returns that
+    same dict (env['p'] = f(env['p']); return env) replays as the eager loop
+    does: every step advances from the previous step's env."""
+    import torch
+
+    from jaxabm_amd import numpy as jnp
+
+    def inplace(env, agents, params, key):
+        env["p"] = env["p"] * 1.5 + jnp.mean(agents["c"]["value"])
+        env["q"] = env["q"] + 1
+        return env
+
+    def metrics(env, agents, params):
+        return {"p": env["p"], "q": env["q"]}
+
+    out = []
+    for graph in (False, True):
+        m = J.JaxModel(params={}, config=J.ModelConfig(steps=30, seed=3),
+                       update_state_fn=inplace, metrics_fn=metrics)
+        m.add_agent_collection("c", J.AgentCollection(Dummy(), 64))
+        m.add_env_state("p", torch.tensor(0.5, device="cuda"))
+        m.add_env_state("q", torch.tensor(0, dtype=torch.int32, device="cuda"))
+        m.add_env_state("increment", 0.25)
+        res = m.run(30, graph=graph)
+        assert m.last_run_info["mode"] == ("graph" if graph else "eager")
+        out.append(([float(v) for v in res["p"]], [int(v) for v in res["q"]]))
+    assert out[0] == out[1]
+    assert out[0][1] == list(range(1, 31))
