@@ -560,7 +560,8 @@ def serving_summary(n_requests: int) -> dict:
     r = serving.benchmark(n_requests=n_requests, rate=200.0, seed=0)
     b, s = r["batched"], r["solo"]
     return {
-        "metric": "sims/s, batched serving vs one request at a time (same GPU, same kernels)",
+        "metric": "sims/s, batched serving vs one request at a time (the server at max_active 1, "
+                  "and plain single-instance engines)",
         "sims_per_s": b["saturated"]["sims_per_s"],
         "solo_sims_per_s": s["saturated"]["sims_per_s"],
         "speedup": r["speedup"],
@@ -576,6 +577,15 @@ def serving_summary(n_requests: int) -> dict:
         "peak_active": b["saturated"]["peak_active"],
         "batched_config": b["config"],
         "solo_config": s["config"],
+        "plain_engine_sims_per_s": r["plain_engine"]["sims_per_s"],
+        "speedup_vs_plain_engine": r["speedup_vs_plain"],
+        "identical_results_plain_engine": r["identical_plain"],
+        "plain_engine_p50_ms": r["plain_engine"]["p50_ms"],
+        "plain_engine_p99_ms": r["plain_engine"]["p99_ms"],
+        "plain_engine_note": ("each request alone through SIRNetworkEngine / WealthEngine, one "
+                              "after another; device set-up, run and result copy timed, host "
+                              "BA graph build excluded; latency from a single-server queue over "
+                              "the measured service times"),
     }
 
 

@@ -246,6 +246,14 @@ int abm_wealth_init(const uint32_t *init_key, int64_t n, float *x, float *y, int
  * state in registers.  Same results either way. */
 int abm_wealth_run_batch(const abm_wealth_inst *insts, int32_t count, int64_t max_n, int mode,
                          void *stream);
+/* One instance across the whole GPU, for large n: two launches per step over
+ * all workgroups (each agent's draws computed and applied, exchange atomics,
+ * with the previous step's mean and Gini recorded by workgroup 0; then the
+ * exchange applied, the wealth histogram, total and maximum).  inst: the
+ * descriptor in host memory (passed to the kernels by value); its draws
+ * field is not used.  acc: device scratch uint32[2].  Same results as
+ * abm_wealth_run_batch. */
+int abm_wealth_run_grid(const abm_wealth_inst *inst, uint32_t *acc, int mode, void *stream);
 
 /* ---- Moore-8 neighbourhood reductions over a (rows, cols) row-major Grid
  *      array (the reference Grid stores only shape and periodic flag,

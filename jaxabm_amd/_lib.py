@@ -164,6 +164,7 @@ SIGNATURES = {
     "abm_sir_states": (_int, [_p, _i64, _p, _p]),
     "abm_wealth_init": (_int, [_p, _i64, _p, _p, _p, _int, _p]),
     "abm_wealth_run_batch": (_int, [_p, _i32, _i64, _int, _p]),
+    "abm_wealth_run_grid": (_int, [_p, _p, _int, _p]),
     "abm_permutation_batch_workspace": (_sz, [_i64, _i32]),
     "abm_permutation_batch": (_int, [_p, _i64, _i32, _i64, _p, _p, _sz, _int, _p]),
     "abm_deffuant_init": (_int, [_p, _i64, _i32, _p, _int, _p]),

@@ -63,6 +63,15 @@ constexpr int kHeavyDeg = 64;                  // rows scanned by a whole wavefr
 constexpr int kResHeavyCap = 1024;
 constexpr int kStepHeavyCap = 256;
 constexpr int kStepBins = 256;                 // per-block wealth bins (stepwise apply)
+// Stepwise SIR skips an S agent whose neighbours cannot be infected: an agent
+// infected after step s (newly, or on its refresh phase (s + v) % H == 0)
+// writes mark = s + H into each neighbour; at step t an S agent scans its row
+// only if (mark - t) mod 2^16 <= H.  An agent infected at the start of step t
+// wrote a mark in some step of [t - H, t - 1] with a value in [t, t + H - 1],
+// and later writes store larger values, so no S agent with an infected
+// neighbour is skipped (stale marks only cost a scan).
+constexpr uint32_t kMarkHorizon = 16;
+constexpr uint16_t kMarkFar = 0x8000;          // initial mark: far from every step
 
 struct Inst {
   int32_t kind, steps, t0, n;
@@ -71,6 +80,7 @@ struct Inst {
   const uint32_t *keys;  // [steps][4]: SIR (k_inf, k_rec); wealth (ck, uk)
   const int32_t *row_ptr, *col;
   uint8_t *state0, *state1;  // SIR: resident home copy / stepwise double buffer
+  uint16_t *mark;            // SIR: neighbour-infection marks (stepwise path)
   int32_t *counts;           // SIR [steps][3]
   float *x, *y;
   int32_t *w, *inc, *hist, *acc;  // wealth: hist[n+1], acc = (max wealth, total)
@@ -216,6 +226,14 @@ __device__ void sir_resident(const Inst &in, int32_t t_end, int mode, uint8_t *l
                              int32_t *heavy, int32_t *nheavy, uint32_t *red) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t n = in.n;
+  // a step after one that left no infected agent is the identity: its
+  // counts are the previous step's, and the state (not part of the results)
+  // need not be touched
+  if (in.t0 > 0 && in.counts[3 * (in.t0 - 1) + 1] == 0) {
+    for (int64_t i = tid; i < 3 * (int64_t)(t_end - in.t0); i += kResThreads)
+      in.counts[3 * in.t0 + i] = in.counts[3 * (in.t0 - 1) + i % 3];
+    return;
+  }
   uint8_t *cur = lds, *nxt = lds + kResidentMax;
   for (int64_t i = tid; i < n; i += kResThreads) cur[i] = in.state0[i];
   for (int32_t t = in.t0; t < t_end; ++t) {
@@ -262,6 +280,11 @@ __device__ void sir_resident(const Inst &in, int32_t t_end, int mode, uint8_t *l
     uint8_t *tmp = cur;
     cur = nxt;
     nxt = tmp;
+    if (ci == 0) {  // block-uniform: every later step is the identity
+      for (int64_t i = tid; i < 3 * (int64_t)(t_end - 1 - t); i += kResThreads)
+        in.counts[3 * (t + 1) + i] = (int32_t)(i % 3 == 0 ? cs : (i % 3 == 1 ? 0u : cr));
+      break;
+    }
   }
   __syncthreads();
   for (int64_t i = tid; i < n; i += kResThreads) in.state0[i] = cur[i];
@@ -336,7 +359,8 @@ __global__ void __launch_bounds__(kStepThreads)
     k_serve_phase1(const Inst *__restrict__ insts, const Range *__restrict__ ranges, int32_t s,
                    int mode) {
   __shared__ int32_t heavy[kStepHeavyCap];
-  __shared__ int32_t nheavy;
+  __shared__ int32_t marking[kStepHeavyCap];
+  __shared__ int32_t nheavy, nmark;
   __shared__ uint32_t red[kStepWaves];
   const Range r = ranges[blockIdx.x];
   const Inst in = insts[r.inst];
@@ -355,15 +379,28 @@ __global__ void __launch_bounds__(kStepThreads)
     }
     return;
   }
+  // identity step (the previous one, computed by an earlier launch, left no
+  // infected agent): only the counts carry over; the state, not part of the
+  // results, is never read again
+  if (t > 0 && in.counts[3 * (t - 1) + 1] == 0) {
+    if (r.begin == 0 && tid < 3) in.counts[3 * t + tid] = in.counts[3 * (t - 1) + tid];
+    return;
+  }
   const Key kinf{in.keys[4 * t], in.keys[4 * t + 1]};
   const Key krec{in.keys[4 * t + 2], in.keys[4 * t + 3]};
   const uint8_t *old = (t & 1) ? in.state1 : in.state0;
   uint8_t *nxt = (t & 1) ? in.state0 : in.state1;
-  if (tid == 0) nheavy = 0;
+  const uint16_t new_mark = (uint16_t)((uint32_t)t + kMarkHorizon);
+  if (tid == 0) nheavy = nmark = 0;
   __syncthreads();
   uint32_t cs = 0, ci = 0, cr = 0;
   for (int64_t v = r.begin + tid; v < r.end; v += kStepThreads) {
-    uint8_t st = old[v];
+    const uint8_t st = old[v];
+    if (st == 0 && (uint16_t)(in.mark[v] - (uint16_t)t) > kMarkHorizon) {
+      nxt[v] = 0;  // no neighbour can be infected
+      ++cs;
+      continue;
+    }
     const int32_t e0 = in.row_ptr[v], e1 = in.row_ptr[v + 1];
     if (st == 0 && e1 - e0 > kHeavyDeg) {
       const int32_t slot = atomicAdd(&nheavy, 1);
@@ -372,22 +409,41 @@ __global__ void __launch_bounds__(kStepThreads)
         continue;
       }
     }
-    st = sir_light(in, old, v, st, e0, e1, kinf, krec, mode);
-    nxt[v] = st;
-    cs += st == 0;
-    ci += st == 1;
-    cr += st == 2;
+    const uint8_t ns = sir_light(in, old, v, st, e0, e1, kinf, krec, mode);
+    nxt[v] = ns;
+    cs += ns == 0;
+    ci += ns == 1;
+    cr += ns == 2;
+    if (ns == 1 && (st == 0 || ((uint32_t)t + (uint32_t)v) % kMarkHorizon == 0)) {
+      if (e1 - e0 > kHeavyDeg) {
+        const int32_t slot = atomicAdd(&nmark, 1);
+        if (slot < kStepHeavyCap) {
+          marking[slot] = (int32_t)v;
+          continue;
+        }
+      }
+      for (int32_t e = e0; e < e1; ++e) in.mark[in.col[e]] = new_mark;
+    }
   }
   __syncthreads();
   const int32_t nh = min(nheavy, kStepHeavyCap);
   for (int32_t h = wave; h < nh; h += kStepWaves) {
     const int32_t v = heavy[h];
-    const bool hit = sir_heavy_hit(in, old, in.row_ptr[v], in.row_ptr[v + 1], kinf, mode);
+    const int32_t e0 = in.row_ptr[v], e1 = in.row_ptr[v + 1];
+    const bool hit = sir_heavy_hit(in, old, e0, e1, kinf, mode);
     if (lane == 0) {
       nxt[v] = hit ? 1 : 0;
       ci += hit;
       cs += !hit;
     }
+    if (hit)  // wave-uniform: the newly infected hub marks its neighbours
+      for (int32_t e = e0 + lane; e < e1; e += 64) in.mark[in.col[e]] = new_mark;
+  }
+  const int32_t nm = min(nmark, kStepHeavyCap);
+  for (int32_t h = wave; h < nm; h += kStepWaves) {
+    const int32_t v = marking[h];
+    for (int32_t e = in.row_ptr[v] + lane; e < in.row_ptr[v + 1]; e += 64)
+      in.mark[in.col[e]] = new_mark;
   }
   cs = block_sum<kStepThreads>(cs, red);
   ci = block_sum<kStepThreads>(ci, red);
@@ -471,13 +527,23 @@ __global__ void k_serve_init_wealth(Inst in, uint32_t ik0, uint32_t ik1, int mod
 
 __global__ void k_serve_init_sir(Inst in) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < in.n) in.state0[i] = 0;
+  if (i < in.n) {
+    in.state0[i] = 0;
+    in.mark[i] = kMarkFar;
+  }
   if (i < 3 * (int64_t)in.steps) in.counts[i] = 0;
 }
 
+// The initial infections, and their marks (value H: at risk for steps 0 .. H,
+// after which the refresh phases take over).
 __global__ void k_serve_seed_sir(Inst in, const int32_t *__restrict__ infected, int32_t n0) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n0) in.state0[infected[i]] = 1;
+  if (i < n0) {
+    const int32_t v = infected[i];
+    in.state0[v] = 1;
+    for (int32_t e = in.row_ptr[v]; e < in.row_ptr[v + 1]; ++e)
+      in.mark[in.col[e]] = (uint16_t)kMarkHorizon;
+  }
 }
 
 // ---------------------------------------------------------------- host side
@@ -595,7 +661,7 @@ struct Req {
   Pinned staging;
   size_t upload_bytes = 0, dev_bytes = 0, result_off = 0, result_bytes = 0;
   size_t off_keys = 0, off_rowptr = 0, off_col = 0, off_infected = 0, off_state0 = 0,
-         off_state1 = 0, off_counts = 0, off_x = 0, off_y = 0, off_w = 0, off_inc = 0,
+         off_state1 = 0, off_mark = 0, off_counts = 0, off_x = 0, off_y = 0, off_w = 0, off_inc = 0,
          off_hist = 0, off_acc = 0, off_mean = 0, off_gini = 0;
   int64_t nnz = 0;
   int32_t n0 = 0;
@@ -876,6 +942,7 @@ void Server::prepare(Req *r) {
     r->upload_bytes = off;
     r->off_state0 = place((size_t)n);
     r->off_state1 = place((size_t)n);
+    r->off_mark = place(2 * (size_t)n);
     r->off_counts = place(12 * (size_t)steps);
     r->result_off = r->off_counts;
     r->result_bytes = 12 * (size_t)steps;
@@ -989,6 +1056,7 @@ void Server::admit() {
       d.col = (const int32_t *)(base + r->off_col);
       d.state0 = (uint8_t *)(base + r->off_state0);
       d.state1 = (uint8_t *)(base + r->off_state1);
+      d.mark = (uint16_t *)(base + r->off_mark);
       d.counts = (int32_t *)(base + r->off_counts);
     } else {
       d.x = (float *)(base + r->off_x);

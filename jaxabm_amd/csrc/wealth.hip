@@ -24,6 +24,8 @@
 // round trips; larger instances run on their global buffers.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "common.hpp"
 #include "jaxabm_amd.h"
 
@@ -326,6 +328,123 @@ __global__ void __launch_bounds__(kWealthThreads)
   }
 }
 
+// ---- one large instance across the whole GPU (abm_wealth_run_grid): two
+// launches per step.  K1(t): every workgroup draws step t for its agents and
+// applies the draws (positions; one exchange atomic per giving agent), and
+// workgroup 0 first records step t-1 from the histogram.  K2(t): the
+// exchange is applied, the wealth histogram built (per-workgroup LDS bins,
+// flushed with one global atomic per bin in use: wealth stays bunched on a
+// few small values, so per-agent global atomics would all hit the same few
+// words), and the total and the largest wealth accumulated.  The same integer
+// and IEEE float operations as the one-workgroup run, so the same bits.
+constexpr int kGridThreads = 256;
+constexpr int kGridBins = 1024;     // LDS histogram bins per workgroup
+constexpr int64_t kGridMaxBlocks = 1024;
+
+// Mean and Gini of step t from hist[0 .. maxw] and the total; clears the
+// bins and the accumulators (one workgroup).
+__device__ void wealth_grid_record(const abm_wealth_inst &in, int32_t t, uint32_t *acc,
+                                   uint32_t *red) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t n = in.n;
+  const uint32_t T = acc[0];
+  const int64_t maxw = (int64_t)acc[1];
+  __shared__ uint32_t carry;
+  if (tid == 0) carry = 0;
+  __syncthreads();
+  uint32_t S_part = 0;
+  for (int64_t base = 0; base <= maxw; base += kGridThreads) {
+    const int64_t w = base + tid;
+    const uint32_t c = w <= maxw ? (uint32_t)in.hist[w] : 0u;
+    uint32_t inc = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(inc, off);
+      if (lane >= off) inc += y;
+    }
+    if (lane == 63) red[wave] = inc;
+    __syncthreads();
+    uint32_t wave_off = 0;
+    for (int q = 0; q < wave; ++q) wave_off += red[q];
+    const uint32_t before = carry + wave_off + inc - c;
+    if (c) {
+      const uint64_t ranksum = (uint64_t)c * before + (uint64_t)c * (c + 1) / 2;
+      S_part += (uint32_t)w * (uint32_t)ranksum;
+      in.hist[w] = 0;
+    }
+    __syncthreads();
+    if (tid == kGridThreads - 1) carry = before + c;
+    __syncthreads();
+  }
+  const uint32_t S = block_sum_u32(S_part, red);
+  if (tid == 0) {
+    in.mean_out[t] = (float)(int32_t)T / (float)n;
+    const int32_t Ti = (int32_t)T;
+    float g = 0.0f;
+    if ((float)Ti > 1e-6f) {
+      const int32_t num = (int32_t)(2u * S);
+      const int32_t den = (int32_t)((uint32_t)n * T);
+      const float q = (float)num / (float)den;
+      const float cst = (float)((double)(n + 1) / (double)n);
+      g = q - cst;
+    }
+    in.gini_out[t] = g;
+    acc[0] = 0;
+    acc[1] = 0;
+  }
+}
+
+__global__ void __launch_bounds__(kGridThreads)
+    k_wealth_grid_agents(abm_wealth_inst in, int32_t t, uint32_t *acc, int mode) {
+  __shared__ uint32_t red[kGridThreads / 64];
+  if (blockIdx.x == 0 && t > 0) wealth_grid_record(in, t - 1, acc, red);
+  if (t >= in.steps) return;
+  const int64_t n = in.n;
+  const int64_t stride = (int64_t)gridDim.x * kGridThreads;
+  for (int64_t i = (int64_t)blockIdx.x * kGridThreads + threadIdx.x; i < n; i += stride) {
+    const WealthDraw d = wealth_draw(in.keys, t, n, i, mode);
+    in.x[i] = mod1(in.x[i] + d.mdx);
+    in.y[i] = mod1(in.y[i] + d.mdy);
+    if (in.w[i] > 0) atomicAdd(&in.inc[d.target], 1);
+  }
+}
+
+__global__ void __launch_bounds__(kGridThreads)
+    k_wealth_grid_apply(abm_wealth_inst in, uint32_t *acc) {
+  __shared__ uint32_t red[kGridThreads / 64];
+  __shared__ uint32_t bins[kGridBins];
+  for (int b = threadIdx.x; b < kGridBins; b += kGridThreads) bins[b] = 0;
+  __syncthreads();
+  const int64_t n = in.n;
+  const int64_t stride = (int64_t)gridDim.x * kGridThreads;
+  uint32_t total = 0;
+  int32_t mx = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kGridThreads + threadIdx.x; i < n; i += stride) {
+    const int32_t w0 = in.w[i];
+    const int32_t wi = w0 - (w0 > 0 ? 1 : 0) + in.inc[i];
+    in.w[i] = wi;
+    in.inc[i] = 0;
+    // 0 <= wi <= n: wealth is conserved
+    if (wi < kGridBins)
+      atomicAdd(&bins[wi], 1u);
+    else
+      atomicAdd(&in.hist[wi], 1);
+    total += (uint32_t)wi;
+    mx = max(mx, wi);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off));
+  const uint32_t T = block_sum_u32(total, red);  // its barriers order the bins too
+  for (int b = threadIdx.x; b < kGridBins; b += kGridThreads)
+    if (bins[b]) atomicAdd(&in.hist[b], (int32_t)bins[b]);
+  if ((threadIdx.x & 63) == 0) atomicMax(&acc[1], (uint32_t)mx);
+  if (threadIdx.x == 0) atomicAdd(&acc[0], T);
+}
+
+__global__ void k_wealth_zero_acc(uint32_t *acc) {
+  if (threadIdx.x < 2) acc[threadIdx.x] = 0;
+}
+
 __global__ void k_wealth_init(const uint32_t *__restrict__ init_key, int64_t n,
                               float *__restrict__ x, float *__restrict__ y,
                               int32_t *__restrict__ w, int mode) {
@@ -336,6 +455,26 @@ __global__ void k_wealth_init(const uint32_t *__restrict__ init_key, int64_t n,
   x[i] = unit_float(bits_scalar(kx, mode));
   y[i] = unit_float(bits_scalar(ky, mode));
   w[i] = 1;
+}
+
+// The LDS-resident run needs more than 64 KB of dynamic LDS: the limit is
+// raised once per device, under a lock.
+constexpr int kMaxDevices = 64;
+std::mutex g_wealth_attr_mu;
+bool g_wealth_attr_set[kMaxDevices];
+
+int wealth_lds_attr() {
+  int dev = -1;
+  ABM_CHECK_HIP(hipGetDevice(&dev));
+  ABM_REQUIRE(dev >= 0 && dev < kMaxDevices, "wealth: device index out of range");
+  std::lock_guard<std::mutex> lock(g_wealth_attr_mu);
+  if (!g_wealth_attr_set[dev]) {
+    ABM_CHECK_HIP(hipFuncSetAttribute((const void *)k_wealth_run<true>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)(kWealthLdsAgents * kWealthLdsBytesPerAgent + 4)));
+    g_wealth_attr_set[dev] = true;
+  }
+  return kOk;
 }
 
 }  // namespace abm
@@ -355,6 +494,28 @@ int abm_wealth_init(const uint32_t *init_key, int64_t n, float *x, float *y, int
   return kOk;
 }
 
+int abm_wealth_run_grid(const abm_wealth_inst *inst, uint32_t *acc, int mode, void *stream) {
+  ABM_REQUIRE(inst && acc && (mode == 0 || mode == 1), "abm_wealth_run_grid: bad arguments");
+  const abm_wealth_inst in = *inst;
+  ABM_REQUIRE(in.n >= 2 && in.n < (1ll << 31) && in.steps >= 0 && in.x && in.y && in.w &&
+                  in.inc && in.hist && in.keys && in.mean_out && in.gini_out,
+              "abm_wealth_run_grid: incomplete instance");
+  hipStream_t s = (hipStream_t)stream;
+  if (in.steps == 0) return kOk;
+  hipLaunchKernelGGL(k_wealth_zero_acc, dim3(1), dim3(64), 0, s, acc);
+  int64_t blocks = (in.n + kGridThreads - 1) / kGridThreads;
+  if (blocks > kGridMaxBlocks) blocks = kGridMaxBlocks;
+  for (int32_t t = 0; t <= in.steps; ++t) {
+    hipLaunchKernelGGL(k_wealth_grid_agents, dim3((unsigned)blocks), dim3(kGridThreads), 0, s,
+                       in, t, acc, mode);
+    if (t < in.steps)
+      hipLaunchKernelGGL(k_wealth_grid_apply, dim3((unsigned)blocks), dim3(kGridThreads), 0, s,
+                         in, acc);
+  }
+  ABM_LAUNCH_CHECK("abm_wealth_run_grid");
+  return kOk;
+}
+
 int abm_wealth_run_batch(const abm_wealth_inst *insts, int32_t count, int64_t max_n, int mode,
                          void *stream) {
   ABM_REQUIRE(insts && count >= 0 && max_n >= 0 && (mode == 0 || mode == 1),
@@ -363,13 +524,7 @@ int abm_wealth_run_batch(const abm_wealth_inst *insts, int32_t count, int64_t ma
   hipLaunchKernelGGL(k_wealth_draws, dim3(512u, (unsigned)count), dim3(256), 0,
                      (hipStream_t)stream, insts, mode);
   if (max_n > 0 && max_n <= kWealthLdsAgents) {
-    static bool attr_set = false;
-    if (!attr_set) {
-      ABM_CHECK_HIP(hipFuncSetAttribute((const void *)k_wealth_run<true>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)(kWealthLdsAgents * kWealthLdsBytesPerAgent + 4)));
-      attr_set = true;
-    }
+    if (int rc = wealth_lds_attr()) return rc;
     size_t dyn = (size_t)(max_n * kWealthLdsBytesPerAgent + 4);
     if (dyn < 2 * kWealthThreads * sizeof(int32_t)) dyn = 2 * kWealthThreads * sizeof(int32_t);
     hipLaunchKernelGGL(k_wealth_run<true>, dim3((unsigned)count), dim3(kWealthThreads), dyn,

@@ -22,6 +22,11 @@ from .. import random as jr
 from .._lib import WEALTH_INST, check, lib, ptr, require_gpu, stream_handle
 
 
+# above the LDS capacity of one workgroup (8,000 agents), a run spreads over
+# the whole GPU
+GRID_MIN_AGENTS = 8000
+
+
 class WealthEngine:
     """Device state of one wealth-exchange instance."""
 
@@ -69,15 +74,25 @@ class WealthEngine:
         return d
 
     def run(self, steps: int):
-        """Run ``steps`` steps; returns host (mean, gini) float32 arrays."""
+        """Run ``steps`` steps; returns host (mean, gini) float32 arrays.
+        Up to GRID_MIN_AGENTS agents one workgroup runs the whole simulation
+        out of LDS; larger instances spread every step over the whole GPU
+        (abm_wealth_run_grid).  Same results either way."""
         keys = self.key_table(steps)
         mean = torch.empty(max(steps, 1), dtype=torch.float32, device=self.device)
         gini = torch.empty(max(steps, 1), dtype=torch.float32, device=self.device)
-        draws = self.draw_buffer(steps)
-        desc = torch.from_numpy(self.descriptor(keys, steps, mean, gini, draws)
-                                .view(np.uint8)).to(self.device)
-        check(lib().abm_wealth_run_batch(ptr(desc), 1, self.n, self.mode, stream_handle()),
-              "wealth_run")
+        if self.n > GRID_MIN_AGENTS:
+            # draws computed inline by the step kernels: no draw buffer
+            host = self.descriptor(keys, steps, mean, gini)
+            acc = torch.empty(2, dtype=torch.int32, device=self.device)
+            check(lib().abm_wealth_run_grid(host.ctypes.data, ptr(acc), self.mode,
+                                            stream_handle()), "wealth_run_grid")
+        else:
+            draws = self.draw_buffer(steps)   # held until the run has read it
+            host = self.descriptor(keys, steps, mean, gini, draws)
+            desc = torch.from_numpy(host.view(np.uint8)).to(self.device)
+            check(lib().abm_wealth_run_batch(ptr(desc), 1, self.n, self.mode, stream_handle()),
+                  "wealth_run")
         return mean[:steps].cpu().numpy(), gini[:steps].cpu().numpy()
 
 

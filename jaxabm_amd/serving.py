@@ -350,8 +350,54 @@ def _records_equal(a: dict, b: dict) -> bool:
     return a.keys() == b.keys() and all(np.array_equal(a[k], b[k]) for k in a)
 
 
+def plain_engine_run(trace: list[SimRequest], mode: int | None = None) -> dict:
+    """The plain per-run GPU baseline: each request alone, one after another,
+    through the single-instance engines (``SIRNetworkEngine``, whose step
+    kernel skips quiet rows and agents; ``WealthEngine``, one workgroup per
+    run with every draw made before the run).  Per request the timing covers
+    the engine's device set-up (graph upload, chunking, seeding), the run and
+    the copy of its records to the host; each SIR graph's host CSR is built
+    before the clock starts, so this baseline leaves out work the server's
+    timing includes.  Latencies at the trace's arrival times follow from the
+    measured service times of one in-order executor (a single-server queue)."""
+    import torch
+
+    from .network import ba_csr
+    from .programs.sir import SIRNetworkEngine
+    from .programs.wealth import WealthEngine
+
+    graphs = {i: ba_csr(r.n, r.m, r.graph_seed) for i, r in enumerate(trace) if r.kind == "sir"}
+    records, service = [], []
+    torch.cuda.synchronize()
+    for i, r in enumerate(trace):
+        t0 = time.perf_counter()
+        if r.kind == "sir":
+            eng = SIRNetworkEngine(r.n, csr=graphs[i], beta=r.beta, gamma=r.gamma,
+                                   n_seed_infected=r.n_infected, seed=r.seed, mode=mode)
+            c = eng.run(r.steps)
+            rec = {"S": c[:, 0].copy(), "I": c[:, 1].copy(), "R": c[:, 2].copy()}
+        else:
+            eng = WealthEngine(r.n, seed=r.seed, mode=mode)
+            mean, gini = eng.run(r.steps)
+            rec = {"mean": mean, "gini": gini}
+        service.append(time.perf_counter() - t0)
+        records.append(rec)
+        del eng
+    total = float(np.sum(service))
+    end, lat = 0.0, []
+    for r, dt in zip(trace, service):
+        start = max(end, r.arrival)
+        end = start + dt
+        lat.append((end - r.arrival) * 1e3)
+    lat_a = np.array(lat)
+    return {"records": records, "sims_per_s": len(trace) / total, "total_s": total,
+            "p50_ms": float(np.percentile(lat_a, 50)), "p99_ms": float(np.percentile(lat_a, 99)),
+            "mean_service_ms": total / len(trace) * 1e3}
+
+
 def benchmark(n_requests: int = 400, rate: float = 200.0, seed: int = 0,
-              solo: dict | None = None, batched: dict | None = None, log=None) -> dict:
+              solo: dict | None = None, batched: dict | None = None, log=None,
+              plain: bool = True) -> dict:
     """The serving workload on one trace: one request at a time (max_active =
     1, its own tuned configuration) against continuous batching.
 
@@ -380,4 +426,14 @@ def benchmark(n_requests: int = 400, rate: float = 200.0, seed: int = 0,
                                                                  results["batched"]))
     out["speedup"] = (out["batched"]["saturated"]["sims_per_s"] /
                       out["solo"]["saturated"]["sims_per_s"])
+    if plain:
+        cfg = batched or tuned_config("batched")
+        p = plain_engine_run(trace, mode=int(cfg.get("mode", jr.MODE_PARTITIONABLE)))
+        out["plain_engine"] = {k: v for k, v in p.items() if k != "records"}
+        out["identical_plain"] = all(_records_equal({k: a[k] for k in b}, b)
+                                     for a, b in zip(results["batched"], p["records"]))
+        out["speedup_vs_plain"] = out["batched"]["saturated"]["sims_per_s"] / p["sims_per_s"]
+        if log:
+            log(f"[serving] plain engines: {p['sims_per_s']:.1f} sims/s, p50 {p['p50_ms']:.2f} "
+                f"ms p99 {p['p99_ms']:.2f} ms")
     return out

@@ -93,6 +93,8 @@ def _declare(L):
         "orc_wealth_init": (None, [_i64, _i64, C.c_int, _f32p, _f32p, _i32p, _u32p]),
         "orc_wealth_run": (None, [_i64, _f32p, _f32p, _i32p, _u32p, _i32,
                                   C.c_int, _f32p, _f32p]),
+        "orc_wealth_run_threads": (None, [_i64, _f32p, _f32p, _i32p, _u32p, _i32,
+                                          C.c_int, _f32p, _f32p, C.c_int]),
         "orc_schelling_init": (_i64, [_i32, _i32, _i64, C.c_int, _i8p, _i32p,
                                       _i8p, _u32p]),
         "orc_schelling_run": (None, [_i32, _i32, _i8p, _i32p, _i8p, _i64,
@@ -306,6 +308,15 @@ def wealth_run(x, y, w, rng, steps: int, mode: int = MODE_PARTITIONABLE):
     mean = np.zeros(max(steps, 1), np.float32)
     gini = np.zeros(max(steps, 1), np.float32)
     lib().orc_wealth_run(len(x), x, y, w, rng, steps, mode, mean, gini)
+    return mean[:steps], gini[:steps]
+
+
+def wealth_run_threads(x, y, w, rng, steps: int, threads: int, mode: int = MODE_PARTITIONABLE):
+    """wealth_run with each step's per-agent draws on ``threads`` threads
+    (identical results)."""
+    mean = np.zeros(max(steps, 1), np.float32)
+    gini = np.zeros(max(steps, 1), np.float32)
+    lib().orc_wealth_run_threads(len(x), x, y, w, rng, steps, mode, mean, gini, int(threads))
     return mean[:steps], gini[:steps]
 
 

@@ -749,6 +749,84 @@ void orc_wealth_run(int64_t n, float *x, float *y, int32_t *wealth,
   free(nw);
 }
 
+/* Element f of JAX's threefry_2x32(key, iota(2 num)) as orc_split lays it
+ * out (key i of split(key, num) is elements 2i and 2i+1), and element j of
+ * random_bits(key, (n,)): the per-element forms of orc_split and
+ * orc_random_bits32, for loops over agents on several threads. */
+static uint32_t split_element(const uint32_t key[2], int64_t num, int64_t f, int mode) {
+  uint32_t o[2];
+  if (mode == ORC_MODE_PARTITIONABLE) {
+    const int64_t i = f >> 1;
+    orc_threefry2x32(key[0], key[1], (uint32_t)((uint64_t)i >> 32), (uint32_t)i, o);
+    return o[f & 1];
+  }
+  if (f < num) {
+    orc_threefry2x32(key[0], key[1], (uint32_t)f, (uint32_t)(num + f), o);
+    return o[0];
+  }
+  orc_threefry2x32(key[0], key[1], (uint32_t)(f - num), (uint32_t)f, o);
+  return o[1];
+}
+
+static uint32_t bits_element(const uint32_t key[2], int64_t n, int64_t j, int mode) {
+  if (mode == ORC_MODE_PARTITIONABLE) {
+    uint32_t o[2];
+    orc_threefry2x32(key[0], key[1], (uint32_t)((uint64_t)j >> 32), (uint32_t)j, o);
+    return o[0] ^ o[1];
+  }
+  return orig_element(key, n, j);
+}
+
+/* orc_wealth_run with the per-agent work of each step (agent keys, position
+ * draws, exchange targets) on `threads` OpenMP threads: the same results,
+ * since every draw is a function of (key, index) alone; the exchange and the
+ * records stay sequential. */
+void orc_wealth_run_threads(int64_t n, float *x, float *y, int32_t *wealth, uint32_t rng[2],
+                            int32_t steps, int mode, float *mean_out, float *gini_out,
+                            int threads) {
+  int32_t *tgt = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+  int32_t *nw = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+  const float step_sd = 0.01f;
+  for (int32_t t = 0; t < steps; ++t) {
+    uint32_t ck[2], uk[2], ks[4];
+    step_keys(rng, ck, uk, mode);
+    orc_split(uk, 2, ks, mode);
+    const uint32_t span = n - 1 <= 0 ? 1u : (uint32_t)(n - 1);
+    uint32_t mult = 65536u % span;
+    mult = mult * mult;
+    mult = mult % span;
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+      const uint32_t ak[2] = {split_element(ck, n, 2 * i, mode),
+                              split_element(ck, n, 2 * i + 1, mode)};
+      float dx, dy;
+      agent_xy_draw(ak, &dx, &dy, 1, mode);
+      volatile float mx = step_sd * dx;
+      volatile float my = step_sd * dy;
+      volatile float sx = x[i] + mx;
+      volatile float sy = y[i] + my;
+      x[i] = mod1(sx);
+      y[i] = mod1(sy);
+      const uint32_t hb = bits_element(ks, n, i, mode), lb = bits_element(ks + 2, n, i, mode);
+      uint32_t off = (hb % span) * mult + (lb % span);
+      tgt[i] = (int32_t)(off % span);
+    }
+    for (int64_t i = 0; i < n; ++i) nw[i] = wealth[i];
+    for (int64_t i = 0; i < n; ++i) {
+      int32_t ti = tgt[i] + (tgt[i] >= i ? 1 : 0);
+      if (wealth[i] > 0) {
+        nw[i] -= 1;
+        nw[ti] += 1;
+      }
+    }
+    memcpy(wealth, nw, sizeof(int32_t) * (size_t)n);
+    mean_out[t] = orc_mean_i32(wealth, n);
+    gini_out[t] = orc_gini_i32(wealth, n);
+  }
+  free(tgt);
+  free(nw);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Schelling segregation on a periodic Grid (reference Grid stores shape and
  * the periodic flag only, jaxabm/agentpy.py:465-527; the neighbourhood step is

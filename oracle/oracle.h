@@ -88,6 +88,9 @@ void orc_trader_run(int64_t n, float *x, int32_t *w, uint32_t rng[2], int32_t st
                     int mode);
 void orc_wealth_init(int64_t n, int64_t seed, int mode, float *x, float *y,
                      int32_t *wealth, uint32_t rng[2]);
+void orc_wealth_run_threads(int64_t n, float *x, float *y, int32_t *wealth, uint32_t rng[2],
+                            int32_t steps, int mode, float *mean_out, float *gini_out,
+                            int threads);
 void orc_wealth_run(int64_t n, float *x, float *y, int32_t *wealth,
                     uint32_t rng[2], int32_t steps, int mode, float *mean_out,
                     float *gini_out);

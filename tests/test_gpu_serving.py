@@ -118,3 +118,39 @@ def test_invalid_requests_fail_cleanly(sv):
         assert s.poll(good) == "done"
         with pytest.raises(Exception, match="failed"):
             s.result(bad)
+
+
+def test_benchmark_trace_prefix_matches_oracle(sv):
+    """The first 32 requests of the benchmark's seed-0 trace at full size
+    (agent counts up to 247,720, 100-365 steps, both models): through the
+    batched server with its tuned configuration and through the plain
+    per-request engines, every record equals the oracle's run of the same
+    request bit for bit.  (Partitionable layout: request i of a trace does
+    not depend on the trace's length, so these are the measured trace's
+    first 32.)"""
+    import concurrent.futures as cf
+    import os
+
+    trace = sv.make_trace(32, seed=0)
+    assert sv.make_trace(400, seed=0)[:32] == trace
+    assert max(r.n for r in trace) > 200_000 and {r.kind for r in trace} == {"sir", "wealth"}
+    with sv.SimulationServer(sv.tuned_config("batched")) as s:
+        tickets = [s.submit(r) for r in trace]
+        s.drain(300)
+        served = [s.result(t) for t in tickets]
+    plain = sv.plain_engine_run(trace)["records"]
+    threads = max(1, min(16, os.cpu_count() or 1) // 4)
+
+    def want(req):
+        if req.kind == "wealth":
+            x, y, w, rng = O.wealth_init(req.n, req.seed, O.MODE_PARTITIONABLE)
+            mean, gini = O.wealth_run_threads(x, y, w, rng, req.steps, threads,
+                                              O.MODE_PARTITIONABLE)
+            return {"mean": mean, "gini": gini}
+        return oracle_records(req)
+
+    with cf.ThreadPoolExecutor(4) as ex:
+        wants = list(ex.map(want, trace))
+    for r, a, b, w in zip(trace, served, plain, wants):
+        assert same(a, w), r
+        assert same(b, w), r

@@ -97,3 +97,52 @@ def test_wealth_global_path_matches_oracle(wealth):
     np.testing.assert_array_equal(eng.w.cpu().numpy(), w)
     np.testing.assert_array_equal(mean, om)
     np.testing.assert_array_equal(gini, og)
+
+
+@pytest.mark.parametrize("n,steps,seed", [(8001, 30, 1), (50_000, 40, 2), (300_000, 12, 3)])
+def test_grid_path_matches_oracle_and_one_workgroup_run(wealth, n, steps, seed):
+    """Above 8,000 agents WealthEngine.run spreads each step over the GPU
+    (abm_wealth_run_grid): the oracle's records and state, and the same bits
+    as the one-workgroup run of the batch entry point on global buffers."""
+    import os
+
+    eng = wealth.WealthEngine(n, seed=seed)
+    assert n > wealth.GRID_MIN_AGENTS
+    mean, gini = eng.run(steps)
+    x, y, w, rng = O.wealth_init(n, seed, O.MODE_PARTITIONABLE)
+    om, og = O.wealth_run_threads(x, y, w, rng, steps, max(1, min(16, os.cpu_count() or 1)),
+                                  O.MODE_PARTITIONABLE)
+    np.testing.assert_array_equal(mean, om)
+    np.testing.assert_array_equal(gini, og)
+    np.testing.assert_array_equal(eng.w.cpu().numpy(), w)
+    np.testing.assert_array_equal(eng.x.cpu().numpy(), x)
+    np.testing.assert_array_equal(eng.y.cpu().numpy(), y)
+    one = wealth.WealthEngine(n, seed=seed)
+    m2, g2 = _run_raw(wealth, one, steps, 0, draws=True)
+    np.testing.assert_array_equal(m2, mean)
+    np.testing.assert_array_equal(g2, gini)
+
+
+def test_grid_path_wealth_above_the_lds_bins(wealth):
+    """A state where one agent holds 5,000 units (total wealth 15,000 <= n,
+    the histogram's range): wealth values past the grid path's per-workgroup
+    LDS bins go to the global histogram directly; the records and state equal
+    the oracle's from the same state."""
+    import os
+
+    import torch
+
+    n, steps, seed = 20_000, 25, 5
+    eng = wealth.WealthEngine(n, seed=seed)
+    x, y, w, rng = O.wealth_init(n, seed, O.MODE_PARTITIONABLE)
+    w[:] = 0
+    w[20:10020] = 1
+    w[7] = 5000
+    eng.w.copy_(torch.from_numpy(w.copy()).to(eng.w.device))
+    mean, gini = eng.run(steps)
+    om, og = O.wealth_run_threads(x, y, w, rng, steps, max(1, min(16, os.cpu_count() or 1)),
+                                  O.MODE_PARTITIONABLE)
+    assert w.max() > 1024
+    np.testing.assert_array_equal(mean, om)
+    np.testing.assert_array_equal(gini, og)
+    np.testing.assert_array_equal(eng.w.cpu().numpy(), w)

@@ -359,6 +359,19 @@ def test_sir_fixture_is_the_oracle_run():
         np.testing.assert_array_equal(got, want)
 
 
+@pytest.mark.parametrize("mode", [O.MODE_ORIGINAL, O.MODE_PARTITIONABLE])
+@pytest.mark.parametrize("n,steps", [(1000, 30), (3001, 12), (2, 5)])
+def test_wealth_threaded_run_is_identical(mode, n, steps):
+    """The threaded wealth oracle (per-element key and draw forms on several
+    threads) gives the serial run's records and final state."""
+    x1, y1, w1, r1 = O.wealth_init(n, 4, mode)
+    x2, y2, w2, r2 = x1.copy(), y1.copy(), w1.copy(), r1.copy()
+    m1, g1 = O.wealth_run(x1, y1, w1, r1, steps, mode)
+    m2, g2 = O.wealth_run_threads(x2, y2, w2, r2, steps, 4, mode)
+    for a, b in ((m1, m2), (g1, g2), (x1, x2), (y1, y2), (w1, w2), (r1, r2)):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_sir_threaded_run_is_identical():
     """The all-cores CPU baseline computes the same simulation: each agent's
     next state depends only on the pre-step copy."""
