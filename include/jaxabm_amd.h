@@ -282,8 +282,12 @@ typedef struct abm_schelling_grid {
   int8_t *type;          /* device [H*W]: type of agent k */
   int32_t *target;       /* device [H*W]: scratch */
   int32_t *empty;        /* device [H*W], 16-byte aligned: scratch (empty-cell bitmap) */
-  int32_t *block_counts; /* device [abm_schelling_block_count(H*W)] */
+  int32_t *block_counts; /* device [abm_schelling_block_count(H*W)]: scratch (cells
+                            per 1024-cell block: agents at init; empty cells
+                            during a run, in 8 copies summed by the readers) */
 } abm_schelling_grid;
+/* Length in int32 of the block_counts scratch for a grid of n_cells cells:
+ * 8 x ceil(n_cells / 1024). */
 int64_t abm_schelling_block_count(int64_t n_cells);
 /* Cells from uniform(key, (H*W,)) and the row-major agent numbering; sets
  * g->n_agents (synchronises the stream once). */

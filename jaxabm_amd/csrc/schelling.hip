@@ -13,21 +13,23 @@
 //         move (their old cell is not reusable until the next step)
 //  record fraction happy (agent phase) and number of moves
 //
-// The empty cells are kept as a bitmap (bit c set when cell c is empty); the
-// moves kernel keeps it current with atomic bit flips (a cell changes state at
-// most once per step), so the row-major empty list is never materialised:
-// every agents block counts the empty cells of each 1024-cell block by
-// popcounts of the (L2-resident, 32 KB at 512 x 512) bitmap, scans the counts
-// in LDS, and empty[j] is then a search of that prefix and a popcount scan of
-// one block's 32 words.  Two kernels per step (captured with the rest of the
-// run):
+// The empty cells are kept as a bitmap (bit c set when cell c is empty) and a
+// count per 1024-cell block; the moves kernel keeps both current with atomics
+// (a bit flip per moved agent, and a count update when a move crosses blocks;
+// a cell changes state at most once per step), so the row-major empty list is
+// never materialised: every agents block scans the block counts (1 KB at
+// 512 x 512) into an LDS prefix, and empty[j] is then a search of that prefix
+// and a popcount scan of one block's 32 bitmap words.  Two kernels per step
+// (captured with the rest of the run):
 //   k_schelling_agents  32x32 cell tiles + 1-cell torus halo in LDS;
 //                       happiness, draw, empty[j], atomicMin claim
-//   k_schelling_moves   winners move; bitmap and claims updated
+//   k_schelling_moves   winners move; bitmap, block counts and claims updated
 // Layout (HBM): cells int8[H*W], id int32[H*W] (agent on a cell or -1),
 // claim int32[H*W] (INT32_MAX when unclaimed), pos int32[n_agents],
 // type int8[n_agents], target int32[n_agents], empty-cell bitmap
-// uint32[ceil(H*W/32)] (in the grid's `empty` scratch).
+// uint32[ceil(H*W/32)] (in the grid's `empty` scratch), empty cells per
+// 1024-cell block int32[8][ceil(H*W/1024)] (8 copies summed by readers; in
+// the `block_counts` scratch).
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -42,6 +44,10 @@ constexpr int kCellsPerBlock = 1024;
 constexpr int kWordsPerBlock = kCellsPerBlock / 32;
 // block prefix of the empty counts in LDS: at most 64 KB
 constexpr int64_t kMaxEmptyBlocks = 16383;
+// copies of the per-block empty counts: the moves kernel's workgroup w updates
+// copy w % kCountCopies, so the count atomics of a busy step spread over 8x
+// the words (each word's device-scope atomics serialise); readers sum them
+constexpr int kCountCopies = 8;
 
 // number of cells with (cells[c] == 0) == want_empty per 1024-cell block
 __global__ void __launch_bounds__(kCellsPerBlock)
@@ -116,42 +122,53 @@ __global__ void __launch_bounds__(kCellsPerBlock)
   }
 }
 
-// Number of empty cells of 1024-cell block b: popcount of its 32 bitmap words
-// (eight 16-byte loads issued together).
-__device__ __forceinline__ int32_t block_empty(const uint32_t *__restrict__ ebits,
-                                               int64_t nwords, int nb, int b) {
-  if (b >= nb) return 0;
-  const int64_t w0 = (int64_t)b * kWordsPerBlock;
+// Empty cells of block b < nb: the sum of its count copies.
+__device__ __forceinline__ int32_t block_empty(const int32_t *__restrict__ ecount, int nb,
+                                               int b) {
+  int32_t v[kCountCopies];
+#pragma unroll
+  for (int r = 0; r < kCountCopies; ++r) v[r] = ecount[(int64_t)r * nb + b];
   int32_t s = 0;
-  if (w0 + kWordsPerBlock <= nwords) {
-    const uint4 *p = reinterpret_cast<const uint4 *>(ebits + w0);
-    uint4 v[kWordsPerBlock / 4];
 #pragma unroll
-    for (int q = 0; q < kWordsPerBlock / 4; ++q) v[q] = p[q];
-#pragma unroll
-    for (int q = 0; q < kWordsPerBlock / 4; ++q)
-      s += __popc(v[q].x) + __popc(v[q].y) + __popc(v[q].z) + __popc(v[q].w);
-  } else {
-    for (int q = 0; q < kWordsPerBlock; ++q)
-      if (w0 + q < nwords) s += __popc(ebits[w0 + q]);
-  }
+  for (int r = 0; r < kCountCopies; ++r) s += v[r];
   return s;
 }
 
-// P[0] = 0, P[b + 1] = empty cells of blocks 0 .. b, for b < nb, in LDS (wave
-// scans, chunks of blockDim.x blocks).  first: this thread's count for the
-// first chunk, loaded by the caller ahead of its other loads.
-__device__ void empty_prefix(const uint32_t *__restrict__ ebits, int64_t nwords, int nb,
-                             int32_t first, int32_t *P, int32_t *wsum, int32_t *carry) {
+// P[0] = 0, P[b + 1] = empty cells of blocks 0 .. b, for b < nb, in LDS, from
+// the per-block counts (wave scans).  first: this thread's count for the
+// first chunk of blockDim.x blocks, loaded by the caller ahead of its other
+// loads.  Up to blockDim.x blocks (grids of up to 1M cells at 1024 threads)
+// the scan takes two barriers; larger grids scan chunk by chunk.
+__device__ void empty_prefix(const int32_t *__restrict__ ecount, int nb, int32_t first,
+                             int32_t *P, int32_t *wsum, int32_t *carry) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nthreads = (int)blockDim.x;
+  if (nb <= nthreads) {
+    int32_t v = first;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int32_t y = __shfl_up(v, off);
+      if (lane >= off) v += y;
+    }
+    if (lane == 63) wsum[wave] = v;
+    if (tid == 0) P[0] = 0;
+    __syncthreads();
+    if (tid < nb) {
+      int32_t before = 0;
+      for (int q = 0; q < wave; ++q) before += wsum[q];
+      P[tid + 1] = before + v;
+    }
+    __syncthreads();
+    return;
+  }
   if (tid == 0) {
     P[0] = 0;
     *carry = 0;
   }
   __syncthreads();
-  for (int base = 0; base < nb; base += (int)blockDim.x) {
+  for (int base = 0; base < nb; base += nthreads) {
     const int b = base + tid;
-    int32_t v = base == 0 ? first : block_empty(ebits, nwords, nb, b);
+    int32_t v = base == 0 ? first : (b < nb ? block_empty(ecount, nb, b) : 0);
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
       const int32_t y = __shfl_up(v, off);
@@ -163,7 +180,7 @@ __device__ void empty_prefix(const uint32_t *__restrict__ ebits, int64_t nwords,
     for (int q = 0; q < wave; ++q) before += wsum[q];
     if (b < nb) P[b + 1] = before + v;
     __syncthreads();
-    if (tid == (int)blockDim.x - 1) *carry = before + v;
+    if (tid == nthreads - 1) *carry = before + v;
     __syncthreads();
   }
 }
@@ -209,6 +226,7 @@ struct SchellingStep {
   const int8_t *cells;
   const int32_t *id;
   const uint32_t *ebits;
+  const int32_t *ecount;
   int64_t nwords;
   int32_t nb;
   int32_t *claim, *target;
@@ -229,8 +247,8 @@ __global__ void __launch_bounds__(kTileDim * kTileDim) k_schelling_agents(Schell
   const int r = r0 + ty, c = c0 + tx;
   const bool inside = r < a.H && c < a.W;
   const int64_t cell = (int64_t)r * a.W + c;
-  // one memory round trip: the tile and its torus halo, the bitmap words of
-  // this thread's block for the prefix, the agent on this thread's cell
+  // one memory round trip: the tile and its torus halo, this thread's block
+  // count for the prefix, the agent on this thread's cell
   int8_t tv[2] = {0, 0};
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
@@ -243,14 +261,14 @@ __global__ void __launch_bounds__(kTileDim * kTileDim) k_schelling_agents(Schell
       tv[i] = a.cells[(int64_t)gr * a.W + gc];
     }
   }
-  const int32_t first = block_empty(a.ebits, a.nwords, a.nb, tid);
+  const int32_t first = tid < a.nb ? block_empty(a.ecount, a.nb, tid) : 0;
   const int32_t k = inside ? a.id[cell] : -1;
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
     const int q = tid + i * (int)blockDim.x;
     if (q < kHalo) tile[q / (kTileDim + 2)][q % (kTileDim + 2)] = tv[i];
   }
-  empty_prefix(a.ebits, a.nwords, a.nb, first, empty_prefix_lds, wc, &carry);
+  empty_prefix(a.ecount, a.nb, first, empty_prefix_lds, wc, &carry);
   bool happy = false;
   if (inside) {
     const int8_t me = tile[ty + 1][tx + 1];
@@ -292,6 +310,7 @@ __global__ void k_schelling_moves(int64_t n_agents, int8_t *__restrict__ cells,
                                   int32_t *__restrict__ id, int32_t *__restrict__ claim,
                                   int32_t *__restrict__ pos, const int8_t *__restrict__ type,
                                   const int32_t *__restrict__ target, uint32_t *__restrict__ ebits,
+                                  int32_t *__restrict__ ecount, int32_t nb,
                                   int32_t *__restrict__ moves) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool moved = false;
@@ -309,6 +328,12 @@ __global__ void k_schelling_moves(int64_t n_agents, int8_t *__restrict__ cells,
       // target was empty before the step, an old cell occupied)
       atomicOr(&ebits[old >> 5], 1u << (old & 31));
       atomicAnd(&ebits[tg >> 5], ~(1u << (tg & 31)));
+      const int32_t bo = old / kCellsPerBlock, bt = tg / kCellsPerBlock;
+      if (bo != bt) {
+        int32_t *copy = ecount + (int64_t)(blockIdx.x % kCountCopies) * nb;
+        atomicAdd(&copy[bo], 1);
+        atomicSub(&copy[bt], 1);
+      }
       moved = true;
     }
   }
@@ -367,7 +392,7 @@ using namespace abm;
 extern "C" {
 
 int64_t abm_schelling_block_count(int64_t n_cells) {
-  return (n_cells + kCellsPerBlock - 1) / kCellsPerBlock;
+  return kCountCopies * ((n_cells + kCellsPerBlock - 1) / kCellsPerBlock);
 }
 
 int abm_schelling_init(abm_schelling_grid *g, const uint32_t *key, int mode,
@@ -429,6 +454,12 @@ int abm_schelling_run(const abm_schelling_grid *g, const uint32_t *keys, int32_t
                      counters, 2 * (int64_t)steps, 0);
   hipLaunchKernelGGL(k_empty_bits, dim3(nb), dim3(kCellsPerBlock), 0, s, g->cells, nc, nwords,
                      ebits);
+  // copy 0 = the grid's empty counts, the other copies start at zero
+  hipLaunchKernelGGL(k_cells_count, dim3(nb), dim3(kCellsPerBlock), 0, s, g->cells, nc, 1,
+                     g->block_counts);
+  const int64_t zero_words = (int64_t)(kCountCopies - 1) * nb;
+  hipLaunchKernelGGL(k_fill_i32_s, dim3(nblk(zero_words, 256)), dim3(256), 0, s,
+                     g->block_counts + nb, zero_words, 0);
   const dim3 tiles((unsigned)((g->width + kTileDim - 1) / kTileDim),
                    (unsigned)((g->height + kTileDim - 1) / kTileDim));
   const size_t prefix_lds = (size_t)(nb + 1) * sizeof(int32_t);
@@ -441,6 +472,7 @@ int abm_schelling_run(const abm_schelling_grid *g, const uint32_t *keys, int32_t
     a.cells = g->cells;
     a.id = g->id;
     a.ebits = ebits;
+    a.ecount = g->block_counts;
     a.nwords = nwords;
     a.nb = (int32_t)nb;
     a.claim = g->claim;
@@ -452,7 +484,7 @@ int abm_schelling_run(const abm_schelling_grid *g, const uint32_t *keys, int32_t
     if (g->n_agents > 0)
       hipLaunchKernelGGL(k_schelling_moves, dim3(nblk(g->n_agents, 256)), dim3(256), 0, s,
                          g->n_agents, g->cells, g->id, g->claim, g->pos, g->type, g->target,
-                         ebits, counters + 2 * (int64_t)t + 1);
+                         ebits, g->block_counts, (int32_t)nb, counters + 2 * (int64_t)t + 1);
   }
   hipLaunchKernelGGL(k_schelling_record, dim3(nblk(steps, 256)), dim3(256), 0, s, counters,
                      steps, g->n_agents > 0 ? g->n_agents : 1, happy_frac, moves);

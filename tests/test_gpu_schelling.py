@@ -23,8 +23,10 @@ def schelling():
 
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("H,W,steps,seed", [(64, 64, 30, 0), (100, 37, 20, 3), (5, 7, 10, 1),
-                                            (128, 96, 12, 2)])
+                                            (128, 96, 12, 2), (1030, 1030, 3, 7)])
 def test_schelling_matches_oracle(schelling, mode, H, W, steps, seed):
+    """1030 x 1030: more than 1,024 blocks of 1,024 cells, the chunked
+    empty-cell prefix."""
     eng = schelling.SchellingEngine(H, W, seed=seed, mode=mode)
     cells, pos, typ, rng = O.schelling_init(H, W, seed, mode)
     assert eng.n_agents == len(pos)
