@@ -167,8 +167,8 @@ int abm_scatter_add_i32(int64_t m, const int32_t *idx, const int32_t *val, int32
  * runs).  Alignment: state 8 bytes, mark and bitmaps 16 bytes.  `t`
  * arguments are absolute step indices. */
 typedef struct abm_sir_graph {
-  int64_t n;                /* agents */
-  int64_t nnz;              /* CSR entries (2 x undirected edges) */
+  int64_t n;                /* agents, below 2^31 - 64 */
+  int64_t nnz;              /* CSR entries (2 x undirected edges), below 2^31 - 64 */
   const int32_t *row_ptr;   /* device [n+1] */
   const int32_t *col;       /* device [nnz] */
   const int32_t *chunk_ptr; /* device [n_chunks+1]: abm_node_chunks with the
