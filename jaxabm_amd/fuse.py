@@ -34,8 +34,10 @@ operand's sign.
   ``normal``, ``bernoulli``, ``randint``, ``gamma``, ``loggamma`` and
   ``beta`` on it give the same values as the vmapped draws (``lognormal``
   and ``exponential`` are built from ``normal`` and ``uniform``).
-* Per-agent vectors of up to 16 values are unrolled.  float32 sums and means
-  over 2 to 7 of them follow torch's summation order; integer sums are exact.
+* Per-agent vectors of up to 16 values are unrolled.  float32 sums, means,
+  products and 2-norms over 1 to 16 of them follow the order torch's
+  reduction uses for a row of that length; integer sums and products are
+  exact.
 
 * ``model_state['agents_<name>'][var][j]`` (another agent's value, j a
   per-agent integer) is a gather; out-of-range indices follow JAX's clamp.
@@ -516,6 +518,21 @@ class SymVec:
             raise Unsupported("a reduction with arguments over a per-agent vector")
         return self.tr.vsum(self, mean=True)
 
+    def prod(self, *a, **k):
+        if a or k:
+            raise Unsupported("a reduction with arguments over a per-agent vector")
+        return self.tr.vprod(self)
+
+    def max(self, *a, **k):
+        if a or k:
+            raise Unsupported("a reduction with arguments over a per-agent vector")
+        return self.tr.vextreme(self, True, along_dim=False)
+
+    def min(self, *a, **k):
+        if a or k:
+            raise Unsupported("a reduction with arguments over a per-agent vector")
+        return self.tr.vextreme(self, False, along_dim=False)
+
     __add__ = Sym.__add__
     __radd__ = Sym.__radd__
     __sub__ = Sym.__sub__
@@ -806,13 +823,34 @@ class _Trace:
         return SymVec(self, [fn(self._vcast(self._elem(a, j), kind),
                                 self._vcast(self._elem(b, j), kind)) for j in range(k)])
 
+    @staticmethod
+    def _sum_slots(k: int) -> int:
+        """Accumulators torch's reduction uses for a row of k contiguous
+        float32 values: the largest power of two <= k, at most 16."""
+        a = 1
+        while a * 2 <= k and a < 16:
+            a *= 2
+        return a
+
+    def _f32_sum(self, names: list[str]) -> str:
+        """The float32 sum of the values `names` in the order torch's
+        reduction adds a row of k = len(names) <= 16 contiguous float32
+        values (measured with torch 2.10 on ROCm for k = 1..20, also under
+        vmap): A = _sum_slots(k) accumulators start from zero, value j is
+        added to accumulator j mod A, and the accumulators are combined in
+        adjacent pairs, ((a0 + a1) + (a2 + a3)) + ...  For k <= 7 that is
+        (0 + x0) + x1, ((0 + x0) + x2) + (0 + x1), and four accumulators."""
+        A = self._sum_slots(len(names))
+        acc = ["0.0f"] * A
+        for j, v in enumerate(names):
+            acc[j % A] = self.tmp("float", f"({acc[j % A]} + {v})")
+        while len(acc) > 1:
+            acc = [self.tmp("float", f"({acc[i]} + {acc[i + 1]})") for i in range(0, len(acc), 2)]
+        return acc[0]
+
     def vsum(self, x: SymVec, mean: bool = False, out_kind: Optional[str] = None) -> Sym:
-        """torch.sum / torch.mean over a per-agent vector, in the order
-        torch's reduction adds k <= 7 contiguous float32 values (measured
-        with torch 2.10 on ROCm for every row count, also under vmap): each accumulator
-        starts from zero; k = 2: (0 + x0) + x1; k = 3: ((0 + x0) + x2) +
-        (0 + x1); k = 4..7: four accumulators by j mod 4, then
-        (a0 + a1) + (a2 + a3).  A mean multiplies the sum by float32(1/k).
+        """torch.sum / torch.mean over a per-agent vector: float32 values in
+        torch's order (_f32_sum); a mean multiplies the sum by float32(1/k).
         Integer and boolean sums are int64, or int32 (wrapping) when asked
         for: exact, so any order."""
         k, kind = len(x), x.kind
@@ -827,26 +865,122 @@ class _Trace:
                 acc = Sym(self, self.tmp("int32_t", f"(int32_t)(uint32_t)(uint64_t){acc.name}"),
                           "i32", x.varying)
             return acc
-        if kind != "f32" or not 2 <= k <= 7:
-            raise Unsupported("a reduction over a vector other than 2..7 float32 values")
-        def add(a: str, b: str) -> str:
-            return self.tmp("float", f"({a} + {b})")
-
-        zero = "0.0f"
-        if k == 2:
-            t = add(add(zero, x.items[0].name), x.items[1].name)
-        elif k == 3:
-            a0 = add(add(zero, x.items[0].name), x.items[2].name)
-            a1 = add(zero, x.items[1].name)
-            t = add(a0, a1)
-        else:
-            acc = [zero] * 4
-            for j, v in enumerate(x.items):
-                acc[j % 4] = add(acc[j % 4], v.name)
-            t = add(add(acc[0], acc[1]), add(acc[2], acc[3]))
+        if kind != "f32" or not 1 <= k <= _MAX_VEC:
+            raise Unsupported("a reduction over a vector other than 1..16 float32 values")
+        t = self._f32_sum([v.name for v in x.items])
         if mean:
             t = self.tmp("float", f"({t} * {_lit(float(np.float32(1.0 / k)), 'f32')})")
         return Sym(self, t, "f32", x.varying)
+
+    def vprod(self, x: SymVec, out_kind: Optional[str] = None) -> Sym:
+        """torch.prod over a per-agent vector.  float32: the order torch's
+        reduction multiplies a row of k <= 16 values in (measured with torch
+        2.10 on ROCm for k = 1..20, also under vmap): 16 accumulators
+        starting from one, value j in accumulator j, combined by halves,
+        a[i] * a[i + 8], then a[i] * a[i + 4], a[i] * a[i + 2], a[0] * a[1]
+        (products with an accumulator still at one are exact and left
+        out).  Integers and booleans: int64 (or int32 when asked for),
+        wrapping, exact in any order."""
+        k, kind = len(x), x.kind
+        if kind in ("bool", "u8", "i32", "i64"):
+            acc = f"(uint64_t)(int64_t)({x.items[0].name})"
+            for v in x.items[1:]:
+                acc = self.tmp("uint64_t", f"({acc} * (uint64_t)(int64_t)({v.name}))")
+            if out_kind == "i32":
+                return Sym(self, self.tmp("int32_t", f"(int32_t)(uint32_t){acc}"), "i32",
+                           x.varying)
+            return Sym(self, self.tmp("int64_t", f"(int64_t){acc}"), "i64", x.varying)
+        if kind != "f32" or not 1 <= k <= _MAX_VEC:
+            raise Unsupported("a product over a vector other than 1..16 float32 values")
+        slots: list = [v.name for v in x.items] + [None] * (16 - k)
+        while len(slots) > 1:
+            h = len(slots) // 2
+            nxt = []
+            for i in range(h):
+                lo, hi = slots[i], slots[i + h]
+                if lo is None or hi is None:
+                    nxt.append(hi if lo is None else lo)
+                else:
+                    nxt.append(self.tmp("float", f"({lo} * {hi})"))
+            slots = nxt
+        return Sym(self, slots[0], "f32", x.varying)
+
+    def vnorm(self, x: SymVec) -> Sym:
+        """torch.linalg.norm / vector_norm (2-norm) of a per-agent float32
+        vector: sqrt of the squares summed in torch's order (each square
+        rounded, then added: no fused multiply-add), as measured for
+        k = 1..20."""
+        if x.kind != "f32" or not 1 <= len(x) <= _MAX_VEC:
+            raise Unsupported("a norm of a vector other than 1..16 float32 values")
+        sq = [self.tmp("float", f"({v.name} * {v.name})") for v in x.items]
+        return Sym(self, self.tmp("float", f"sqrtf({self._f32_sum(sq)})"), "f32", x.varying)
+
+    def vextreme(self, x: SymVec, largest: bool, along_dim: bool) -> Sym:
+        """torch.max / torch.min (whole vector) and torch.amax / torch.amin
+        (over its dim) of a per-agent vector, as torch's reductions give
+        them (measured with torch 2.10 on ROCm for k = 1..16 with signed
+        zeros, infinities and NaN, also under vmap).  A NaN propagates.
+        max/min scan the values in order and keep the running value on a
+        tie (the first of +0 and -0 wins); amax/amin keep _sum_slots(k)
+        running values (value j in slot j mod A), combined in adjacent
+        pairs, and take the later value on a tie.  Integers and booleans:
+        exact."""
+        kind = x.kind
+        if kind not in ("bool", "u8", "i32", "i64", "f32", "f64"):
+            raise Unsupported("a max/min of this dtype")
+        ct = _CT[kind]
+        lt = "<" if largest else ">"
+        gt = ">" if largest else "<"
+        floats = kind in ("f32", "f64")
+
+        def keep_first(a: str, b: str) -> str:   # running a, next value b
+            if floats:
+                return self.tmp(ct, f"(({a} != {a}) ? {a} : (({b} != {b}) ? {b} : "
+                                    f"(({a} {lt} {b}) ? {b} : {a})))")
+            return self.tmp(ct, f"(({a} {lt} {b}) ? {b} : {a})")
+
+        def take_later(a: str, b: str) -> str:
+            if floats:
+                return self.tmp(ct, f"((({a} != {a}) || ({a} {gt} {b})) ? {a} : {b})")
+            return self.tmp(ct, f"(({a} {gt} {b}) ? {a} : {b})")
+
+        names = [v.name for v in x.items]
+        if not along_dim:
+            acc = names[0]
+            for v in names[1:]:
+                acc = keep_first(acc, v)
+            return Sym(self, acc, kind, x.varying)
+        A = self._sum_slots(len(names))
+        slots: list = [None] * A
+        for j, v in enumerate(names):
+            slots[j % A] = v if slots[j % A] is None else take_later(slots[j % A], v)
+        while len(slots) > 1:
+            nxt = []
+            for i in range(0, len(slots), 2):
+                a, b = slots[i], slots[i + 1]
+                nxt.append(b if a is None else (a if b is None else take_later(a, b)))
+            slots = nxt
+        return Sym(self, slots[0], kind, x.varying)
+
+    def varg(self, x: SymVec, largest: bool) -> Sym:
+        """torch.argmax / argmin of a per-agent vector (int64): the first
+        index of the largest (smallest) value; a NaN counts as larger than
+        everything, so the first NaN's index when there is one."""
+        kind = x.kind
+        if kind not in ("bool", "u8", "i32", "i64", "f32", "f64"):
+            raise Unsupported("an argmax/argmin of this dtype")
+        ct = _CT[kind]
+        cmp = ">" if largest else "<"
+        best, idx = x.items[0].name, "(int64_t)0"
+        for j, v in enumerate(x.items[1:], start=1):
+            if kind in ("f32", "f64"):
+                take = self.tmp("bool", f"(({best} == {best}) && (({v.name} != {v.name}) || "
+                                        f"({v.name} {cmp} {best})))")
+            else:
+                take = self.tmp("bool", f"({v.name} {cmp} {best})")
+            best = self.tmp(ct, f"({take} ? {v.name} : {best})")
+            idx = self.tmp("int64_t", f"({take} ? (int64_t){j} : {idx})")
+        return Sym(self, idx, "i64", x.varying)
 
     def stack(self, values) -> SymVec:
         """jnp.array / jnp.stack of per-agent or uniform scalars (Python
@@ -969,9 +1103,9 @@ class _Trace:
 
     def power(self, x, e):
         """x ** e for a Python number e, as torch's pow of a tensor by a
-        scalar: 2 and 3 multiply, 0.5 is sqrt, -1 and -2 reciprocals, any
-        other exponent of a float pow(x, e) with e in x's dtype (-0.5 is not
-        traced); integer bases take only 2 and 3."""
+        scalar: 2 and 3 multiply, 0.5 is sqrt, -0.5 torch's rsqrt (float32
+        only), -1 and -2 reciprocals, any other exponent of a float pow(x, e)
+        with e in x's dtype; integer bases take only 2 and 3."""
         if isinstance(e, bool) or not isinstance(e, (int, float)):
             raise Unsupported("a power with an exponent that is not a Python number")
         if isinstance(x, SymVec):
@@ -991,9 +1125,12 @@ class _Trace:
             if e == 0.5:
                 expr = f"{'sqrtf' if f32 else 'sqrt'}({cx})"
             elif e == -0.5:
-                # torch's own form of this case is not rsqrtf (one ulp apart
-                # on 5% of float32 values): left to the vmapped path
-                raise Unsupported("x ** -0.5")
+                # torch computes this case as its rsqrt: for float32 that is
+                # (float)(1 / sqrt((double)x)) on every bit pattern (rsqrtf
+                # and 1 / sqrtf differ from it); float64 stays vmapped
+                if not f32:
+                    raise Unsupported("x ** -0.5 of a float64 value")
+                expr = f"((float)(1.0 / sqrt((double){cx})))"
             elif e == -1:
                 expr = f"({one} / {cx})"
             elif e == -2:
@@ -1036,6 +1173,16 @@ class _Trace:
             if k in _FLOATS:
                 raise Unsupported("~ on floats")
             return Sym(self, self.tmp(ct, expr), k, x.varying)
+        if op == "rsqrt":
+            # torch.rsqrt of float32 equals (float)(1 / sqrt((double)x)) on
+            # every float32 bit pattern (measured: torch 2.10, ROCm, MI355X);
+            # float64 is left to the vmapped path
+            if k not in _FLOATS:
+                x = self.cast(x, "f32")
+            if x.kind != "f32":
+                raise Unsupported("rsqrt of a float64 value")
+            return Sym(self, self.tmp("float", f"((float)(1.0 / sqrt((double){x.name})))"),
+                       "f32", x.varying)
         if op in self._UNARY_F:
             if k not in _FLOATS:
                 x = self.cast(x, "f32")
@@ -1176,7 +1323,8 @@ class _Trace:
         name = getattr(func, "__name__", "")
         unary = {"abs": "abs", "absolute": "abs", "neg": "neg", "negative": "neg",
                  "exp": "exp", "log": "log", "log1p": "log1p", "expm1": "expm1",
-                 "sqrt": "sqrt", "sin": "sin", "cos": "cos", "tan": "tan", "tanh": "tanh",
+                 "sqrt": "sqrt", "rsqrt": "rsqrt", "sin": "sin", "cos": "cos", "tan": "tan",
+                 "tanh": "tanh",
                  "atan": "atan", "arctan": "atan", "floor": "floor", "ceil": "ceil",
                  "trunc": "trunc", "square": "square", "isnan": "isnan",
                  "isfinite": "isfinite", "sign": "sign", "logical_not": "logical_not"}
@@ -1236,6 +1384,30 @@ class _Trace:
                     (dt is None or (dt == torch.int32 and int_in and name == "sum")):
                 return self.vsum(args[0], mean=name == "mean",
                                  out_kind="i32" if dt == torch.int32 else None)
+        if name in ("max", "min") and len(args) == 1 and not kwargs and \
+                isinstance(args[0], SymVec):
+            return self.vextreme(args[0], name == "max", along_dim=False)
+        if name in ("amax", "amin") and len(args) == 1 and isinstance(args[0], SymVec) and \
+                set(kwargs) <= {"dim", "keepdim"} and kwargs.get("dim") in (0, -1, (0,), (-1,), [0], [-1]) \
+                and not kwargs.get("keepdim"):
+            return self.vextreme(args[0], name == "amax", along_dim=True)
+        if name in ("argmax", "argmin") and len(args) == 1 and isinstance(args[0], SymVec) and \
+                set(kwargs) <= {"dim", "keepdim"} and kwargs.get("dim") in (None, 0, -1) and \
+                not kwargs.get("keepdim"):
+            return self.varg(args[0], name == "argmax")
+        if name == "prod" and len(args) == 1 and isinstance(args[0], SymVec):
+            dim = kwargs.get("dim")
+            dt = kwargs.get("dtype")
+            int_in = args[0].kind in ("bool", "u8", "i32")
+            if dim in (None, 0, -1) and not kwargs.get("keepdim") and \
+                    (dt is None or (dt == torch.int32 and int_in)):
+                return self.vprod(args[0], out_kind="i32" if dt == torch.int32 else None)
+        if name in ("linalg_norm", "linalg_vector_norm") and len(args) == 1 and \
+                isinstance(args[0], SymVec) and \
+                set(kwargs) <= {"ord", "dim", "keepdim", "dtype"} and \
+                kwargs.get("ord") in (None, 2) and kwargs.get("dim") in (None, 0, -1) and \
+                not kwargs.get("keepdim") and kwargs.get("dtype") is None:
+            return self.vnorm(args[0])
         if name == "stack" and len(args) >= 1 and isinstance(args[0], (list, tuple)) and \
                 kwargs.get("dim", args[1] if len(args) > 1 else 0) in (0, -1):
             return self.stack(args[0])

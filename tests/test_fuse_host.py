@@ -96,8 +96,9 @@ def test_refused(fn, why):
 def test_vector_limits_refused():
     with pytest.raises(fuse.Unsupported):      # longer than 16
         _trace(lambda s, m, k: {"p": s["p"] + 1.0}, {"p": torch.rand(N, 17)}, {"env": {}})
-    with pytest.raises(fuse.Unsupported):      # a reduction over 8 or more values
-        _trace(lambda s, m, k: {"r": jnp.sum(s["p"])}, {"p": torch.rand(N, 8)}, {"env": {}})
+    with pytest.raises(fuse.Unsupported):      # a norm other than the 2-norm
+        _trace(lambda s, m, k: {"r": jnp.linalg.norm(s["p"], ord=1)}, {"p": torch.rand(N, 8)},
+               {"env": {}})
     name, src, tr, outs = _trace(lambda s, m, k: {"r": jnp.sum(s["p"] * s["p"]),
                                                   "m": jnp.mean(s["p"]),
                                                   "c": jnp.sum(s["p"] > 0.5)},
@@ -107,6 +108,30 @@ def test_vector_limits_refused():
     with pytest.raises(fuse.Unsupported):      # vectors of different lengths
         _trace(lambda s, m, k: {"r": s["p"] + s["q"]},
                {"p": torch.rand(N, 2), "q": torch.rand(N, 3)}, {"env": {}})
+
+
+@pytest.mark.parametrize("k", [1, 8, 11, 16])
+def test_vector_sums_products_norms_trace_and_compile(k):
+    """Reductions over per-agent vectors of 1..16 values: float32 sums,
+    means, products, 2-norms, max / min / argmax / argmin, int32 products
+    (JAX x64 off) and a -0.5 power, all in one generated kernel."""
+    st = {"p": torch.rand(N, k) - 0.5, "q": torch.randint(-3, 4, (N, k), dtype=torch.int32)}
+
+    def upd(s, m, key):
+        v = s["p"]
+        return {"s": jnp.sum(v), "m": jnp.mean(v * 2.0), "pr": jnp.prod(v + 1.0),
+                "pv": v.prod(), "nm": jnp.linalg.norm(v), "ip": jnp.prod(s["q"]),
+                "is": jnp.sum(s["q"]), "mx": jnp.max(v), "amn": jnp.min(v, axis=0),
+                "am": jnp.argmax(v), "iam": jnp.argmin(s["q"]), "imx": s["q"].max(),
+                "rs": jnp.abs(v) ** -0.5, "rq": torch.rsqrt(jnp.abs(v))}
+
+    name, src, tr, outs = _trace(upd, st, {"env": {}})
+    assert {kk: vv.dtype for kk, vv in outs.items()} == {
+        "s": torch.float32, "m": torch.float32, "pr": torch.float32, "pv": torch.float32,
+        "nm": torch.float32, "ip": torch.int32, "is": torch.int32, "mx": torch.float32,
+        "amn": torch.float32, "am": torch.int32, "iam": torch.int32, "imx": torch.int32,
+        "rs": torch.float32, "rq": torch.float32}
+    fuse.compile_source(src)
 
 
 def test_walker_with_vectors_traces_and_compiles():

@@ -177,8 +177,8 @@ def test_fused_equals_vmapped(J, agent, n, mode):
 
 
 def test_refused_function_runs_vmapped(J):
-    """A product over a per-agent vector is not covered: the update runs the
-    vmapped path (and the refusal is remembered); the init is fused."""
+    """A variance over a per-agent vector is not covered: the update runs
+    the vmapped path (and the refusal is remembered); the init is fused."""
     from jaxabm_amd import fuse
     from jaxabm_amd import random as jr
 
@@ -189,7 +189,7 @@ def test_refused_function_runs_vmapped(J):
         def update(self, s, ms, cfg, key):
             from jaxabm_amd import numpy as jnp
 
-            return {"p": s["p"] * jnp.prod(s["p"])}
+            return {"p": s["p"] * jnp.var(s["p"])}
 
     before = fuse.stats()["launches"]
     c = J.AgentCollection(Vec(), 50)
@@ -198,10 +198,11 @@ def test_refused_function_runs_vmapped(J):
     c.update({"env": {}}, jr.PRNGKey(2), J.ModelConfig())
     assert c.states["p"].shape == (50, 2)
     assert fuse.stats()["launches"] == before + 1
-    assert any("prod" in r for r in fuse.refused().values())
+    assert any("var" in r for r in fuse.refused().values())
     import torch
 
-    assert torch.equal(c.states["p"], p0 * p0.prod(1, keepdim=True))
+    v = torch.func.vmap(lambda r: torch.var(r, correction=0))(p0)
+    assert torch.equal(c.states["p"], p0 * v[:, None])
 
 
 OPS = {
@@ -232,13 +233,18 @@ OPS = {
     "flag": lambda s, m: (s["b"] & (s["x"] > m["env"]["thr"])) | ((s["w"] < 0) ^ m["env"]["on"]),
     "delta": lambda s, m: s["x"] + m["env"]["delta"], "invert": lambda s, m: ~s["b"],
     "half": lambda s, m: s["x"] * 0.5, "sub_t": lambda s, m: s["x"] - s["x"] * 0.5,
+    "pow_mhalf": lambda s, m: s["x"] ** -0.5, "pow_w_mhalf": lambda s, m: s["w"] ** -0.5,
+    "rsqrt": "rsqrt",
 }
 
 
 def _jnp_op(name):
+    import torch
+
     from jaxabm_amd import numpy as jnp
 
     return {
+        "rsqrt": lambda s, m: torch.rsqrt(s["x"]),
         "log": lambda s, m: jnp.log(s["x"]),
         "min_nan": lambda s, m: jnp.minimum(jnp.log(s["x"]), s["x"] * 0.0),
         "max_sz": lambda s, m: jnp.maximum(s["x"] * 0.0, -(s["x"] * 0.0)),
@@ -363,11 +369,15 @@ def test_facade_walker_model_fused_equals_vmapped(J):
     assert int(sb["steps_taken"][0]) == 120 and bool((sb["color"] == 1).any())
 
 
-@pytest.mark.parametrize("k", [2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("k", list(range(1, 17)))
 def test_vector_reductions_fused_equal_vmapped(J, k):
-    """jnp.sum / jnp.mean over a per-agent vector of k float32 values (the
-    order torch adds them in) and integer counts, on values over +-2^10
-    with signed zeros."""
+    """jnp.sum / jnp.mean / jnp.prod / jnp.linalg.norm over a per-agent
+    vector of k float32 values (the orders torch reduces them in), integer
+    counts and products, on values over +-2^10 with signed zeros (products
+    of 16 such values overflow to inf; the scaled product stays finite);
+    max / min (whole vector and over its axis) and argmax / argmin of
+    vectors full of ties, signed zeros, infinities and NaN, and of int32
+    vectors; a power of -0.5."""
     import torch
 
     from jaxabm_amd import fuse
@@ -380,6 +390,15 @@ def test_vector_reductions_fused_equal_vmapped(J, k):
         torch.randint(-10, 11, (n, k), device="cuda", generator=g).float())
     p[:4, 0] = torch.tensor([-0.0, 0.0, -0.0, 1.0])
     p[:4, 1:] = -0.0
+    # extremes: many ties, signed zeros, infinities and NaN
+    sel = torch.rand(n, k, device="cuda", generator=g)
+    q = torch.randint(-3, 4, (n, k), device="cuda", generator=g).float()
+    q[sel < 0.2] = 0.0
+    q[(sel >= 0.2) & (sel < 0.4)] = -0.0
+    q[(sel >= 0.4) & (sel < 0.43)] = float("inf")
+    q[(sel >= 0.43) & (sel < 0.46)] = -float("inf")
+    q[(sel >= 0.46) & (sel < 0.48)] = float("nan")
+    iq = torch.randint(-5, 6, (n, k), device="cuda", generator=g, dtype=torch.int32)
 
     class Red:
         def init_state(self, cfg, key):
@@ -388,7 +407,15 @@ def test_vector_reductions_fused_equal_vmapped(J, k):
         def update(self, s, ms, cfg, key):
             v = s["p"]
             return {"s": jnp.sum(v), "s2": jnp.sum(v * v), "m": jnp.mean(v),
-                    "c": jnp.sum(v > 0.0), "d": v.sum() - jnp.mean(v * 2.0)}
+                    "c": jnp.sum(v > 0.0), "d": v.sum() - jnp.mean(v * 2.0),
+                    "pr": jnp.prod(v * 0.01 + 1.0), "pr0": jnp.prod(v), "pv": v.prod(),
+                    "nm": jnp.linalg.norm(v),
+                    "ip": jnp.prod((v > 0.0).astype(jnp.int32) + 1),
+                    "mx": jnp.max(s["q"]), "mn": jnp.min(s["q"]), "mxm": s["q"].max(),
+                    "amx": jnp.max(s["q"], axis=0), "amn": jnp.min(s["q"], axis=0),
+                    "am": jnp.argmax(s["q"]), "an": jnp.argmin(s["q"]),
+                    "imx": jnp.max(s["iq"]), "iam": jnp.argmax(s["iq"]),
+                    "rs": jnp.abs(v) ** -0.5}
 
     res = []
     for fused in (False, True):
@@ -397,7 +424,7 @@ def test_vector_reductions_fused_equal_vmapped(J, k):
             before = fuse.stats()["launches"]
             c = J.AgentCollection(Red(), n)
             c.model_config = J.ModelConfig()
-            c._states = {"p": p.clone()}
+            c._states = {"p": p.clone(), "q": q.clone(), "iq": iq.clone()}
             c.update({"env": {}}, jr.PRNGKey(1), J.ModelConfig())
             assert fuse.stats()["launches"] - before == (1 if fused else 0), fuse.refused()
             res.append({kk: vv.clone() for kk, vv in c.states.items()})
