@@ -611,18 +611,27 @@ def bench_sir(args) -> dict:
     import torch
 
     from jaxabm_amd._lib import check, lib, ptr
-    from jaxabm_amd.network import ba_csr, csr_checksum
+    from jaxabm_amd.network import ba_csr_device, csr_checksum
     from jaxabm_amd.programs.sir import SIRNetworkEngine
 
     gold = sir_fixture()
-    row_ptr, col = ba_csr(N_AGENTS, BA_M, GRAPH_SEED)
+    # the graph is built on the device (BA edges, then the CSR), outside the
+    # timed region; its host copy is checksummed.  A small build first loads
+    # the kernels, so graph_build_ms is the build itself.
+    ba_csr_device(1000, BA_M, GRAPH_SEED)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter()
+    net = ba_csr_device(N_AGENTS, BA_M, GRAPH_SEED)
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t_build) * 1e3
+    row_ptr, col = net.host_row_ptr, net.host_col
     digest = csr_checksum(row_ptr, col)
     if digest != gold["csr_sha256"]:
         raise ParityError(f"BA({N_AGENTS}, {BA_M}) CSR checksum {digest} differs from the "
                           f"committed {gold['csr_sha256']}")
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
-        eng = SIRNetworkEngine(N_AGENTS, csr=(row_ptr, col), beta=BETA, gamma=GAMMA,
+        eng = SIRNetworkEngine(N_AGENTS, csr=net, beta=BETA, gamma=GAMMA,
                                n_seed_infected=N_SEED, seed=SEED)
         init_state = eng.state.clone()
         init_bitmaps = eng.bitmaps.clone()
@@ -709,6 +718,8 @@ def bench_sir(args) -> dict:
             "n_agents": N_AGENTS, "ba_m": BA_M, "undirected_edges": nnz // 2, "csr_nnz": nnz,
             "sim_steps": SIM_STEPS, "beta": BETA, "gamma": GAMMA, "seed_infected": N_SEED,
             "rng": "jax threefry2x32, partitionable stream", "hipgraph": graph is not None,
+            "graph_build": "on the device: abm_ba_edges_device + abm_csr_from_edges, "
+                           "outside the timed region", "graph_build_ms": build_ms,
             "bench_step": "one full 365-step simulation",
         },
         "graph_checksum_ok": True,

@@ -114,6 +114,15 @@ int64_t abm_ba_num_edges(int64_t n, int32_t m);
 /* Barabasi-Albert edge list; returns the number of edges or -1. */
 int64_t abm_ba_edges(int64_t n, int32_t m, const uint32_t key[2], int32_t *src,
                      int32_t *dst);
+/* The same Barabasi-Albert edge list built on the device (replaces the
+ * add_edge loop of jaxabm/agentpy.py:559-582 that builds a Network):
+ * src/dst device int32[abm_ba_num_edges(n, m)], key host uint32[2],
+ * unresolved device uint32[1] scratch.  Rounds of one thread per node resolve
+ * the preferential-attachment draws against the closed-form repeated-node
+ * list; the host checks for completion every 8 rounds (synchronises the
+ * stream). */
+int abm_ba_edges_device(int64_t n, int32_t m, const uint32_t key[2], int32_t *src,
+                        int32_t *dst, uint32_t *unresolved, void *stream);
 /* Undirected CSR (both directions, rows ascending). */
 int abm_csr_undirected(int64_t n, int64_t n_edges, const int32_t *src, const int32_t *dst,
                        int32_t *row_ptr, int32_t *col);

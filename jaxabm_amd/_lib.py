@@ -145,6 +145,7 @@ SIGNATURES = {
     "abm_step_keys": (_int, [_u32arr, _i32, _i32, _i32, _i32, _int, _u32arr]),
     "abm_ba_num_edges": (_i64, [_i64, _i32]),
     "abm_ba_edges": (_i64, [_i64, _i32, _u32arr, _i32arr, _i32arr]),
+    "abm_ba_edges_device": (_int, [_i64, _i32, _u32arr, _p, _p, _p, _p]),
     "abm_csr_undirected": (_int, [_i64, _i64, _i32arr, _i32arr, _i32arr, _i32arr]),
     "abm_node_chunks": (_i64, [_i64, _i32arr, _i32, _i64, _i32, _i64, _i64, _i32, _p]),
     "abm_csr_from_edges_workspace": (_sz, [_i64, _i32]),

@@ -27,6 +27,29 @@ def ba_edges(n: int, m: int, graph_seed: int) -> tuple[np.ndarray, np.ndarray]:
     return src[:E], dst[:E]
 
 
+def ba_edges_device(n: int, m: int, graph_seed: int, device="cuda"
+                    ) -> tuple[torch.Tensor, torch.Tensor]:
+    """The Barabasi-Albert(n, m) edges of ``ba_edges`` built on the device
+    (abm_ba_edges_device): the same (src, dst) int32 tensors, entry for
+    entry."""
+    E = int(lib().abm_ba_num_edges(int(n), int(m)))
+    src = torch.empty(max(E, 1), dtype=torch.int32, device=device)
+    dst = torch.empty(max(E, 1), dtype=torch.int32, device=device)
+    open_ = torch.empty(1, dtype=torch.int32, device=device)
+    key = np.array([0, int(graph_seed) & 0xFFFFFFFF], np.uint32)
+    check(lib().abm_ba_edges_device(int(n), int(m), key, ptr(src), ptr(dst), ptr(open_),
+                                    stream_handle()), "ba_edges_device")
+    return src[:E], dst[:E]
+
+
+def ba_csr_device(n: int, m: int, graph_seed: int, device="cuda") -> "DeviceCSR":
+    """Barabasi-Albert(n, m) as an undirected CSR built entirely on the device
+    (edges, then abm_csr_from_edges with both directions): the arrays of
+    ``ba_csr``."""
+    src, dst = ba_edges_device(n, m, graph_seed, device)
+    return csr_from_edges(n, src, dst, symmetrize=True, device=device)
+
+
 def csr_undirected(n: int, src: np.ndarray, dst: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     src = np.ascontiguousarray(src, np.int32)
     dst = np.ascontiguousarray(dst, np.int32)

@@ -104,13 +104,14 @@ class NetworkSIRModel(_ProgramModel):
         return self._results(steps, {"S": rec["S"], "I": rec["I"], "R": rec["R"]}, t0)
 
     def run(self, steps: Optional[int] = None) -> Results:
-        from ..network import ba_csr
         from .sir import SIRNetworkEngine
 
         steps = int(steps if steps is not None else self.p.get("steps", 365))
         t0 = time.time()
-        csr = ba_csr(int(self.p["n"]), int(self.p["m"]), int(self.p["graph_seed"]))
-        self.engine = SIRNetworkEngine(int(self.p["n"]), csr=csr, beta=float(self.p["beta"]),
+        # the BA graph is built on the device by the engine
+        self.engine = SIRNetworkEngine(int(self.p["n"]), m=int(self.p["m"]),
+                                       graph_seed=int(self.p["graph_seed"]),
+                                       beta=float(self.p["beta"]),
                                        gamma=float(self.p["gamma"]),
                                        n_seed_infected=int(self.p["n_infected"]),
                                        seed=int(self.seed))

@@ -22,7 +22,7 @@ import torch
 
 from .. import random as jr
 from .._lib import SirGraph, check, lib, ptr, require_gpu, stream_handle
-from ..network import DeviceCSR, ba_csr, node_chunks
+from ..network import DeviceCSR, ba_csr_device, node_chunks
 
 S, I, R = 0, 1, 2
 
@@ -32,11 +32,14 @@ class SIRNetworkEngine:
 
     def __init__(self, n: int, m: int = 4, graph_seed: int = 0, beta: float = 0.05,
                  gamma: float = 0.1, n_seed_infected: int = 100, seed: int = 0,
-                 mode: int | None = None, csr: tuple[np.ndarray, np.ndarray] | None = None,
+                 mode: int | None = None,
+                 csr: "tuple[np.ndarray, np.ndarray] | DeviceCSR | None" = None,
                  device: str = "cuda", chunk_cost: int | None = None):
-        """``chunk_cost`` overrides the cost target of the agent chunks (one
-        per block by default); a large value gives chunks of the maximum tile
-        size, whatever the graph size."""
+        """``csr``: the graph as host (row_ptr, col) arrays or a DeviceCSR;
+        by default BA(n, m) from ``graph_seed``, built on the device
+        (network.ba_csr_device).  ``chunk_cost`` overrides the cost target of
+        the agent chunks (one per block by default); a large value gives
+        chunks of the maximum tile size, whatever the graph size."""
         require_gpu()
         self.n = int(n)
         self.beta = float(np.float32(beta))
@@ -44,9 +47,15 @@ class SIRNetworkEngine:
         self.mode = jr._mode(mode)
         self.device = torch.device(device)
         if csr is None:
-            csr = ba_csr(self.n, m, graph_seed)
-        row_ptr, col = csr
-        self.graph = DeviceCSR.from_host(row_ptr, col, self.device)
+            csr = ba_csr_device(self.n, m, graph_seed, self.device)
+        if isinstance(csr, DeviceCSR):
+            if csr.n != self.n:
+                raise ValueError(f"the graph has {csr.n} nodes, the engine {self.n}")
+            self.graph = csr
+            row_ptr = csr.host_row_ptr   # the chunk table is cut on the host
+        else:
+            row_ptr, col = csr
+            self.graph = DeviceCSR.from_host(row_ptr, col, self.device)
         align, node_cost, heavy_deg, heavy_cost = C.c_int32(), C.c_int64(), C.c_int32(), C.c_int64()
         per_block, blocks, max_nodes = C.c_int32(), C.c_int32(), C.c_int32()
         lib().abm_sir_chunk_params(C.byref(align), C.byref(node_cost), C.byref(heavy_deg),
