@@ -283,11 +283,17 @@ def _deffuant_cpu(n, eps, mu, target, runs):
     b, steps, mode = 32, 10, O.MODE_PARTITIONABLE
     x0, rng = O.deffuant_init(n, b, 0, mode)
     e, m = eps[:b].astype(np.float64), mu[:b].astype(np.float64)
-    return cpu_sample(lambda: O.deffuant_grad(x0, e, m, rng, steps, target, mode), n * b * steps,
-                      "agent-instance-steps/s",
-                      f"forward + float64 reverse sweep of {n:,} agents x {b} of the 256 "
-                      f"(eps, mu) instances x {steps} steps (the per-step pairing is shared by "
-                      f"the instances, as in the measured batch)", min(runs, 3))
+    out = cpu_sample(lambda: O.deffuant_grad(x0, e, m, rng, steps, target, mode), n * b * steps,
+                     "agent-instance-steps/s",
+                     f"forward + float64 reverse sweep of {n:,} agents x {b} of the 256 "
+                     f"(eps, mu) instances x {steps} steps (the per-step pairing is shared by "
+                     f"the instances, as in the measured batch)", min(runs, 3))
+    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    many = cpu_sample(lambda: O.deffuant_grad_threads(x0, e, m, rng, steps, target, ncores, mode),
+                      n * b * steps, "agent-instance-steps/s", "", min(runs, 3))
+    out["all_cores"] = {"value": many["value"], "cores": ncores, "run_s": many["run_s"],
+                        "kind": "port (OpenMP over the instances; same bits)"}
+    return out
 
 
 def _wealth_cpu(n, steps, runs):
@@ -318,9 +324,20 @@ def _schelling_cpu(h, w, runs):
         cells, pos, typ, rng = (a.copy() for a in init)
         O.schelling_run(h, w, cells, pos, typ, rng, steps, mode)
 
-    return cpu_sample(run, h * w * steps, "cell-steps/s",
-                      f"the first {steps} of the 200 steps (the busiest) of the {h}x{w} "
-                      f"simulation from the same seeded grid", min(runs, 3))
+    out = cpu_sample(run, h * w * steps, "cell-steps/s",
+                     f"the first {steps} of the 200 steps (the busiest) of the {h}x{w} "
+                     f"simulation from the same seeded grid", min(runs, 3))
+    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+
+    def run_all():
+        cells, pos, typ, rng = (a.copy() for a in init)
+        O.schelling_run_threads(h, w, cells, pos, typ, rng, steps, ncores, mode)
+
+    many = cpu_sample(run_all, h * w * steps, "cell-steps/s", "", min(runs, 3))
+    out["all_cores"] = {"value": many["value"], "cores": ncores, "run_s": many["run_s"],
+                        "kind": "port (OpenMP over the agents of each step; claims and moves "
+                                "sequential; same results)"}
+    return out
 
 
 def _trader_cpu(n, steps, runs):

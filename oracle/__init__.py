@@ -99,6 +99,8 @@ def _declare(L):
                                       _i8p, _u32p]),
         "orc_schelling_run": (None, [_i32, _i32, _i8p, _i32p, _i8p, _i64,
                                      _u32p, _i32, C.c_int, _f32p, _i32p]),
+        "orc_schelling_run_threads": (None, [_i32, _i32, _i8p, _i32p, _i8p, _i64,
+                                             _u32p, _i32, C.c_int, _f32p, _i32p, C.c_int]),
         "orc_deffuant_init": (None, [_i64, _i32, _i64, C.c_int, _f32p, _u32p]),
         "orc_deffuant_forward": (None, [_i64, _i32, _f32p, _f32p, _f32p, _u32p,
                                         _i32, C.c_int]),
@@ -106,6 +108,8 @@ def _declare(L):
         "orc_deffuant_loss": (C.c_double, [_f64p, _f64p]),
         "orc_deffuant_grad": (None, [_i64, _i32, _f64p, _f64p, _f32p, _u32p,
                                      _i32, C.c_int, _f64p, _f64p, _f64p, _f64p]),
+        "orc_deffuant_grad_threads": (None, [_i64, _i32, _f64p, _f64p, _f32p, _u32p,
+                                     _i32, C.c_int, _f64p, _f64p, _f64p, _f64p, _i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -339,6 +343,17 @@ def schelling_run(height: int, width: int, cells, pos, typ, rng, steps: int,
     return hf[:steps], mv[:steps]
 
 
+def schelling_run_threads(height: int, width: int, cells, pos, typ, rng, steps: int,
+                          threads: int, mode: int = MODE_PARTITIONABLE):
+    """schelling_run with each step's agent phase on ``threads`` OpenMP
+    threads (same results)."""
+    hf = np.zeros(max(steps, 1), np.float32)
+    mv = np.zeros(max(steps, 1), np.int32)
+    lib().orc_schelling_run_threads(height, width, cells, pos, typ, len(pos), rng, steps,
+                                    mode, hf, mv, int(threads))
+    return hf[:steps], mv[:steps]
+
+
 def deffuant_init(n: int, batch: int, seed: int, mode: int = MODE_PARTITIONABLE):
     x = np.zeros(n * batch, np.float32)
     rng = np.zeros(2, np.uint32)
@@ -365,6 +380,23 @@ def deffuant_stats(x) -> np.ndarray:
 def deffuant_loss(stats, target) -> float:
     return float(lib().orc_deffuant_loss(np.ascontiguousarray(stats, np.float64),
                                          np.ascontiguousarray(target, np.float64)))
+
+
+def deffuant_grad_threads(x0, eps, mu, rng0, steps: int, target, threads: int,
+                          mode: int = MODE_PARTITIONABLE):
+    """deffuant_grad with the instances over ``threads`` OpenMP threads (same
+    bits)."""
+    n, batch = x0.shape
+    loss = np.zeros(batch, np.float64)
+    ge = np.zeros(batch, np.float64)
+    gm = np.zeros(batch, np.float64)
+    lib().orc_deffuant_grad_threads(n, batch, np.ascontiguousarray(eps, np.float64),
+                                    np.ascontiguousarray(mu, np.float64),
+                                    np.ascontiguousarray(x0, np.float32).reshape(-1),
+                                    _key(rng0), steps, mode,
+                                    np.ascontiguousarray(target, np.float64), loss, ge, gm,
+                                    int(threads))
+    return loss, ge, gm
 
 
 def deffuant_grad(x0, eps, mu, rng0, steps: int, target, mode: int = MODE_PARTITIONABLE):

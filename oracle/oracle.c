@@ -862,6 +862,57 @@ int64_t orc_schelling_init(int32_t height, int32_t width, int64_t seed,
   return na;
 }
 
+/* Agent k's phase of a Schelling step: Moore-8 happiness on the torus and,
+ * when unhappy, the claimed empty cell (-1 otherwise). */
+static int schelling_agent(int32_t height, int32_t width, const int8_t *cells,
+                           const int32_t *agent_pos, const int8_t *agent_type, int64_t k,
+                           const uint32_t ak[2], const int32_t *empty, int64_t ne, int mode,
+                           int32_t *target) {
+  int32_t p = agent_pos[k];
+  int32_t r = p / width, c = p % width;
+  int32_t occ = 0, same = 0;
+  for (int dr = -1; dr <= 1; ++dr)
+    for (int dc = -1; dc <= 1; ++dc) {
+      if (!dr && !dc) continue;
+      int32_t rr = (r + dr + height) % height;
+      int32_t cc = (c + dc + width) % width;
+      int8_t v = cells[(int64_t)rr * width + cc];
+      if (v) {
+        occ++;
+        if (v == agent_type[k]) same++;
+      }
+    }
+  int h = 8 * same >= 3 * occ;
+  target[k] = -1;
+  if (!h && ne > 0) {
+    int32_t j;
+    orc_randint_i32(ak, 1, 0, (int32_t)ne, &j, mode);
+    target[k] = empty[j];
+  }
+  return h;
+}
+
+/* The env phase: each claimed cell goes to its lowest-id claimant, winners
+ * move; returns the moves. */
+static int32_t schelling_claims(int64_t nc, int8_t *cells, int32_t *agent_pos,
+                                const int8_t *agent_type, int64_t n_agents,
+                                const int32_t *target, int32_t *claim) {
+  for (int64_t c = 0; c < nc; ++c) claim[c] = INT32_MAX;
+  for (int64_t k = 0; k < n_agents; ++k)
+    if (target[k] >= 0 && (int32_t)k < claim[target[k]])
+      claim[target[k]] = (int32_t)k;
+  int32_t moves = 0;
+  for (int64_t k = 0; k < n_agents; ++k) {
+    if (target[k] >= 0 && claim[target[k]] == (int32_t)k) {
+      cells[agent_pos[k]] = 0;
+      cells[target[k]] = agent_type[k];
+      agent_pos[k] = target[k];
+      moves++;
+    }
+  }
+  return moves;
+}
+
 void orc_schelling_run(int32_t height, int32_t width, int8_t *cells,
                        int32_t *agent_pos, const int8_t *agent_type,
                        int64_t n_agents, uint32_t rng[2], int32_t steps,
@@ -879,43 +930,10 @@ void orc_schelling_run(int32_t height, int32_t width, int8_t *cells,
     for (int64_t c = 0; c < nc; ++c)
       if (cells[c] == 0) empty[ne++] = (int32_t)c;
     int64_t happy = 0;
-    for (int64_t k = 0; k < n_agents; ++k) {
-      int32_t p = agent_pos[k];
-      int32_t r = p / width, c = p % width;
-      int32_t occ = 0, same = 0;
-      for (int dr = -1; dr <= 1; ++dr)
-        for (int dc = -1; dc <= 1; ++dc) {
-          if (!dr && !dc) continue;
-          int32_t rr = (r + dr + height) % height;
-          int32_t cc = (c + dc + width) % width;
-          int8_t v = cells[(int64_t)rr * width + cc];
-          if (v) {
-            occ++;
-            if (v == agent_type[k]) same++;
-          }
-        }
-      int h = 8 * same >= 3 * occ;
-      happy += h;
-      target[k] = -1;
-      if (!h && ne > 0) {
-        int32_t j;
-        orc_randint_i32(ak + 2 * k, 1, 0, (int32_t)ne, &j, mode);
-        target[k] = empty[j];
-      }
-    }
-    for (int64_t c = 0; c < nc; ++c) claim[c] = INT32_MAX;
     for (int64_t k = 0; k < n_agents; ++k)
-      if (target[k] >= 0 && (int32_t)k < claim[target[k]])
-        claim[target[k]] = (int32_t)k;
-    int32_t moves = 0;
-    for (int64_t k = 0; k < n_agents; ++k) {
-      if (target[k] >= 0 && claim[target[k]] == (int32_t)k) {
-        cells[agent_pos[k]] = 0;
-        cells[target[k]] = agent_type[k];
-        agent_pos[k] = target[k];
-        moves++;
-      }
-    }
+      happy += schelling_agent(height, width, cells, agent_pos, agent_type, k, ak + 2 * k,
+                               empty, ne, mode, target);
+    int32_t moves = schelling_claims(nc, cells, agent_pos, agent_type, n_agents, target, claim);
     volatile float hf = (float)happy / (float)n_agents;
     happy_frac_out[t] = hf;
     moves_out[t] = moves;
@@ -924,6 +942,43 @@ void orc_schelling_run(int32_t height, int32_t width, int8_t *cells,
   free(claim);
   free(target);
   free(ak);
+}
+
+/* orc_schelling_run with the agent phase (neighbourhoods, happiness, agent
+ * keys and draws) on `threads` OpenMP threads: the same results, since each
+ * agent's phase reads only the pre-step grid and its own key; the empty list,
+ * the claims and the moves stay sequential. */
+void orc_schelling_run_threads(int32_t height, int32_t width, int8_t *cells,
+                               int32_t *agent_pos, const int8_t *agent_type,
+                               int64_t n_agents, uint32_t rng[2], int32_t steps,
+                               int mode, float *happy_frac_out, int32_t *moves_out,
+                               int threads) {
+  int64_t nc = (int64_t)height * width;
+  int32_t *empty = (int32_t *)malloc(sizeof(int32_t) * (size_t)nc);
+  int32_t *claim = (int32_t *)malloc(sizeof(int32_t) * (size_t)nc);
+  int32_t *target = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n_agents + 1));
+  for (int32_t t = 0; t < steps; ++t) {
+    uint32_t ck[2], uk[2];
+    step_keys(rng, ck, uk, mode);
+    int64_t ne = 0;
+    for (int64_t c = 0; c < nc; ++c)
+      if (cells[c] == 0) empty[ne++] = (int32_t)c;
+    int64_t happy = 0;
+#pragma omp parallel for num_threads(threads) schedule(static) reduction(+ : happy)
+    for (int64_t k = 0; k < n_agents; ++k) {
+      const uint32_t ak[2] = {split_element(ck, n_agents, 2 * k, mode),
+                              split_element(ck, n_agents, 2 * k + 1, mode)};
+      happy += schelling_agent(height, width, cells, agent_pos, agent_type, k, ak, empty, ne,
+                               mode, target);
+    }
+    int32_t moves = schelling_claims(nc, cells, agent_pos, agent_type, n_agents, target, claim);
+    volatile float hf = (float)happy / (float)n_agents;
+    happy_frac_out[t] = hf;
+    moves_out[t] = moves;
+  }
+  free(empty);
+  free(claim);
+  free(target);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1136,5 +1191,113 @@ void orc_deffuant_grad(int64_t n, int32_t batch, const double *eps,
   }
   free(lam);
   free(traj);
+  free(perms);
+}
+
+/* orc_deffuant_grad with the instances spread over `threads` OpenMP threads
+ * (a CPU baseline on all cores).  The pairings are drawn first, in step
+ * order; then each instance runs alone on its own column arrays, with the
+ * same operations in the same order as its column of orc_deffuant_grad, so
+ * every output has the same bits. */
+static void deffuant_grad_column(int64_t n, int32_t batch, int32_t b, double eps,
+                                 double mu, const float *x0, const int32_t *perms,
+                                 int32_t steps, const double *target, double *loss_out,
+                                 double *grad_eps, double *grad_mu) {
+  double *traj = (double *)malloc(sizeof(double) * (size_t)n * (size_t)(steps + 1));
+  double *lam = (double *)malloc(sizeof(double) * (size_t)n);
+  const double s = (double)DEFF_SCALE;
+  for (int64_t i = 0; i < n; ++i) traj[i] = x0[i * batch + b];
+  for (int32_t t = 0; t < steps; ++t) {
+    const int32_t *perm = perms + (size_t)t * n;
+    const double *xo = traj + (size_t)t * n;
+    double *xn = traj + (size_t)(t + 1) * n;
+    memcpy(xn, xo, sizeof(double) * (size_t)n);
+    for (int64_t i = 0; i + 1 < n; i += 2) {
+      int64_t a = perm[i], c = perm[i + 1];
+      double xa = xo[a], xc = xo[c];
+      double d = xc - xa;
+      double g = sigmoid_d((eps - fabs(d)) / s);
+      double delta = mu * g * d;
+      xn[a] = xa + delta;
+      xn[c] = xc - delta;
+    }
+  }
+  const double *xf = traj + (size_t)steps * n;
+  double m = 0.0, v = 0.0, h[ORC_HIST_BINS];
+  for (int64_t i = 0; i < n; ++i) m += xf[i];
+  m /= (double)n;
+  for (int64_t i = 0; i < n; ++i) {
+    double d = xf[i] - m;
+    v += d * d;
+  }
+  v /= (double)n;
+  for (int k = 0; k < ORC_HIST_BINS; ++k) h[k] = 0.0;
+  for (int64_t i = 0; i < n; ++i)
+    for (int k = 0; k < ORC_HIST_BINS; ++k) {
+      double xi = xf[i];
+      h[k] += sigmoid_d((xi - (double)k / ORC_HIST_BINS) * HIST_TAU_INV) -
+              sigmoid_d((xi - (double)(k + 1) / ORC_HIST_BINS) * HIST_TAU_INV);
+    }
+  double loss = (m - target[0]) * (m - target[0]) + (v - target[1]) * (v - target[1]);
+  double gh[ORC_HIST_BINS];
+  for (int k = 0; k < ORC_HIST_BINS; ++k) {
+    h[k] /= (double)n;
+    loss += (h[k] - target[2 + k]) * (h[k] - target[2 + k]);
+    gh[k] = 2.0 * (h[k] - target[2 + k]) / (double)n;
+  }
+  loss_out[b] = loss;
+  double gm = 2.0 * (m - target[0]) / (double)n;
+  double gv = 2.0 * (v - target[1]) * 2.0 / (double)n;
+  for (int64_t i = 0; i < n; ++i) {
+    double xi = xf[i];
+    double g = gm + gv * (xi - m);
+    for (int k = 0; k < ORC_HIST_BINS; ++k) {
+      double sl = sigmoid_d((xi - (double)k / ORC_HIST_BINS) * HIST_TAU_INV);
+      double sh = sigmoid_d((xi - (double)(k + 1) / ORC_HIST_BINS) * HIST_TAU_INV);
+      g += gh[k] * HIST_TAU_INV * (sl * (1.0 - sl) - sh * (1.0 - sh));
+    }
+    lam[i] = g;
+  }
+  double ge = 0.0, gmu = 0.0;
+  for (int32_t t = steps - 1; t >= 0; --t) {
+    const int32_t *perm = perms + (size_t)t * n;
+    const double *xo = traj + (size_t)t * n;
+    for (int64_t i = 0; i + 1 < n; i += 2) {
+      int64_t a = perm[i], c = perm[i + 1];
+      double xa = xo[a], xc = xo[c];
+      double d = xc - xa;
+      double ad = fabs(d);
+      double g = sigmoid_d((eps - ad) / s);
+      double gp = g * (1.0 - g) / s;
+      double sgn = d > 0 ? 1.0 : (d < 0 ? -1.0 : 0.0);
+      double fd = mu * (g - d * sgn * gp);
+      double fe = mu * d * gp;
+      double fm = g * d;
+      double la = lam[a], lc = lam[c];
+      double w = la - lc;
+      lam[a] = la - w * fd;
+      lam[c] = lc + w * fd;
+      ge += w * fe;
+      gmu += w * fm;
+    }
+  }
+  grad_eps[b] = ge;
+  grad_mu[b] = gmu;
+  free(lam);
+  free(traj);
+}
+
+void orc_deffuant_grad_threads(int64_t n, int32_t batch, const double *eps,
+                               const double *mu, const float *x0,
+                               const uint32_t rng0[2], int32_t steps, int mode,
+                               const double *target, double *loss_out,
+                               double *grad_eps, double *grad_mu, int32_t threads) {
+  int32_t *perms = (int32_t *)malloc(sizeof(int32_t) * (size_t)n * (size_t)steps);
+  uint32_t rng[2] = {rng0[0], rng0[1]};
+  for (int32_t t = 0; t < steps; ++t) deffuant_step_pairs(rng, n, mode, perms + (size_t)t * n);
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+  for (int32_t b = 0; b < batch; ++b)
+    deffuant_grad_column(n, batch, b, eps[b], mu[b], x0, perms, steps, target, loss_out,
+                         grad_eps, grad_mu);
   free(perms);
 }

@@ -102,6 +102,13 @@ void orc_schelling_run(int32_t height, int32_t width, int8_t *cells,
                        int32_t *agent_pos, const int8_t *agent_type,
                        int64_t n_agents, uint32_t rng[2], int32_t steps,
                        int mode, float *happy_frac_out, int32_t *moves_out);
+/* The same run with each step's agent phase on `threads` OpenMP threads
+ * (same results; a CPU baseline on all cores). */
+void orc_schelling_run_threads(int32_t height, int32_t width, int8_t *cells,
+                               int32_t *agent_pos, const int8_t *agent_type,
+                               int64_t n_agents, uint32_t rng[2], int32_t steps,
+                               int mode, float *happy_frac_out, int32_t *moves_out,
+                               int threads);
 
 void orc_deffuant_init(int64_t n, int32_t batch, int64_t seed, int mode,
                        float *x, uint32_t rng[2]);
@@ -116,6 +123,13 @@ void orc_deffuant_grad(int64_t n, int32_t batch, const double *eps,
                        const uint32_t rng0[2], int32_t steps, int mode,
                        const double *target, double *loss_out,
                        double *grad_eps, double *grad_mu);
+/* The same outputs, bit for bit, with the instances over `threads` OpenMP
+ * threads (a CPU baseline on all cores). */
+void orc_deffuant_grad_threads(int64_t n, int32_t batch, const double *eps,
+                               const double *mu, const float *x0,
+                               const uint32_t rng0[2], int32_t steps, int mode,
+                               const double *target, double *loss_out,
+                               double *grad_eps, double *grad_mu, int32_t threads);
 
 #define ORC_HIST_BINS 32
 #define ORC_STATS_LEN (2 + ORC_HIST_BINS)

@@ -384,3 +384,36 @@ def test_sir_threaded_run_is_identical():
     np.testing.assert_array_equal(c1, c2)
     np.testing.assert_array_equal(st1, st2)
     np.testing.assert_array_equal(r1, r2)
+
+
+def test_deffuant_grad_threads_same_bits_as_one_thread():
+    """The all-cores Deffuant gradient (instances over OpenMP threads, each on
+    its own column arrays) gives the single-threaded oracle's loss and
+    gradients bit for bit."""
+    n, batch, steps = 3000, 7, 6
+    x0, rng = O.deffuant_init(n, batch, 4, O.MODE_PARTITIONABLE)
+    eps = np.linspace(0.05, 0.5, batch)
+    mu = np.linspace(0.1, 0.5, batch)[::-1].copy()
+    target = np.linspace(0.0, 0.3, O.STATS_LEN)
+    a = O.deffuant_grad(x0, eps, mu, rng, steps, target)
+    for threads in (1, 3, 8):
+        b = O.deffuant_grad_threads(x0, eps, mu, rng, steps, target, threads)
+        for u, v in zip(a, b):
+            np.testing.assert_array_equal(u.view(np.uint64), v.view(np.uint64))
+
+
+@pytest.mark.parametrize("mode", [O.MODE_ORIGINAL, O.MODE_PARTITIONABLE])
+def test_schelling_run_threads_same_results(mode):
+    """The all-cores Schelling oracle (agent phase on OpenMP threads, keys per
+    agent) gives the single-threaded run's records, grid and positions."""
+    H, W, steps = 96, 80, 12
+    a = O.schelling_init(H, W, 3, mode)
+    b = tuple(np.array(v, copy=True) for v in a)
+    hf1, mv1 = O.schelling_run(H, W, a[0], a[1], a[2], a[3], steps, mode)
+    hf2, mv2 = O.schelling_run_threads(H, W, b[0], b[1], b[2], b[3], steps, 5, mode)
+    np.testing.assert_array_equal(hf1, hf2)
+    np.testing.assert_array_equal(mv1, mv2)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    np.testing.assert_array_equal(a[3], b[3])
+    assert mv1.sum() > 0
