@@ -182,11 +182,16 @@ __device__ __forceinline__ void wealth_steps(const abm_wealth_inst &in, float *g
 
 // n + 1 <= kWealthThreads with precomputed draws: one agent and one
 // histogram bin per thread, x, y and wealth in registers for the whole run,
-// increments and histogram in LDS; four barriers per step.  T and S are
-// reduced together.
+// increments and two histograms (steps alternate) in LDS; two barriers per
+// step.  Wealth is conserved, so the total T is summed once.  The rank sum S
+// of step t is taken by wave 0 alone from histogram t & 1 while the other
+// waves go on to step t + 1; that histogram's bins are cleared by their
+// owners after the next step's first barrier, when wave 0 has read them.
 __device__ __forceinline__ void wealth_steps_small(const abm_wealth_inst &in, int32_t *linc,
-                                                   int32_t *lhist, uint32_t *red_c,
+                                                   int32_t *lhist2, uint32_t *red_c,
                                                    uint32_t *red_t, uint32_t *red_s) {
+  (void)red_c;
+  (void)red_s;
   const int64_t n = in.n;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const bool agent = tid < n;
@@ -194,7 +199,8 @@ __device__ __forceinline__ void wealth_steps_small(const abm_wealth_inst &in, in
   int32_t w = agent ? in.w[tid] : 0;
   if (tid <= n) {
     linc[tid] = 0;
-    lhist[tid] = 0;
+    lhist2[tid] = 0;
+    lhist2[kWealthThreads + tid] = 0;
   }
   // the draws of step t + 1 are loaded during step t (barriers wait on LDS
   // only, so the loads overlap the histogram phases)
@@ -204,8 +210,19 @@ __device__ __forceinline__ void wealth_steps_small(const abm_wealth_inst &in, in
     dy = in.draws[n + tid];
     dt = in.draws[2 * n + tid];
   }
+  // T: the total wealth, the same at every step
+  uint32_t tw = agent ? (uint32_t)w : 0u;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) tw += __shfl_xor(tw, off);
+  if (lane == 0) red_t[wave] = tw;
   __syncthreads();
+  uint32_t T = 0;
+  for (int q = 0; q < kWealthThreads / 64; ++q) T += red_t[q];
+  const float mean = (float)(int32_t)T / (float)n;
+  const float cst = (float)((double)(n + 1) / (double)n);
   for (int32_t t = 0; t < in.steps; ++t) {
+    int32_t *hist = lhist2 + (t & 1) * kWealthThreads;
+    int32_t *prev = lhist2 + ((t + 1) & 1) * kWealthThreads;
     const float mdx = __uint_as_float(dx), mdy = __uint_as_float(dy);
     const uint32_t target = dt;
     if (agent && t + 1 < in.steps) {
@@ -220,58 +237,47 @@ __device__ __forceinline__ void wealth_steps_small(const abm_wealth_inst &in, in
       if (w > 0) atomicAdd(&linc[target], 1);
     }
     __syncthreads();
+    if (tid <= n) prev[tid] = 0;  // step t - 1's bins, read by wave 0 before this barrier
     if (agent) {
       w = w - (w > 0 ? 1 : 0) + linc[tid];
       linc[tid] = 0;
-      atomicAdd(&lhist[w], 1);  // 0 <= w <= n: wealth is conserved
+      atomicAdd(&hist[w], 1);  // 0 <= w <= n: wealth is conserved
     }
     __syncthreads();
-    // bin `tid` of the histogram: agents of wealth tid and their rank sum
-    const uint32_t c = tid <= n ? (uint32_t)lhist[tid] : 0u;
-    if (c) lhist[tid] = 0;
-    uint32_t inc = c;
+    if (wave == 0) {
+      // rank sum from the histogram, 64 bins a round until all n agents are
+      // counted (one round while every wealth is below 64)
+      uint32_t sp = 0, counted = 0;
+      for (int64_t b0 = 0; counted < (uint32_t)n && b0 <= n; b0 += 64) {
+        const int64_t wv = b0 + lane;
+        const uint32_t c = wv <= n ? (uint32_t)hist[wv] : 0u;
+        uint32_t inc = c;
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const uint32_t v = __shfl_up(inc, off);
-      if (lane >= off) inc += v;
-    }
-    uint32_t tw = agent ? (uint32_t)w : 0u;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) tw += __shfl_xor(tw, off);
-    if (lane == 63) red_c[wave] = inc;
-    if (lane == 0) red_t[wave] = tw;
-    __syncthreads();
-    // bins before this wave: lanes below `wave` each read one wave total
-    uint32_t below = lane < wave ? red_c[lane] : 0u;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) below += __shfl_xor(below, off);
-    const uint32_t before = below + inc - c;
-    uint32_t sp = 0;
-    if (c) {
-      const uint64_t ranksum = (uint64_t)c * before + (uint64_t)c * (c + 1) / 2;
-      sp = (uint32_t)tid * (uint32_t)ranksum;
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) sp += __shfl_xor(sp, off);
-    if (lane == 0) red_s[wave] = sp;
-    __syncthreads();
-    if (tid == 0) {
-      uint32_t T = 0, S = 0;
-      for (int q = 0; q < kWealthThreads / 64; ++q) {
-        T += red_t[q];
-        S += red_s[q];
+        for (int off = 1; off < 64; off <<= 1) {
+          const uint32_t v = __shfl_up(inc, off);
+          if (lane >= off) inc += v;
+        }
+        const uint32_t before = counted + inc - c;
+        if (c) {
+          const uint64_t ranksum = (uint64_t)c * before + (uint64_t)c * (c + 1) / 2;
+          sp += (uint32_t)wv * (uint32_t)ranksum;
+        }
+        counted += __shfl(inc, 63);
       }
-      in.mean_out[t] = (float)(int32_t)T / (float)n;
-      const int32_t Ti = (int32_t)T;
-      float g = 0.0f;
-      if ((float)Ti > 1e-6f) {
-        const int32_t num = (int32_t)(2u * S);
-        const int32_t den = (int32_t)((uint32_t)n * T);
-        const float q = (float)num / (float)den;
-        const float cst = (float)((double)(n + 1) / (double)n);
-        g = q - cst;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) sp += __shfl_xor(sp, off);
+      if (lane == 0) {
+        in.mean_out[t] = mean;
+        const int32_t Ti = (int32_t)T;
+        float g = 0.0f;
+        if ((float)Ti > 1e-6f) {
+          const int32_t num = (int32_t)(2u * sp);
+          const int32_t den = (int32_t)((uint32_t)n * T);
+          const float q = (float)num / (float)den;
+          g = q - cst;
+        }
+        in.gini_out[t] = g;
       }
-      in.gini_out[t] = g;
     }
   }
   if (agent) {
@@ -294,12 +300,13 @@ __global__ void __launch_bounds__(kWealthThreads)
   const int64_t n = in.n;
   const int tid = threadIdx.x;
   if (in.draws && n + 1 <= kWealthThreads) {
-    // increments and histogram: the dynamic LDS (at least 2 x kWealthThreads
-    // words in this launch), or a static array where the launch has none
+    // increments and two histograms: the dynamic LDS (at least 3 x
+    // kWealthThreads words in this launch), or a static array where the
+    // launch has none
     if constexpr (kLds) {
       wealth_steps_small(in, lds_agents, lds_agents + kWealthThreads, red, red_t, red_s);
     } else {
-      __shared__ int32_t small_lds[2 * kWealthThreads];
+      __shared__ int32_t small_lds[3 * kWealthThreads];
       wealth_steps_small(in, small_lds, small_lds + kWealthThreads, red, red_t, red_s);
     }
   } else if (kLds && n <= max_lds) {
@@ -526,7 +533,9 @@ int abm_wealth_run_batch(const abm_wealth_inst *insts, int32_t count, int64_t ma
   if (max_n > 0 && max_n <= kWealthLdsAgents) {
     if (int rc = wealth_lds_attr()) return rc;
     size_t dyn = (size_t)(max_n * kWealthLdsBytesPerAgent + 4);
-    if (dyn < 2 * kWealthThreads * sizeof(int32_t)) dyn = 2 * kWealthThreads * sizeof(int32_t);
+    // an instance of n < kWealthThreads keeps its increments and two
+    // histograms there
+    if (dyn < 3 * kWealthThreads * sizeof(int32_t)) dyn = 3 * kWealthThreads * sizeof(int32_t);
     hipLaunchKernelGGL(k_wealth_run<true>, dim3((unsigned)count), dim3(kWealthThreads), dyn,
                        (hipStream_t)stream, insts, max_n, mode);
   } else {
