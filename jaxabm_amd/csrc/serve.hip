@@ -10,8 +10,8 @@
 // with every other active instance in ticks of K steps:
 //   resident path  one workgroup owns a small instance and runs its K steps
 //                  in one launch, the working set in LDS (SIR: both state
-//                  copies; wealth: packed 16-bit incoming counts and the
-//                  wealth histogram)
+//                  copies; wealth: packed 16-bit incoming counts and two
+//                  alternating wealth histograms)
 //   stepwise path  a large instance is split into agent ranges; each phase of
 //                  a step is one launch over the ranges of every stepwise
 //                  instance (SIR: one phase, double-buffered state; wealth:
@@ -57,8 +57,8 @@ constexpr int kResWaves = kResThreads / 64;
 constexpr int kStepThreads = 256;
 constexpr int kStepWaves = kStepThreads / 64;
 constexpr int64_t kResidentMax = 65536;        // agents of a resident instance
-constexpr int kHistLds = 4096;                 // wealth bins kept in LDS
-constexpr size_t kResLds = 2 * kResidentMax + kHistLds * 4;  // 144 KiB
+constexpr int kHistLds = 2048;                 // wealth bins per LDS histogram (two, alternating)
+constexpr size_t kResLds = 2 * kResidentMax + 2 * kHistLds * 4;  // 144 KiB
 constexpr int kHeavyDeg = 64;                  // rows scanned by a whole wavefront
 constexpr int kResHeavyCap = 1024;
 constexpr int kStepHeavyCap = 256;
@@ -222,8 +222,12 @@ __device__ __forceinline__ bool sir_heavy_hit(const Inst &in, const uint8_t *old
 }
 
 // ---------------------------------------------------------------- resident path
+// Two barriers per step: after the light rows (heavy-row list complete), and
+// after the heavy rows and the counts.  The S/I/R counts and the heavy-row
+// count live in LDS, double-buffered by step parity: a step's buffer is
+// reset during the step before it, after every thread has read it.
 __device__ void sir_resident(const Inst &in, int32_t t_end, int mode, uint8_t *lds,
-                             int32_t *heavy, int32_t *nheavy, uint32_t *red) {
+                             int32_t *heavy, int32_t *nheavy2, uint32_t *cnt2) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t n = in.n;
   // a step after one that left no infected agent is the identity: its
@@ -236,11 +240,14 @@ __device__ void sir_resident(const Inst &in, int32_t t_end, int mode, uint8_t *l
   }
   uint8_t *cur = lds, *nxt = lds + kResidentMax;
   for (int64_t i = tid; i < n; i += kResThreads) cur[i] = in.state0[i];
+  if (tid == 0) nheavy2[in.t0 & 1] = 0;
+  if (tid < 3) cnt2[3 * (in.t0 & 1) + tid] = 0;
+  __syncthreads();
   for (int32_t t = in.t0; t < t_end; ++t) {
     const Key kinf{in.keys[4 * t], in.keys[4 * t + 1]};
     const Key krec{in.keys[4 * t + 2], in.keys[4 * t + 3]};
-    if (tid == 0) *nheavy = 0;
-    __syncthreads();
+    int32_t *nheavy = nheavy2 + (t & 1);
+    uint32_t *cnt = cnt2 + 3 * (t & 1);
     uint32_t cs = 0, ci = 0, cr = 0;
     for (int64_t v = tid; v < n; v += kResThreads) {
       uint8_t s = cur[v];
@@ -269,20 +276,32 @@ __device__ void sir_resident(const Inst &in, int32_t t_end, int mode, uint8_t *l
         cs += !hit;
       }
     }
-    cs = block_sum<kResThreads>(cs, red);
-    ci = block_sum<kResThreads>(ci, red);
-    cr = block_sum<kResThreads>(cr, red);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      cs += __shfl_xor(cs, off);
+      ci += __shfl_xor(ci, off);
+      cr += __shfl_xor(cr, off);
+    }
+    if (lane == 0) {
+      if (cs) atomicAdd(&cnt[0], cs);
+      if (ci) atomicAdd(&cnt[1], ci);
+      if (cr) atomicAdd(&cnt[2], cr);
+    }
+    if (tid == 0) nheavy2[(t + 1) & 1] = 0;  // read last before this step's first barrier
+    if (tid < 3) cnt2[3 * ((t + 1) & 1) + tid] = 0;
+    __syncthreads();
+    const uint32_t S = cnt[0], I = cnt[1], Rc = cnt[2];
     if (tid == 0) {
-      in.counts[3 * t] = (int32_t)cs;
-      in.counts[3 * t + 1] = (int32_t)ci;
-      in.counts[3 * t + 2] = (int32_t)cr;
+      in.counts[3 * t] = (int32_t)S;
+      in.counts[3 * t + 1] = (int32_t)I;
+      in.counts[3 * t + 2] = (int32_t)Rc;
     }
     uint8_t *tmp = cur;
     cur = nxt;
     nxt = tmp;
-    if (ci == 0) {  // block-uniform: every later step is the identity
+    if (I == 0) {  // block-uniform: every later step is the identity
       for (int64_t i = tid; i < 3 * (int64_t)(t_end - 1 - t); i += kResThreads)
-        in.counts[3 * (t + 1) + i] = (int32_t)(i % 3 == 0 ? cs : (i % 3 == 1 ? 0u : cr));
+        in.counts[3 * (t + 1) + i] = (int32_t)(i % 3 == 0 ? S : (i % 3 == 1 ? 0u : Rc));
       break;
     }
   }
@@ -290,18 +309,31 @@ __device__ void sir_resident(const Inst &in, int32_t t_end, int mode, uint8_t *l
   for (int64_t i = tid; i < n; i += kResThreads) in.state0[i] = cur[i];
 }
 
+// Two barriers per step: after the exchange increments and after the
+// histogram.  Wealth is conserved, so the total T is summed once per launch;
+// the rank sum S of step t is taken by wave 0 alone from histogram t & 1 (64
+// bins a round until all n agents are counted) while the other waves go on to
+// step t + 1.  That histogram's LDS bins are cleared by their owners after
+// the next step's first barrier; values of kHistLds or more count in the
+// instance's global bins, which wave 0 clears as it reads them.  The packed
+// increments of agents 2j and 2j + 1 are read and cleared by one thread
+// before the second barrier.
 __device__ void wealth_resident(const Inst &in, int32_t t_end, int mode, uint8_t *lds,
-                                uint32_t *red, uint32_t *carry) {
-  const int tid = threadIdx.x;
+                                uint32_t *red) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t n = in.n;
   uint32_t *inc2 = reinterpret_cast<uint32_t *>(lds);  // two 16-bit counts per word
-  uint32_t *hist = reinterpret_cast<uint32_t *>(lds + 2 * kResidentMax);
+  uint32_t *hist2 = reinterpret_cast<uint32_t *>(lds + 2 * kResidentMax);  // 2 x kHistLds
   const int64_t words = (n + 1) >> 1;
   const uint32_t span = (uint32_t)(n - 1);
   for (int64_t i = tid; i < words; i += kResThreads) inc2[i] = 0;
-  for (int i = tid; i < kHistLds; i += kResThreads) hist[i] = 0;
-  __syncthreads();
+  for (int i = tid; i < 2 * kHistLds; i += kResThreads) hist2[i] = 0;
+  uint32_t tw = 0;
+  for (int64_t i = tid; i < n; i += kResThreads) tw += (uint32_t)in.w[i];
+  const uint32_t T = block_sum<kResThreads>(tw, red);  // its barriers order the clears too
   for (int32_t t = in.t0; t < t_end; ++t) {
+    uint32_t *hist = hist2 + (t & 1) * kHistLds;
+    uint32_t *prev = hist2 + ((t + 1) & 1) * kHistLds;
     const Key ck{in.keys[4 * t], in.keys[4 * t + 1]};
     const Key uk{in.keys[4 * t + 2], in.keys[4 * t + 3]};
     Key kk1, kk2;
@@ -312,27 +344,55 @@ __device__ void wealth_resident(const Inst &in, int32_t t_end, int mode, uint8_t
       if (ti >= 0) atomicAdd(&inc2[ti >> 1], 1u << ((ti & 1) * 16));
     }
     __syncthreads();
-    uint32_t total = 0;
-    int32_t mx = 0;
-    for (int64_t i = tid; i < n; i += kResThreads) {
-      const int32_t got = (int32_t)((inc2[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu);
-      const int32_t w0 = in.w[i];
-      const int32_t wi = w0 - (w0 > 0 ? 1 : 0) + got;
-      in.w[i] = wi;
-      total += (uint32_t)wi;
-      mx = max(mx, wi);
-      if (wi < kHistLds)
-        atomicAdd(&hist[wi], 1u);
-      else
-        atomicAdd(&in.hist[wi], 1);  // 0 <= wi <= n: wealth is conserved
+    for (int b = tid; b < kHistLds; b += kResThreads) prev[b] = 0;
+    for (int64_t j = tid; j < words; j += kResThreads) {
+      const uint32_t word = inc2[j];
+      inc2[j] = 0;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int64_t i = 2 * j + h;
+        if (i >= n) break;
+        const int32_t got = (int32_t)((word >> (16 * h)) & 0xFFFFu);
+        const int32_t w0 = in.w[i];
+        const int32_t wi = w0 - (w0 > 0 ? 1 : 0) + got;
+        in.w[i] = wi;
+        if (wi < kHistLds)
+          atomicAdd(&hist[wi], 1u);
+        else
+          atomicAdd(&in.hist[wi], 1);  // 0 <= wi <= n: wealth is conserved
+      }
     }
     __syncthreads();
-    for (int64_t i = tid; i < words; i += kResThreads) inc2[i] = 0;
-    const uint32_t T = block_sum<kResThreads>(total, red);
-    const int32_t maxw = block_max<kResThreads>(mx, red);
-    uint32_t S = rank_sum_and_clear<kResThreads>(hist, kHistLds, in.hist, maxw, red, carry);
-    S = block_sum<kResThreads>(S, red);
-    if (tid == 0) wealth_record(n, T, S, in.mean, in.gini, t);
+    if (wave == 0) {
+      uint32_t sp = 0, counted = 0;
+      for (int64_t b0 = 0; counted < (uint32_t)n && b0 <= n; b0 += 64) {
+        const int64_t wv = b0 + lane;
+        uint32_t c = 0;
+        if (wv <= n) {
+          if (wv < kHistLds) {
+            c = hist[wv];
+          } else {
+            c = (uint32_t)in.hist[wv];
+            if (c) in.hist[wv] = 0;
+          }
+        }
+        uint32_t incl = c;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          const uint32_t y = __shfl_up(incl, off);
+          if (lane >= off) incl += y;
+        }
+        const uint32_t before = counted + incl - c;
+        if (c) {
+          const uint64_t rs = (uint64_t)c * before + (uint64_t)c * (c + 1) / 2;
+          sp += (uint32_t)wv * (uint32_t)rs;
+        }
+        counted += __shfl(incl, 63);
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) sp += __shfl_xor(sp, off);
+      if (lane == 0) wealth_record(n, T, sp, in.mean, in.gini, t);
+    }
   }
 }
 
@@ -341,16 +401,16 @@ __global__ void __launch_bounds__(kResThreads)
                      int mode) {
   extern __shared__ __align__(16) uint8_t lds[];
   __shared__ int32_t heavy[kResHeavyCap];
-  __shared__ int32_t nheavy;
+  __shared__ int32_t nheavy2[2];
+  __shared__ uint32_t cnt2[6];
   __shared__ uint32_t red[kResWaves];
-  __shared__ uint32_t carry;
   const Inst in = insts[list[blockIdx.x]];
   const int32_t t_end = min(in.t0 + K, in.steps);
   if (in.t0 >= t_end) return;
   if (in.kind == kSir)
-    sir_resident(in, t_end, mode, lds, heavy, &nheavy, red);
+    sir_resident(in, t_end, mode, lds, heavy, nheavy2, cnt2);
   else
-    wealth_resident(in, t_end, mode, lds, red, &carry);
+    wealth_resident(in, t_end, mode, lds, red);
 }
 
 // ---------------------------------------------------------------- stepwise path
