@@ -414,7 +414,26 @@ __global__ void __launch_bounds__(kResThreads)
 }
 
 // ---------------------------------------------------------------- stepwise path
+// Mean and Gini of wealth step t (one workgroup): rank sum of the histogram
+// phase 2 built, the total and the largest wealth from acc; clears the bins
+// and acc for the next step.
+__device__ void wealth_stepwise_record(const Inst &in, int32_t t, uint32_t *red,
+                                       uint32_t *carry) {
+  const int32_t maxw = in.acc[0];
+  const uint32_t T = (uint32_t)in.acc[1];
+  uint32_t S = rank_sum_and_clear<kStepThreads>(nullptr, 0, in.hist, maxw, red, carry);
+  S = block_sum<kStepThreads>(S, red);
+  if (threadIdx.x == 0) {
+    wealth_record(in.n, T, S, in.mean, in.gini, t);
+    in.acc[0] = 0;
+    in.acc[1] = 0;
+  }
+}
+
 // Phase 1 of sub-step s: SIR step of the range, or the wealth agent phase.
+// For wealth, the block of the instance's first range first records step
+// t - 1 of this tick (s >= 1), also when t is past the last step; the last
+// sub-step of a tick is recorded by phase 3.
 __global__ void __launch_bounds__(kStepThreads)
     k_serve_phase1(const Inst *__restrict__ insts, const Range *__restrict__ ranges, int32_t s,
                    int mode) {
@@ -422,9 +441,12 @@ __global__ void __launch_bounds__(kStepThreads)
   __shared__ int32_t marking[kStepHeavyCap];
   __shared__ int32_t nheavy, nmark;
   __shared__ uint32_t red[kStepWaves];
+  __shared__ uint32_t carry;
   const Range r = ranges[blockIdx.x];
   const Inst in = insts[r.inst];
   const int32_t t = in.t0 + s;
+  if (in.kind == kWealth && s > 0 && r.begin == 0 && t - 1 < in.steps)  // block-uniform
+    wealth_stepwise_record(in, t - 1, red, &carry);
   if (t >= in.steps) return;  // block-uniform
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (in.kind == kWealth) {
@@ -551,7 +573,8 @@ __global__ void __launch_bounds__(kStepThreads)
   }
 }
 
-// Phase 3 (wealth): one workgroup per instance records mean and Gini.
+// Phase 3 (wealth): one workgroup per instance records mean and Gini of
+// sub-step s (the tick's last; the others are recorded by phase 1).
 __global__ void __launch_bounds__(kStepThreads)
     k_serve_phase3(const Inst *__restrict__ insts, const int32_t *__restrict__ list, int32_t s) {
   __shared__ uint32_t red[kStepWaves];
@@ -559,15 +582,7 @@ __global__ void __launch_bounds__(kStepThreads)
   const Inst in = insts[list[blockIdx.x]];
   const int32_t t = in.t0 + s;
   if (t >= in.steps) return;
-  const int32_t maxw = in.acc[0];
-  const uint32_t T = (uint32_t)in.acc[1];
-  uint32_t S = rank_sum_and_clear<kStepThreads>(nullptr, 0, in.hist, maxw, red, &carry);
-  S = block_sum<kStepThreads>(S, red);
-  if (threadIdx.x == 0) {
-    wealth_record(in.n, T, S, in.mean, in.gini, t);
-    in.acc[0] = 0;
-    in.acc[1] = 0;
-  }
+  wealth_stepwise_record(in, t, red, &carry);
 }
 
 // ---------------------------------------------------------------- admission
@@ -1233,13 +1248,14 @@ int Server::launch_tick() {
     for (int32_t s = 0; s < K; ++s) {
       hipLaunchKernelGGL(k_serve_phase1, dim3((unsigned)i1), dim3(kStepThreads), 0, sW_, di, d1,
                          s, cfg_.mode);
-      if (i2) {
+      if (i2)
         hipLaunchKernelGGL(k_serve_phase2, dim3((unsigned)i2), dim3(kStepThreads), 0, sW_, di, d2,
                            s);
-        hipLaunchKernelGGL(k_serve_phase3, dim3((unsigned)nw3), dim3(kStepThreads), 0, sW_, di,
-                           dw3, s);
-      }
     }
+    // phase 1 of sub-step s records wealth step s - 1; the tick's last here
+    if (i2)
+      hipLaunchKernelGGL(k_serve_phase3, dim3((unsigned)nw3), dim3(kStepThreads), 0, sW_, di,
+                         dw3, K - 1);
   }
   ABM_LAUNCH_CHECK("serve tick");
   auto take = [this]() {
