@@ -461,6 +461,18 @@ __global__ void __launch_bounds__(kStepThreads)
     }
     return;
   }
+  // the step's keys and this thread's first agent (state, mark, row bounds)
+  // are loaded together with the previous step's infected count
+  const Key kinf{in.keys[4 * t], in.keys[4 * t + 1]};
+  const Key krec{in.keys[4 * t + 2], in.keys[4 * t + 3]};
+  const uint8_t *old = (t & 1) ? in.state1 : in.state0;
+  uint8_t *nxt = (t & 1) ? in.state0 : in.state1;
+  const int64_t v_first = r.begin + tid;
+  const bool has_first = v_first < r.end;
+  const uint8_t st_first = has_first ? old[v_first] : 0;
+  const uint16_t mk_first = has_first ? in.mark[v_first] : 0;
+  const int32_t e0_first = has_first ? in.row_ptr[v_first] : 0;
+  const int32_t e1_first = has_first ? in.row_ptr[v_first + 1] : 0;
   // identity step (the previous one, computed by an earlier launch, left no
   // infected agent): only the counts carry over; the state, not part of the
   // results, is never read again
@@ -468,22 +480,21 @@ __global__ void __launch_bounds__(kStepThreads)
     if (r.begin == 0 && tid < 3) in.counts[3 * t + tid] = in.counts[3 * (t - 1) + tid];
     return;
   }
-  const Key kinf{in.keys[4 * t], in.keys[4 * t + 1]};
-  const Key krec{in.keys[4 * t + 2], in.keys[4 * t + 3]};
-  const uint8_t *old = (t & 1) ? in.state1 : in.state0;
-  uint8_t *nxt = (t & 1) ? in.state0 : in.state1;
   const uint16_t new_mark = (uint16_t)((uint32_t)t + kMarkHorizon);
   if (tid == 0) nheavy = nmark = 0;
   __syncthreads();
   uint32_t cs = 0, ci = 0, cr = 0;
-  for (int64_t v = r.begin + tid; v < r.end; v += kStepThreads) {
-    const uint8_t st = old[v];
-    if (st == 0 && (uint16_t)(in.mark[v] - (uint16_t)t) > kMarkHorizon) {
+  for (int64_t v = v_first; v < r.end; v += kStepThreads) {
+    const bool first = v == v_first;
+    const uint8_t st = first ? st_first : old[v];
+    const uint16_t mk = first ? mk_first : in.mark[v];
+    if (st == 0 && (uint16_t)(mk - (uint16_t)t) > kMarkHorizon) {
       nxt[v] = 0;  // no neighbour can be infected
       ++cs;
       continue;
     }
-    const int32_t e0 = in.row_ptr[v], e1 = in.row_ptr[v + 1];
+    const int32_t e0 = first ? e0_first : in.row_ptr[v];
+    const int32_t e1 = first ? e1_first : in.row_ptr[v + 1];
     if (st == 0 && e1 - e0 > kHeavyDeg) {
       const int32_t slot = atomicAdd(&nheavy, 1);
       if (slot < kStepHeavyCap) {
