@@ -12,10 +12,11 @@
 //                  in one launch, the working set in LDS (SIR: both state
 //                  copies; wealth: packed 16-bit incoming counts and two
 //                  alternating wealth histograms)
-//   stepwise path  a large instance is split into agent ranges; each phase of
-//                  a step is one launch over the ranges of every stepwise
-//                  instance (SIR: one phase, double-buffered state; wealth:
-//                  agent phase + scatter, apply + histogram, record)
+//   stepwise path  a large instance is split into agent ranges; each step is
+//                  one launch over the ranges of every stepwise instance
+//                  (SIR: double-buffered state; wealth: step t's agent phase
+//                  and scatter, step t - 1's apply and histogram and step
+//                  t - 2's record, with double-buffered increments)
 // Instances join and leave at tick boundaries (continuous batching).  Every
 // instance runs the same arithmetic whatever else is on the GPU, so its
 // results are identical alone and in a mix.
@@ -83,7 +84,10 @@ struct Inst {
   uint16_t *mark;            // SIR: neighbour-infection marks (stepwise path)
   int32_t *counts;           // SIR [steps][3]
   float *x, *y;
-  int32_t *w, *inc, *hist, *acc;  // wealth: hist[n+1], acc = (max wealth, total)
+  // wealth: w[n]; the stepwise path keeps two copies of the exchange
+  // increments inc[2][n], the histogram hist[2][n+1] and acc[2] = (max wealth,
+  // total), one per step parity (the resident path uses copy 0 of hist)
+  int32_t *w, *inc, *hist, *acc;
   float *mean, *gini;             // wealth [steps]
 };
 
@@ -414,26 +418,87 @@ __global__ void __launch_bounds__(kResThreads)
 }
 
 // ---------------------------------------------------------------- stepwise path
-// Mean and Gini of wealth step t (one workgroup): rank sum of the histogram
-// phase 2 built, the total and the largest wealth from acc; clears the bins
-// and acc for the next step.
+// Mean and Gini of wealth step t (one workgroup): rank sum of the step's
+// histogram copy, the total and the largest wealth from its acc copy; clears
+// both for step t + 2.
 __device__ void wealth_stepwise_record(const Inst &in, int32_t t, uint32_t *red,
                                        uint32_t *carry) {
-  const int32_t maxw = in.acc[0];
-  const uint32_t T = (uint32_t)in.acc[1];
-  uint32_t S = rank_sum_and_clear<kStepThreads>(nullptr, 0, in.hist, maxw, red, carry);
+  int32_t *hist = in.hist + (int64_t)(t & 1) * (in.n + 1);
+  int32_t *acc = in.acc + 2 * (t & 1);
+  const int32_t maxw = acc[0];
+  const uint32_t T = (uint32_t)acc[1];
+  uint32_t S = rank_sum_and_clear<kStepThreads>(nullptr, 0, hist, maxw, red, carry);
   S = block_sum<kStepThreads>(S, red);
   if (threadIdx.x == 0) {
     wealth_record(in.n, T, S, in.mean, in.gini, t);
-    in.acc[0] = 0;
-    in.acc[1] = 0;
+    acc[0] = 0;
+    acc[1] = 0;
   }
 }
 
-// Phase 1 of sub-step s: SIR step of the range, or the wealth agent phase.
-// For wealth, the block of the instance's first range first records step
-// t - 1 of this tick (s >= 1), also when t is past the last step; the last
-// sub-step of a tick is recorded by phase 3.
+// Wealth sub-step t of range r (see k_serve_phase1).  The exchange of step
+// t - 1 and the agent phase of step t touch each agent from the same thread;
+// step t's increments go to the other inc copy, so no agent reads an
+// increment of the step being drawn.
+__device__ void wealth_stepwise(const Inst &in, const Range &r, int32_t t, int mode,
+                                uint32_t *red, uint32_t *carry, uint32_t *bins) {
+  const int tid = threadIdx.x;
+  const int64_t n = in.n;
+  if (r.begin == 0 && t >= 2) wealth_stepwise_record(in, t - 2, red, carry);  // block-uniform
+  const bool apply = t >= 1 && t - 1 < in.steps;  // block-uniform
+  const bool draw = t < in.steps;
+  int32_t *inc_old = in.inc + (int64_t)((t + 1) & 1) * n;  // step t - 1's increments
+  int32_t *inc_new = in.inc + (int64_t)(t & 1) * n;
+  int32_t *hist = in.hist + (int64_t)((t + 1) & 1) * (n + 1);  // step t - 1's histogram
+  int32_t *acc = in.acc + 2 * ((t + 1) & 1);
+  if (apply) {
+    bins[tid] = 0;
+    __syncthreads();
+  }
+  Key ck{0u, 0u}, kk1{0u, 0u}, kk2{0u, 0u};
+  if (draw) {
+    ck = Key{in.keys[4 * t], in.keys[4 * t + 1]};
+    split2(Key{in.keys[4 * t + 2], in.keys[4 * t + 3]}, kk1, kk2, mode);
+  }
+  const uint32_t span = (uint32_t)(n - 1);
+  uint32_t total = 0;
+  int32_t mx = 0;
+  for (int64_t i = r.begin + tid; i < r.end; i += kStepThreads) {
+    if (apply) {
+      const int32_t w0 = in.w[i];
+      const int32_t wi = w0 - (w0 > 0 ? 1 : 0) + inc_old[i];
+      in.w[i] = wi;
+      inc_old[i] = 0;
+      total += (uint32_t)wi;
+      mx = max(mx, wi);
+      if (wi < kStepBins)
+        atomicAdd(&bins[wi], 1u);
+      else
+        atomicAdd(&hist[wi], 1);  // 0 <= wi <= n: wealth is conserved
+    }
+    if (draw) {
+      const int32_t ti = wealth_agent(in, i, ck, kk1, kk2, span, mode);
+      if (ti >= 0) atomicAdd(&inc_new[ti], 1);
+    }
+  }
+  if (apply) {
+    __syncthreads();
+    if (bins[tid]) atomicAdd(&hist[tid], (int32_t)bins[tid]);
+    const uint32_t T = block_sum<kStepThreads>(total, red);
+    const int32_t m = block_max<kStepThreads>(mx, red);
+    if (tid == 0) {
+      atomicAdd(&acc[1], (int32_t)T);
+      atomicMax(&acc[0], m);
+    }
+  }
+}
+
+// Phase 1 of sub-step s, the one launch per step: SIR step of the range, or
+// for wealth (t = t0 + s): the record of step t - 2 (by the block of the
+// instance's first range), the exchange of step t - 1 applied to the range's
+// agents (wealth, histogram, total and maximum), then step t's agent phase
+// (positions and exchange increments).  A wealth instance runs two sub-steps
+// past its last step to apply and record it.
 __global__ void __launch_bounds__(kStepThreads)
     k_serve_phase1(const Inst *__restrict__ insts, const Range *__restrict__ ranges, int32_t s,
                    int mode) {
@@ -442,25 +507,17 @@ __global__ void __launch_bounds__(kStepThreads)
   __shared__ int32_t nheavy, nmark;
   __shared__ uint32_t red[kStepWaves];
   __shared__ uint32_t carry;
+  __shared__ uint32_t bins[kStepBins];
   const Range r = ranges[blockIdx.x];
   const Inst in = insts[r.inst];
   const int32_t t = in.t0 + s;
-  if (in.kind == kWealth && s > 0 && r.begin == 0 && t - 1 < in.steps)  // block-uniform
-    wealth_stepwise_record(in, t - 1, red, &carry);
-  if (t >= in.steps) return;  // block-uniform
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (in.kind == kWealth) {
-    const Key ck{in.keys[4 * t], in.keys[4 * t + 1]};
-    const Key uk{in.keys[4 * t + 2], in.keys[4 * t + 3]};
-    Key kk1, kk2;
-    split2(uk, kk1, kk2, mode);
-    const uint32_t span = (uint32_t)(in.n - 1);
-    for (int64_t i = r.begin + tid; i < r.end; i += kStepThreads) {
-      const int32_t ti = wealth_agent(in, i, ck, kk1, kk2, span, mode);
-      if (ti >= 0) atomicAdd(&in.inc[ti], 1);
-    }
+    if (t >= in.steps + 2) return;  // block-uniform
+    wealth_stepwise(in, r, t, mode, red, &carry, bins);
     return;
   }
+  if (t >= in.steps) return;  // block-uniform
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   // the step's keys and this thread's first agent (state, mark, row bounds)
   // are loaded together with the previous step's infected count
   const Key kinf{in.keys[4 * t], in.keys[4 * t + 1]};
@@ -548,54 +605,6 @@ __global__ void __launch_bounds__(kStepThreads)
   }
 }
 
-// Phase 2 (wealth): apply the exchange; histogram, max and total.
-__global__ void __launch_bounds__(kStepThreads)
-    k_serve_phase2(const Inst *__restrict__ insts, const Range *__restrict__ ranges, int32_t s) {
-  __shared__ uint32_t bins[kStepBins];
-  __shared__ uint32_t red[kStepWaves];
-  const Range r = ranges[blockIdx.x];
-  const Inst in = insts[r.inst];
-  const int32_t t = in.t0 + s;
-  if (t >= in.steps) return;
-  const int tid = threadIdx.x;
-  bins[tid] = 0;
-  __syncthreads();
-  uint32_t total = 0;
-  int32_t mx = 0;
-  for (int64_t i = r.begin + tid; i < r.end; i += kStepThreads) {
-    const int32_t w0 = in.w[i];
-    const int32_t wi = w0 - (w0 > 0 ? 1 : 0) + in.inc[i];
-    in.w[i] = wi;
-    in.inc[i] = 0;
-    total += (uint32_t)wi;
-    mx = max(mx, wi);
-    if (wi < kStepBins)
-      atomicAdd(&bins[wi], 1u);
-    else
-      atomicAdd(&in.hist[wi], 1);
-  }
-  __syncthreads();
-  if (bins[tid]) atomicAdd(&in.hist[tid], (int32_t)bins[tid]);
-  const uint32_t T = block_sum<kStepThreads>(total, red);
-  const int32_t m = block_max<kStepThreads>(mx, red);
-  if (tid == 0) {
-    atomicAdd(&in.acc[1], (int32_t)T);
-    atomicMax(&in.acc[0], m);
-  }
-}
-
-// Phase 3 (wealth): one workgroup per instance records mean and Gini of
-// sub-step s (the tick's last; the others are recorded by phase 1).
-__global__ void __launch_bounds__(kStepThreads)
-    k_serve_phase3(const Inst *__restrict__ insts, const int32_t *__restrict__ list, int32_t s) {
-  __shared__ uint32_t red[kStepWaves];
-  __shared__ uint32_t carry;
-  const Inst in = insts[list[blockIdx.x]];
-  const int32_t t = in.t0 + s;
-  if (t >= in.steps) return;
-  wealth_stepwise_record(in, t, red, &carry);
-}
-
 // ---------------------------------------------------------------- admission
 __global__ void k_serve_init_wealth(Inst in, uint32_t ik0, uint32_t ik1, int mode) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -606,9 +615,13 @@ __global__ void k_serve_init_wealth(Inst in, uint32_t ik0, uint32_t ik1, int mod
     in.y[i] = unit_float(bits_scalar(ky, mode));
     in.w[i] = 1;
     in.inc[i] = 0;
+    in.inc[in.n + i] = 0;
   }
-  if (i <= in.n) in.hist[i] = 0;
-  if (i < 2) in.acc[i] = 0;
+  if (i <= in.n) {
+    in.hist[i] = 0;
+    in.hist[in.n + 1 + i] = 0;
+  }
+  if (i < 4) in.acc[i] = 0;
 }
 
 __global__ void k_serve_init_sir(Inst in) {
@@ -788,7 +801,7 @@ class Server {
   void admit();
   void activate_uploads();
   void fail(Req *r, const std::string &msg);
-  int ensure_tables(size_t n_inst, size_t n1, size_t n2);
+  int ensure_tables(size_t n_inst, size_t n1);
 
   abm_serve_config cfg_;
   int device_ = 0;
@@ -843,7 +856,7 @@ int Server::start() {
   arena_.init(cfg_.hbm_budget);
   ABM_CHECK_HIP(hipFuncSetAttribute((const void *)k_serve_resident,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kResLds));
-  if (int rc = ensure_tables(cfg_.max_active, 4096, 4096)) return rc;
+  if (int rc = ensure_tables(cfg_.max_active, 4096)) return rc;
   for (int i = 0; i < cfg_.prep_threads; ++i) workers_.emplace_back([this] { prep_loop(); });
   sched_ = std::thread([this] { sched_loop(); });
   return kOk;
@@ -882,11 +895,11 @@ Server::~Server() {
 }
 
 // Table sets: pinned host / device pairs, each at least `bytes`; a tick
-// packs Inst[A] | Range[n1] | Range[n2] | int32 resident[] | int32 wealth[]
-// at 256-byte aligned offsets and copies only what it uses.
-int Server::ensure_tables(size_t n_inst, size_t n1, size_t n2) {
+// packs Inst[A] | Range[n1] | int32 resident[] at 256-byte aligned offsets
+// and copies only what it uses.
+int Server::ensure_tables(size_t n_inst, size_t n1) {
   const size_t need = align256(sizeof(Inst) * n_inst) + align256(sizeof(Range) * n1) +
-                      align256(sizeof(Range) * n2) + 2 * align256(4 * n_inst);
+                      align256(4 * n_inst);
   if (need <= sets_[0].bytes) return kOk;
   // the sets may be in use by queued ticks: drain them first
   for (hipStream_t s : {sR_, sW_}) ABM_CHECK_HIP(hipStreamSynchronize(s));
@@ -1038,9 +1051,9 @@ void Server::prepare(Req *r) {
     r->off_x = place(4 * (size_t)n);
     r->off_y = place(4 * (size_t)n);
     r->off_w = place(4 * (size_t)n);
-    r->off_inc = place(4 * (size_t)n);
-    r->off_hist = place(4 * ((size_t)n + 1));
-    r->off_acc = place(8);
+    r->off_inc = place(8 * (size_t)n);           // two copies (step parity)
+    r->off_hist = place(8 * ((size_t)n + 1));
+    r->off_acc = place(16);
     r->off_mean = place(8 * (size_t)steps);  // mean[steps] then gini[steps]
     r->off_gini = r->off_mean + 4 * (size_t)steps;
     r->result_off = r->off_mean;
@@ -1202,33 +1215,24 @@ void Server::activate_uploads() {
 
 int Server::launch_tick() {
   const int32_t K = cfg_.steps_per_tick;
-  size_t n1 = 0, n2 = 0;
+  size_t n1 = 0;
   for (Req *r : active_)
-    if (!r->resident) {
-      n1 += r->ranges.size() - 1;
-      if (r->d.kind == kWealth) n2 += r->ranges.size() - 1;
-    }
-  if (int rc = ensure_tables(active_.size(), n1, n2)) return rc;
+    if (!r->resident) n1 += r->ranges.size() - 1;
+  if (int rc = ensure_tables(active_.size(), n1)) return rc;
   TableSet &ts = sets_[tick_idx_ % kSets];
   const size_t o1 = align256(sizeof(Inst) * active_.size());
   const size_t o2 = o1 + align256(sizeof(Range) * n1);
-  const size_t o3 = o2 + align256(sizeof(Range) * n2);
-  const size_t o4 = o3 + align256(4 * active_.size());
-  const size_t used = o4 + align256(4 * active_.size());
+  const size_t used = o2 + align256(4 * active_.size());
   char *h = (char *)ts.host;
   Inst *hi = (Inst *)h;
   Range *h1 = (Range *)(h + o1);
-  Range *h2 = (Range *)(h + o2);
-  int32_t *hres = (int32_t *)(h + o3);
-  int32_t *hw3 = (int32_t *)(h + o4);
+  int32_t *hres = (int32_t *)(h + o2);
   const char *dv = (const char *)ts.dev;
   const Inst *di = (const Inst *)dv;
   const Range *d1 = (const Range *)(dv + o1);
-  const Range *d2 = (const Range *)(dv + o2);
-  const int32_t *dres = (const int32_t *)(dv + o3);
-  const int32_t *dw3 = (const int32_t *)(dv + o4);
-  int32_t nres = 0, nw3 = 0;
-  size_t i1 = 0, i2 = 0;
+  const int32_t *dres = (const int32_t *)(dv + o2);
+  int32_t nres = 0;
+  size_t i1 = 0;
   Tick tick;
   for (size_t i = 0; i < active_.size(); ++i) {
     Req *r = active_[i];
@@ -1237,17 +1241,16 @@ int Server::launch_tick() {
     if (r->resident) {
       hres[nres++] = (int32_t)i;
     } else {
-      for (size_t c = 0; c + 1 < r->ranges.size(); ++c) {
-        const Range rg{(int32_t)i, r->ranges[c], r->ranges[c + 1], r->d.kind};
-        h1[i1++] = rg;
-        if (r->d.kind == kWealth) h2[i2++] = rg;
-      }
-      if (r->d.kind == kWealth) hw3[nw3++] = (int32_t)i;
+      for (size_t c = 0; c + 1 < r->ranges.size(); ++c)
+        h1[i1++] = Range{(int32_t)i, r->ranges[c], r->ranges[c + 1], r->d.kind};
     }
-    const int32_t run = std::min<int32_t>(K, r->d.steps - r->t0);
+    const int32_t run = std::max<int32_t>(0, std::min<int32_t>(K, r->d.steps - r->t0));
     agent_steps_ += (int64_t)run * r->d.n;
     r->t0 += K;
-    if (r->t0 >= r->d.steps) tick.finishing.push_back(r);
+    // a stepwise wealth instance applies and records its last step in the
+    // two sub-steps after it
+    const int32_t end = r->d.steps + (!r->resident && r->d.kind == kWealth ? 2 : 0);
+    if (r->t0 >= end) tick.finishing.push_back(r);
   }
   ABM_CHECK_HIP(hipMemcpyAsync(ts.dev, ts.host, used, hipMemcpyHostToDevice, sW_));
   ABM_CHECK_HIP(hipEventRecord(ev_tab_, sW_));
@@ -1255,19 +1258,9 @@ int Server::launch_tick() {
   if (nres)
     hipLaunchKernelGGL(k_serve_resident, dim3((unsigned)nres), dim3(kResThreads), kResLds, sR_,
                        di, dres, K, cfg_.mode);
-  if (i1) {
-    for (int32_t s = 0; s < K; ++s) {
-      hipLaunchKernelGGL(k_serve_phase1, dim3((unsigned)i1), dim3(kStepThreads), 0, sW_, di, d1,
-                         s, cfg_.mode);
-      if (i2)
-        hipLaunchKernelGGL(k_serve_phase2, dim3((unsigned)i2), dim3(kStepThreads), 0, sW_, di, d2,
-                           s);
-    }
-    // phase 1 of sub-step s records wealth step s - 1; the tick's last here
-    if (i2)
-      hipLaunchKernelGGL(k_serve_phase3, dim3((unsigned)nw3), dim3(kStepThreads), 0, sW_, di,
-                         dw3, K - 1);
-  }
+  for (int32_t s = 0; i1 && s < K; ++s)
+    hipLaunchKernelGGL(k_serve_phase1, dim3((unsigned)i1), dim3(kStepThreads), 0, sW_, di, d1, s,
+                       cfg_.mode);
   ABM_LAUNCH_CHECK("serve tick");
   auto take = [this]() {
     if (!free_events_.empty()) {
