@@ -35,9 +35,9 @@ operand's sign.
   ``beta`` on it give the same values as the vmapped draws (``lognormal``
   and ``exponential`` are built from ``normal`` and ``uniform``).
 * Per-agent vectors of up to 16 values are unrolled.  float32 sums, means,
-  products and 2-norms over 1 to 16 of them follow the order torch's
-  reduction uses for a row of that length; integer sums and products are
-  exact.
+  products, 2-norms, variances and standard deviations over 1 to 16 of them
+  follow the order torch's reduction uses for a row of that length; integer
+  sums and products are exact.
 
 * ``model_state['agents_<name>'][var][j]`` (another agent's value, j a
   per-agent integer) is a gather; out-of-range indices follow JAX's clamp.
@@ -915,6 +915,36 @@ class _Trace:
         sq = [self.tmp("float", f"({v.name} * {v.name})") for v in x.items]
         return Sym(self, self.tmp("float", f"sqrtf({self._f32_sum(sq)})"), "f32", x.varying)
 
+    def vvar(self, x: SymVec, correction: float, take_sqrt: bool) -> Sym:
+        """torch.var / torch.std over a per-agent float32 vector, as torch's
+        Welford reduction gives them (measured with torch 2.10 on ROCm for
+        k = 1..20, also under vmap).  A = _sum_slots(k) lanes; lane l holds
+        value l and, when l + A < k, value l + A, each started from the
+        empty accumulator and the two merged; the lanes merge in adjacent
+        pairs.  Merges are Chan et al.'s pairwise update with its two
+        multiply-adds fused (abm_wf_merge); the result is m2 / (n - c), or
+        m2 / 0 when n <= c, and its square root for std."""
+        k = len(x)
+        if x.kind != "f32" or not 1 <= k <= _MAX_VEC:
+            raise Unsupported("a variance of a vector other than 1..16 float32 values")
+        A = self._sum_slots(k)
+        names = [v.name for v in x.items]
+        lanes = []
+        for lane in range(A):
+            acc = self.tmp("abm_welford", f"abm_wf_first({names[lane]})")
+            if lane + A < k:
+                acc = self.tmp("abm_welford",
+                               f"abm_wf_merge({acc}, abm_wf_first({names[lane + A]}))")
+            lanes.append(acc)
+        while len(lanes) > 1:
+            lanes = [self.tmp("abm_welford", f"abm_wf_merge({lanes[i]}, {lanes[i + 1]})")
+                     for i in range(0, len(lanes), 2)]
+        c = _lit(float(np.float32(correction)), "f32")
+        v = self.tmp("float", f"abm_wf_var({lanes[0]}, {c})")
+        if take_sqrt:
+            v = self.tmp("float", f"sqrtf({v})")
+        return Sym(self, v, "f32", x.varying)
+
     def vextreme(self, x: SymVec, largest: bool, along_dim: bool) -> Sym:
         """torch.max / torch.min (whole vector) and torch.amax / torch.amin
         (over its dim) of a per-agent vector, as torch's reductions give
@@ -1408,6 +1438,16 @@ class _Trace:
                 kwargs.get("ord") in (None, 2) and kwargs.get("dim") in (None, 0, -1) and \
                 not kwargs.get("keepdim") and kwargs.get("dtype") is None:
             return self.vnorm(args[0])
+        if name in ("var", "std") and len(args) == 1 and isinstance(args[0], SymVec) and \
+                set(kwargs) <= {"dim", "keepdim", "correction", "unbiased"} and \
+                kwargs.get("dim") in (None, 0, -1, (0,), (-1,), [0], [-1]) and \
+                not kwargs.get("keepdim") and \
+                not ("correction" in kwargs and "unbiased" in kwargs):
+            corr = kwargs.get("correction")
+            if corr is None:
+                corr = 1 if kwargs.get("unbiased", True) else 0
+            if isinstance(corr, (bool, int, float)) and not isinstance(corr, bool):
+                return self.vvar(args[0], float(corr), take_sqrt=name == "std")
         if name == "stack" and len(args) >= 1 and isinstance(args[0], (list, tuple)) and \
                 kwargs.get("dim", args[1] if len(args) > 1 else 0) in (0, -1):
             return self.stack(args[0])
@@ -1492,6 +1532,30 @@ template <typename T> __device__ __forceinline__ T abm_max(T a, T b) {
   if (abm_isnan(a)) return a;
   if (abm_isnan(b)) return b;
   return abm_hi2<T>(a, b);
+}
+// Welford accumulators as torch's var/std reduction keeps them in float32
+// (mean, sum of squared deviations, count); abm_wf_first is one value added
+// to the empty accumulator, abm_wf_merge the pairwise merge of Chan, Golub
+// and LeVeque (1979) with its two multiply-adds fused, as torch's compiled
+// merge does, and abm_wf_var the variance with correction c
+struct abm_welford { float m, m2, nf; };
+__device__ __forceinline__ abm_welford abm_wf_first(float x) {
+  const float d = x - 0.0f;
+  const float m = 0.0f + d / 1.0f;
+  return {m, 0.0f + d * (x - m), 1.0f};
+}
+__device__ __forceinline__ abm_welford abm_wf_merge(abm_welford a, abm_welford b) {
+  if (a.nf == 0.0f) return b;
+  if (b.nf == 0.0f) return a;
+  const float d = b.m - a.m;
+  const float cnt = a.nf + b.nf;
+  const float w = b.nf / cnt;
+  const float t = d * d * a.nf;
+  return {__builtin_fmaf(d, w, a.m), __builtin_fmaf(t, w, a.m2 + b.m2), cnt};
+}
+__device__ __forceinline__ float abm_wf_var(abm_welford a, float c) {
+  const float div = a.nf > c ? a.nf - c : 0.0f;
+  return a.m2 / div;
 }
 template <typename T> __device__ __forceinline__ T abm_clamp_lo(T x, T lo) {
   if (abm_isnan(x)) return x;

@@ -113,7 +113,7 @@ def test_vector_limits_refused():
 @pytest.mark.parametrize("k", [1, 8, 11, 16])
 def test_vector_sums_products_norms_trace_and_compile(k):
     """Reductions over per-agent vectors of 1..16 values: float32 sums,
-    means, products, 2-norms, max / min / argmax / argmin, int32 products
+    means, products, 2-norms, variances, max / min / argmax / argmin, int32 products
     (JAX x64 off) and a -0.5 power, all in one generated kernel."""
     st = {"p": torch.rand(N, k) - 0.5, "q": torch.randint(-3, 4, (N, k), dtype=torch.int32)}
 
@@ -123,14 +123,17 @@ def test_vector_sums_products_norms_trace_and_compile(k):
                 "pv": v.prod(), "nm": jnp.linalg.norm(v), "ip": jnp.prod(s["q"]),
                 "is": jnp.sum(s["q"]), "mx": jnp.max(v), "amn": jnp.min(v, axis=0),
                 "am": jnp.argmax(v), "iam": jnp.argmin(s["q"]), "imx": s["q"].max(),
-                "rs": jnp.abs(v) ** -0.5, "rq": torch.rsqrt(jnp.abs(v))}
+                "rs": jnp.abs(v) ** -0.5, "rq": torch.rsqrt(jnp.abs(v)),
+                "vr": jnp.var(v), "sd": jnp.std(v, ddof=1), "tv": torch.var(v, dim=0)}
 
     name, src, tr, outs = _trace(upd, st, {"env": {}})
     assert {kk: vv.dtype for kk, vv in outs.items()} == {
         "s": torch.float32, "m": torch.float32, "pr": torch.float32, "pv": torch.float32,
         "nm": torch.float32, "ip": torch.int32, "is": torch.int32, "mx": torch.float32,
         "amn": torch.float32, "am": torch.int32, "iam": torch.int32, "imx": torch.int32,
-        "rs": torch.float32, "rq": torch.float32}
+        "rs": torch.float32, "rq": torch.float32, "vr": torch.float32, "sd": torch.float32,
+        "tv": torch.float32}
+    assert "abm_wf_merge" in src if k > 1 else "abm_wf_first" in src
     fuse.compile_source(src)
 
 

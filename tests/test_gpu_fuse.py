@@ -177,8 +177,8 @@ def test_fused_equals_vmapped(J, agent, n, mode):
 
 
 def test_refused_function_runs_vmapped(J):
-    """A variance over a per-agent vector is not covered: the update runs
-    the vmapped path (and the refusal is remembered); the init is fused."""
+    """A sort of a per-agent vector is not covered: the update runs the
+    vmapped path (and the refusal is remembered); the init is fused."""
     from jaxabm_amd import fuse
     from jaxabm_amd import random as jr
 
@@ -189,7 +189,7 @@ def test_refused_function_runs_vmapped(J):
         def update(self, s, ms, cfg, key):
             from jaxabm_amd import numpy as jnp
 
-            return {"p": s["p"] * jnp.var(s["p"])}
+            return {"p": jnp.sort(s["p"]) * 2.0}
 
     before = fuse.stats()["launches"]
     c = J.AgentCollection(Vec(), 50)
@@ -198,11 +198,10 @@ def test_refused_function_runs_vmapped(J):
     c.update({"env": {}}, jr.PRNGKey(2), J.ModelConfig())
     assert c.states["p"].shape == (50, 2)
     assert fuse.stats()["launches"] == before + 1
-    assert any("var" in r for r in fuse.refused().values())
+    assert any("sort" in r for r in fuse.refused().values())
     import torch
 
-    v = torch.func.vmap(lambda r: torch.var(r, correction=0))(p0)
-    assert torch.equal(c.states["p"], p0 * v[:, None])
+    assert torch.equal(c.states["p"], torch.sort(p0, dim=1, stable=True).values * 2.0)
 
 
 OPS = {
@@ -371,8 +370,9 @@ def test_facade_walker_model_fused_equals_vmapped(J):
 
 @pytest.mark.parametrize("k", list(range(1, 17)))
 def test_vector_reductions_fused_equal_vmapped(J, k):
-    """jnp.sum / jnp.mean / jnp.prod / jnp.linalg.norm over a per-agent
-    vector of k float32 values (the orders torch reduces them in), integer
+    """jnp.sum / jnp.mean / jnp.prod / jnp.linalg.norm / jnp.var / jnp.std
+    over a per-agent vector of k float32 values (the orders torch reduces
+    them in), integer
     counts and products, on values over +-2^10 with signed zeros (products
     of 16 such values overflow to inf; the scaled product stays finite);
     max / min (whole vector and over its axis) and argmax / argmin of
@@ -415,7 +415,11 @@ def test_vector_reductions_fused_equal_vmapped(J, k):
                     "amx": jnp.max(s["q"], axis=0), "amn": jnp.min(s["q"], axis=0),
                     "am": jnp.argmax(s["q"]), "an": jnp.argmin(s["q"]),
                     "imx": jnp.max(s["iq"]), "iam": jnp.argmax(s["iq"]),
-                    "rs": jnp.abs(v) ** -0.5}
+                    "rs": jnp.abs(v) ** -0.5,
+                    "vr": jnp.var(v), "vr1": jnp.var(v, ddof=1), "sd": jnp.std(v),
+                    "tv": torch.var(v), "ts": torch.std(v, unbiased=False),
+                    "vc": torch.var(v, dim=-1, correction=2.5), "qv": jnp.var(s["q"]),
+                    "vs": jnp.var(v * 1e-3 + 1000.0)}
 
     res = []
     for fused in (False, True):
