@@ -137,6 +137,28 @@ def test_vector_sums_products_norms_trace_and_compile(k):
     fuse.compile_source(src)
 
 
+def test_dot_products_trace_compile_and_refuse():
+    """jnp.dot / torch.matmul / @ of two per-agent float32 vectors of up to
+    15 values is one multiply-add chain; 16 values and a product with an env
+    vector are refused (torch orders those otherwise)."""
+    st = {"p": torch.rand(N, 15), "q": torch.rand(N, 16)}
+
+    def upd(s, m, key):
+        v = s["p"]
+        return {"d": jnp.dot(v, v * 2.0), "m": v @ (v + 1.0), "t": torch.dot(v, v)}
+
+    name, src, tr, outs = _trace(upd, st, {"env": {}})
+    assert {kk: vv.dtype for kk, vv in outs.items()} == {"d": torch.float32, "m": torch.float32,
+                                                         "t": torch.float32}
+    assert src.count("__builtin_fmaf(") >= 45
+    fuse.compile_source(src)
+    with pytest.raises(fuse.Unsupported, match="dot product"):
+        _trace(lambda s, m, k: {"d": jnp.dot(s["q"], s["q"])}, st, {"env": {}})
+    with pytest.raises(fuse.Unsupported, match="dot product"):
+        _trace(lambda s, m, k: {"d": jnp.dot(s["p"], m["env"]["w"])}, st,
+               {"env": {"w": torch.rand(15)}}, )
+
+
 def test_walker_with_vectors_traces_and_compiles():
     """A bouncing walker in the style of the reference's basic example
     (examples/basic_example.py:20-70): 2-vectors of position and velocity,

@@ -440,6 +440,54 @@ def test_vector_reductions_fused_equal_vmapped(J, k):
         np.testing.assert_array_equal(bits(a[kk]), bits(b[kk]), err_msg=kk)
 
 
+@pytest.mark.parametrize("k", list(range(1, 16)))
+def test_dot_products_fused_equal_vmapped(J, k):
+    """jnp.dot, torch.matmul and @ of two per-agent float32 vectors of k <= 15
+    values (one multiply-add chain, torch's batched-product order), on
+    values over +-2^8 with signed zeros, fused and vmapped alike."""
+    import torch
+
+    from jaxabm_amd import fuse
+    from jaxabm_amd import numpy as jnp
+    from jaxabm_amd import random as jr
+
+    g = torch.Generator(device="cuda").manual_seed(100 + k)
+    n = 30_000
+    p = torch.randn(n, k, device="cuda", generator=g) * torch.exp2(
+        torch.randint(-8, 9, (n, k), device="cuda", generator=g).float())
+    z = torch.rand(n, k, device="cuda", generator=g)
+    p[z < 0.1] = 0.0
+    p[(z >= 0.1) & (z < 0.2)] = -0.0
+    q = torch.randn(n, k, device="cuda", generator=g)
+
+    class Dot:
+        def init_state(self, cfg, key):
+            return {"x": 0.0}
+
+        def update(self, s, ms, cfg, key):
+            v, w = s["p"], s["q"]
+            return {"d": jnp.dot(v, w), "m": torch.matmul(w, v), "a": v @ (w * 0.5),
+                    "s": jnp.dot(v, v) - jnp.sum(v * v)}
+
+    res = []
+    for fused in (False, True):
+        fuse.enable(fused)
+        try:
+            before = fuse.stats()["launches"]
+            c = J.AgentCollection(Dot(), n)
+            c.model_config = J.ModelConfig()
+            c._states = {"p": p.clone(), "q": q.clone()}
+            c.update({"env": {}}, jr.PRNGKey(1), J.ModelConfig())
+            assert fuse.stats()["launches"] - before == (1 if fused else 0), fuse.refused()
+            res.append({kk: vv.clone() for kk, vv in c.states.items()})
+        finally:
+            fuse.enable(True)
+    a, b = res
+    for kk in a:
+        assert a[kk].dtype == b[kk].dtype, kk
+        np.testing.assert_array_equal(bits(a[kk]), bits(b[kk]), err_msg=kk)
+
+
 def test_vector_at_updates_and_bool_subtraction_fused_equal_vmapped(J):
     """x.at[j].set / add / multiply on per-agent vectors, .astype, and the
     subtraction of a boolean (JAX promotes it; the vmapped path runs it under
