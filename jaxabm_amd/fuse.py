@@ -38,7 +38,9 @@ operand's sign.
   products, 2-norms, variances and standard deviations over 1 to 16 of them
   follow the order torch's reduction uses for a row of that length, and dot
   products of two such vectors of up to 15 values the order of torch's
-  batched product; integer sums and products are exact.
+  batched product; cumulative sums follow torch's scan, and stable sorts
+  and argsorts torch's radix key order; integer sums, products and
+  cumulative sums are exact.
 
 * ``model_state['agents_<name>'][var][j]`` (another agent's value, j a
   per-agent integer) is a gather; out-of-range indices follow JAX's clamp.
@@ -524,6 +526,11 @@ class SymVec:
             raise Unsupported("a reduction with arguments over a per-agent vector")
         return self.tr.vprod(self)
 
+    def reshape(self, *shape):
+        if shape in ((-1,), ((-1,),), (len(self.items),), ((len(self.items),),)):
+            return self
+        raise Unsupported("reshape of a per-agent vector")
+
     def __matmul__(self, other):
         return self.tr.vdot(self, other)
 
@@ -970,6 +977,78 @@ class _Trace:
         for x, y in zip(a.items, b.items):
             acc = self.tmp("float", f"__builtin_fmaf({x.name}, {y.name}, {acc})")
         return Sym(self, acc, "f32", True)
+
+    def vcumsum(self, x: SymVec, out_kind: Optional[str] = None) -> SymVec:
+        """torch.cumsum over a per-agent vector.  float32: torch's scan of a
+        row (measured with torch 2.10 on ROCm for k = 1..20, also under
+        vmap): the first value plus the zero carry-in, then Sklansky levels
+        m = 0, 1, ...: with s = 2^m, every position in the upper half of a
+        block of 2s adds the last position of the lower half.  Integers and
+        booleans: int64 (int32, wrapping, when asked for), exact in any
+        order."""
+        k, kind = len(x), x.kind
+        if kind in ("bool", "u8", "i32", "i64"):
+            ok = out_kind or "i64"
+            ct = _CT[ok]
+            acc, out = None, []
+            for v in x.items:
+                term = f"({ct})({v.name})"
+                acc = self.tmp(ct, term if acc is None else
+                               (f"({ct})((uint64_t)(int64_t){acc} + (uint64_t)(int64_t){term})"
+                                if ok == "i64" else
+                                f"({ct})((uint32_t){acc} + (uint32_t){term})"))
+                out.append(Sym(self, acc, ok, x.varying))
+            return SymVec(self, out)
+        if kind != "f32" or not 1 <= k <= _MAX_VEC:
+            raise Unsupported("a cumulative sum of a vector other than 1..16 float32 values")
+        buf = [v.name for v in x.items]
+        buf[0] = self.tmp("float", f"({buf[0]} + 0.0f)")
+        m = 0
+        while (1 << m) < k:
+            s = 1 << m
+            for t in range(k):
+                a = ((t >> m) << (m + 1)) | s
+                ti, si = a + (t % s), a - 1
+                if ti < k:
+                    buf[ti] = self.tmp("float", f"({buf[ti]} + {buf[si]})")
+            m += 1
+        return SymVec(self, [Sym(self, b, "f32", x.varying) for b in buf])
+
+    def vsort(self, x: SymVec) -> tuple:
+        """torch.sort(stable=True) / torch.argsort(stable=True) of a
+        per-agent vector: (values, int64 indices).  float32 keys are ordered
+        as torch's radix sort orders them (measured with torch 2.10 on ROCm
+        for k = 1..16 on rows of ties, signed zeros, infinities and NaNs of
+        both signs, also under vmap): -0 counts as +0, then the bits are
+        mapped to an unsigned order (abm_sort_key), so a NaN with its sign
+        bit set comes first and one without it last; equal keys keep their
+        order.  Integers and booleans: by value, stable.  The rank of value
+        i is the number of values placed before it; output p selects the
+        value of rank p."""
+        k, kind = len(x), x.kind
+        if kind not in ("bool", "u8", "i32", "i64", "f32"):
+            raise Unsupported("a sort of this dtype")
+        ct = _CT[kind]
+        if kind == "f32":
+            keys = [self.tmp("uint32_t", f"abm_sort_key({v.name})") for v in x.items]
+        else:
+            keys = [v.name for v in x.items]
+        rank = ["0"] * k
+        for i in range(k):
+            for j in range(i + 1, k):
+                after = self.tmp("int", f"(int)({keys[j]} < {keys[i]})")   # j placed before i
+                rank[i] = self.tmp("int", f"({rank[i]} + {after})")
+                rank[j] = self.tmp("int", f"({rank[j]} + 1 - {after})")
+        vals, idxs = [], []
+        for p in range(k):
+            v, ix = x.items[0].name, "(int64_t)0"
+            for i in range(1, k):
+                hit = self.tmp("bool", f"({rank[i]} == {p})")
+                v = self.tmp(ct, f"({hit} ? {x.items[i].name} : {v})")
+                ix = self.tmp("int64_t", f"({hit} ? (int64_t){i} : {ix})")
+            vals.append(Sym(self, v, kind, x.varying))
+            idxs.append(Sym(self, ix, "i64", x.varying))
+        return SymVec(self, vals), SymVec(self, idxs)
 
     def vextreme(self, x: SymVec, largest: bool, along_dim: bool) -> Sym:
         """torch.max / torch.min (whole vector) and torch.amax / torch.amin
@@ -1470,6 +1549,20 @@ class _Trace:
         if name in ("__matmul__", "__rmatmul__") and len(args) == 2 and \
                 (isinstance(args[0], SymVec) or isinstance(args[1], SymVec)):
             return self.vdot(*(args if name == "__matmul__" else args[::-1]))
+        if name == "cumsum" and len(args) in (1, 2) and isinstance(args[0], SymVec) and \
+                set(kwargs) <= {"dim", "dtype"}:
+            dim = args[1] if len(args) == 2 else kwargs.get("dim")
+            dt = kwargs.get("dtype")
+            int_in = args[0].kind in ("bool", "u8", "i32")
+            if dim in (0, -1) and (dt is None or (dt == torch.int32 and int_in)):
+                return self.vcumsum(args[0], out_kind="i32" if dt == torch.int32 else None)
+        if name in ("sort", "argsort") and len(args) in (1, 2) and isinstance(args[0], SymVec) and \
+                set(kwargs) <= {"dim", "stable", "descending"} and kwargs.get("stable") is True \
+                and not kwargs.get("descending"):
+            dim = args[1] if len(args) == 2 else kwargs.get("dim", -1)
+            if dim in (0, -1):
+                values, indices = self.vsort(args[0])
+                return indices if name == "argsort" else torch.return_types.sort((values, indices))
         if name in ("var", "std") and len(args) == 1 and isinstance(args[0], SymVec) and \
                 set(kwargs) <= {"dim", "keepdim", "correction", "unbiased"} and \
                 kwargs.get("dim") in (None, 0, -1, (0,), (-1,), [0], [-1]) and \
@@ -1588,6 +1681,14 @@ __device__ __forceinline__ abm_welford abm_wf_merge(abm_welford a, abm_welford b
 __device__ __forceinline__ float abm_wf_var(abm_welford a, float c) {
   const float div = a.nf > c ? a.nf - c : 0.0f;
   return a.m2 / div;
+}
+// the order torch's radix sort gives float32 keys: -0 as +0, then the bits
+// mapped so that unsigned order is the float order (a NaN with its sign bit
+// set below -inf, one without it above +inf)
+__device__ __forceinline__ uint32_t abm_sort_key(float x) {
+  uint32_t b = __float_as_uint(x);
+  if (b == 0x80000000u) b = 0u;
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 template <typename T> __device__ __forceinline__ T abm_clamp_lo(T x, T lo) {
   if (abm_isnan(x)) return x;

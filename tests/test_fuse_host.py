@@ -159,6 +159,44 @@ def test_dot_products_trace_compile_and_refuse():
                {"env": {"w": torch.rand(15)}}, )
 
 
+def test_cumulative_sums_trace_and_compile():
+    """jnp.cumsum of float32, int32 and boolean per-agent vectors: a
+    vector of the input's length, float32 / int32 / int32 (JAX x64 off)."""
+    st = {"p": torch.rand(N, 13), "q": torch.randint(-3, 4, (N, 13), dtype=torch.int32)}
+
+    def upd(s, m, key):
+        return {"c": jnp.cumsum(s["p"]), "i": jnp.cumsum(s["q"]), "b": jnp.cumsum(s["p"] > 0.5),
+                "t": torch.cumsum(s["p"], dim=-1)}
+
+    name, src, tr, outs = _trace(upd, st, {"env": {}})
+    assert {kk: (vv.dtype, tuple(vv.shape)) for kk, vv in outs.items()} == {
+        "c": (torch.float32, (N, 13)), "i": (torch.int32, (N, 13)), "b": (torch.int32, (N, 13)),
+        "t": (torch.float32, (N, 13))}
+    fuse.compile_source(src)
+
+
+def test_sorts_trace_and_compile():
+    """jnp.sort / jnp.argsort (stable) of float32 and int32 per-agent
+    vectors: values of the input's dtype, argsort int32 (JAX x64 off),
+    torch.sort's indices int64; an unstable or descending sort is refused."""
+    st = {"p": torch.rand(N, 16), "q": torch.randint(-3, 4, (N, 5), dtype=torch.int32)}
+
+    def upd(s, m, key):
+        return {"s": jnp.sort(s["p"]), "a": jnp.argsort(s["p"]), "i": jnp.sort(s["q"]),
+                "t": torch.sort(s["q"], stable=True).indices}
+
+    name, src, tr, outs = _trace(upd, st, {"env": {}})
+    assert {kk: (vv.dtype, tuple(vv.shape)) for kk, vv in outs.items()} == {
+        "s": (torch.float32, (N, 16)), "a": (torch.int32, (N, 16)), "i": (torch.int32, (N, 5)),
+        "t": (torch.int64, (N, 5))}
+    assert "abm_sort_key" in src
+    fuse.compile_source(src)
+    for bad in (lambda s, m, k: {"s": torch.sort(s["p"]).values},
+                lambda s, m, k: {"s": torch.sort(s["p"], stable=True, descending=True).values}):
+        with pytest.raises(fuse.Unsupported):
+            _trace(bad, st, {"env": {}})
+
+
 def test_walker_with_vectors_traces_and_compiles():
     """A bouncing walker in the style of the reference's basic example
     (examples/basic_example.py:20-70): 2-vectors of position and velocity,

@@ -177,8 +177,8 @@ def test_fused_equals_vmapped(J, agent, n, mode):
 
 
 def test_refused_function_runs_vmapped(J):
-    """A sort of a per-agent vector is not covered: the update runs the
-    vmapped path (and the refusal is remembered); the init is fused."""
+    """A difference of a per-agent vector is not covered: the update runs
+    the vmapped path (and the refusal is remembered); the init is fused."""
     from jaxabm_amd import fuse
     from jaxabm_amd import random as jr
 
@@ -189,7 +189,7 @@ def test_refused_function_runs_vmapped(J):
         def update(self, s, ms, cfg, key):
             from jaxabm_amd import numpy as jnp
 
-            return {"p": jnp.sort(s["p"]) * 2.0}
+            return {"p": s["p"] + jnp.diff(s["p"])}
 
     before = fuse.stats()["launches"]
     c = J.AgentCollection(Vec(), 50)
@@ -198,10 +198,10 @@ def test_refused_function_runs_vmapped(J):
     c.update({"env": {}}, jr.PRNGKey(2), J.ModelConfig())
     assert c.states["p"].shape == (50, 2)
     assert fuse.stats()["launches"] == before + 1
-    assert any("sort" in r for r in fuse.refused().values())
+    assert any("diff" in r for r in fuse.refused().values())
     import torch
 
-    assert torch.equal(c.states["p"], torch.sort(p0, dim=1, stable=True).values * 2.0)
+    assert torch.equal(c.states["p"], p0 + (p0[:, 1:] - p0[:, :1]))
 
 
 OPS = {
@@ -486,6 +486,119 @@ def test_dot_products_fused_equal_vmapped(J, k):
     for kk in a:
         assert a[kk].dtype == b[kk].dtype, kk
         np.testing.assert_array_equal(bits(a[kk]), bits(b[kk]), err_msg=kk)
+
+
+@pytest.mark.parametrize("k", list(range(1, 17)))
+def test_cumulative_sums_fused_equal_vmapped(J, k):
+    """jnp.cumsum / torch.cumsum of per-agent float32 vectors (torch's scan
+    order) with signed zeros, infinities and NaN, and of int32 and boolean
+    vectors (int32 as JAX with x64 off), fused and vmapped alike."""
+    import torch
+
+    from jaxabm_amd import fuse
+    from jaxabm_amd import numpy as jnp
+    from jaxabm_amd import random as jr
+
+    g = torch.Generator(device="cuda").manual_seed(200 + k)
+    n = 30_000
+    p = torch.randn(n, k, device="cuda", generator=g) * torch.exp2(
+        torch.randint(-10, 11, (n, k), device="cuda", generator=g).float())
+    z = torch.rand(n, k, device="cuda", generator=g)
+    p[z < 0.1] = 0.0
+    p[(z >= 0.1) & (z < 0.25)] = -0.0
+    p[(z >= 0.25) & (z < 0.27)] = float("inf")
+    p[(z >= 0.27) & (z < 0.28)] = float("nan")
+    iq = torch.randint(-2**30, 2**30, (n, k), device="cuda", generator=g, dtype=torch.int32)
+
+    class Scan:
+        def init_state(self, cfg, key):
+            return {"x": 0.0}
+
+        def update(self, s, ms, cfg, key):
+            v = s["p"]
+            return {"c": jnp.cumsum(v), "t": torch.cumsum(v * 0.5, dim=0),
+                    "i": jnp.cumsum(s["iq"]), "b": jnp.cumsum(v > 0.0),
+                    "l": jnp.cumsum(v)[k - 1] - jnp.sum(v)}
+
+    res = []
+    for fused in (False, True):
+        fuse.enable(fused)
+        try:
+            before = fuse.stats()["launches"]
+            c = J.AgentCollection(Scan(), n)
+            c.model_config = J.ModelConfig()
+            c._states = {"p": p.clone(), "iq": iq.clone()}
+            c.update({"env": {}}, jr.PRNGKey(1), J.ModelConfig())
+            assert fuse.stats()["launches"] - before == (1 if fused else 0), fuse.refused()
+            res.append({kk: vv.clone() for kk, vv in c.states.items()})
+        finally:
+            fuse.enable(True)
+    a, b = res
+    assert a["i"].dtype == torch.int32 and a["b"].dtype == torch.int32
+    for kk in a:
+        assert a[kk].dtype == b[kk].dtype, kk
+        np.testing.assert_array_equal(bits(a[kk]), bits(b[kk]), err_msg=kk)
+
+
+@pytest.mark.parametrize("k", list(range(1, 17)))
+def test_sorts_fused_equal_vmapped(J, k):
+    """jnp.sort / jnp.argsort / torch.sort(stable=True) of per-agent float32
+    vectors full of ties, signed zeros, infinities and NaNs of both signs
+    (torch's radix key order), and of int32 and boolean vectors, fused and
+    vmapped alike: values bit for bit and indices."""
+    import torch
+
+    from jaxabm_amd import fuse
+    from jaxabm_amd import numpy as jnp
+    from jaxabm_amd import random as jr
+
+    g = torch.Generator(device="cuda").manual_seed(300 + k)
+    n = 30_000
+    pool = torch.tensor([-1.0, -0.0, 0.0, 1.0, float("inf"), -float("inf"), 2.0, 0.5],
+                        device="cuda")
+    p = pool[torch.randint(0, 8, (n, k), device="cuda", generator=g)]
+    z = torch.rand(n, k, device="cuda", generator=g)
+    neg_nan = torch.tensor([-0x400000], dtype=torch.int32, device="cuda").view(torch.float32)
+    p[z < 0.05] = float("nan")
+    p[(z >= 0.05) & (z < 0.1)] = neg_nan
+    p[z >= 0.9] = torch.randn(int((z >= 0.9).sum()), device="cuda", generator=g)
+    iq = torch.randint(-3, 4, (n, k), device="cuda", generator=g, dtype=torch.int32)
+
+    class Sorter:
+        def init_state(self, cfg, key):
+            return {"x": 0.0}
+
+        def update(self, s, ms, cfg, key):
+            v = s["p"]
+            return {"s": jnp.sort(v), "a": jnp.argsort(v), "i": jnp.sort(s["iq"]),
+                    "ia": jnp.argsort(s["iq"]), "b": jnp.argsort(v > 0.0),
+                    "t": torch.sort(v * 2.0, dim=-1, stable=True).indices,
+                    "lo": jnp.sort(v)[0] + jnp.sort(v)[k - 1]}
+
+    res = []
+    for fused in (False, True):
+        fuse.enable(fused)
+        try:
+            before = fuse.stats()["launches"]
+            c = J.AgentCollection(Sorter(), n)
+            c.model_config = J.ModelConfig()
+            c._states = {"p": p.clone(), "iq": iq.clone()}
+            c.update({"env": {}}, jr.PRNGKey(1), J.ModelConfig())
+            assert fuse.stats()["launches"] - before == (1 if fused else 0), fuse.refused()
+            res.append({kk: vv.clone() for kk, vv in c.states.items()})
+        finally:
+            fuse.enable(True)
+    a, b = res
+    assert a["a"].dtype == torch.int32 and a["t"].dtype == torch.int64
+    for kk in a:
+        assert a[kk].dtype == b[kk].dtype, kk
+        if kk == "lo":                     # arithmetic: NaN as NaN
+            np.testing.assert_array_equal(bits(a[kk]), bits(b[kk]), err_msg=kk)
+        elif a[kk].dtype == torch.float32:  # sorted values keep their bits, NaN signs included
+            np.testing.assert_array_equal(a[kk].view(torch.int32).cpu().numpy(),
+                                          b[kk].view(torch.int32).cpu().numpy(), err_msg=kk)
+        else:
+            np.testing.assert_array_equal(a[kk].cpu().numpy(), b[kk].cpu().numpy(), err_msg=kk)
 
 
 def test_vector_at_updates_and_bool_subtraction_fused_equal_vmapped(J):
