@@ -485,6 +485,27 @@ int abm_sum_i32(const int32_t *x, int64_t n, int32_t *out, void *ws, size_t ws_b
                 void *stream);
 int abm_sum_u8(const uint8_t *x, int64_t n, int32_t *out, void *ws, size_t ws_bytes,
                void *stream);
+/* Several of the sums and means above in one launch, each of up to 65,536
+ * values and computed exactly as its own call computes it (same order, same
+ * rounding): the reductions of one step's env and metrics callbacks
+ * (jaxabm/model.py:182-200 update_state_fn / metrics_fn, jaxabm/agentpy.py:
+ * 926-942 compute_metrics), batched.  op: ABM_REDUCE_SUM_F32 (float *out),
+ * ABM_REDUCE_MEAN_F32, ABM_REDUCE_SUM_I32 (int32_t *out), ABM_REDUCE_MEAN_I32
+ * (float), ABM_REDUCE_SUM_U8 (int32_t).  1 <= count <= ABM_REDUCE_BATCH_MAX. */
+#define ABM_REDUCE_BATCH_MAX 16
+#define ABM_REDUCE_SUM_F32 0
+#define ABM_REDUCE_MEAN_F32 1
+#define ABM_REDUCE_SUM_I32 2
+#define ABM_REDUCE_MEAN_I32 3
+#define ABM_REDUCE_SUM_U8 4
+typedef struct abm_reduce_item {
+  const void *x;
+  int64_t n;
+  void *out;
+  int32_t op;
+  int32_t reserved;
+} abm_reduce_item;
+int abm_reduce_batch(const abm_reduce_item *items, int32_t count, void *stream);
 /* *out = Gini of int32 values: sorted x, S = sum (i+1) x_(i), T = sum x in
  * int32 (wrapping), G = f32(2S) / f32(nT) - f32((n+1)/n), 0 when T <= 1e-6. */
 int abm_gini_i32(const int32_t *x, int64_t n, float *out, void *ws, size_t ws_bytes,

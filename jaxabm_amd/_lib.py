@@ -25,6 +25,15 @@ class CopyItem(C.Structure):
     _fields_ = [("src", C.c_void_p), ("dst", C.c_void_p), ("bytes", C.c_int64),
                 ("row_stride", C.c_int64), ("row_limit", C.c_int64)]
 
+REDUCE_BATCH_MAX = 16              # abm_reduce_batch
+(REDUCE_SUM_F32, REDUCE_MEAN_F32, REDUCE_SUM_I32, REDUCE_MEAN_I32,
+ REDUCE_SUM_U8) = range(5)
+
+
+class ReduceItem(C.Structure):
+    _fields_ = [("x", C.c_void_p), ("n", C.c_int64), ("out", C.c_void_p), ("op", C.c_int32),
+                ("reserved", C.c_int32)]
+
 _lock = threading.Lock()
 _lib = None
 
@@ -209,6 +218,7 @@ SIGNATURES = {
     "abm_sum_f32": (_int, [_p, _i64, _p, _p, _sz, _p]),
     "abm_sum_i32": (_int, [_p, _i64, _p, _p, _sz, _p]),
     "abm_sum_u8": (_int, [_p, _i64, _p, _p, _sz, _p]),
+    "abm_reduce_batch": (_int, [_p, _i32, _p]),
     "abm_soft_hist_f32": (_int, [_p, _i64, _i32, C.c_double, _p, _p, _sz, _p]),
     "abm_gini_i32_noalloc": (_int, [_p, _i64, _p, _p]),
     "abm_rtc_compile": (_int, [C.c_char_p, C.c_char_p, C.POINTER(_p), C.POINTER(C.c_size_t)]),

@@ -96,10 +96,13 @@ __global__ void __launch_bounds__(kRedThreads)
 // (wrapping), 3 float32 mean of int32 (f32(exact sum) / f32(n)).
 constexpr int64_t kSmallMax = 65536;
 
+// One 1024-thread workgroup reduces x[0, n): thread t sums x[t], x[t+1024],
+// ... in Acc, the wavefront sums by xor shuffles, then thread 0 adds the 16
+// wavefront sums in order.  Every whole-array sum and mean of up to kSmallMax
+// values runs exactly this sequence, alone or batched.
 template <typename In, typename Acc, int OUT>
-__global__ void __launch_bounds__(1024)
-    k_reduce_one_block(const In *__restrict__ x, int64_t n, void *__restrict__ out) {
-  __shared__ Acc red[16];
+__device__ __forceinline__ void reduce_one_block(const In *__restrict__ x, int64_t n,
+                                                 void *__restrict__ out, Acc *red) {
   Acc v = 0;
   for (int64_t i = threadIdx.x; i < n; i += 1024) v += (Acc)x[i];
 #pragma unroll
@@ -113,6 +116,42 @@ __global__ void __launch_bounds__(1024)
     else if constexpr (OUT == 1) *(float *)out = (float)((double)s / (double)n);
     else if constexpr (OUT == 2) *(int32_t *)out = (int32_t)(uint32_t)(uint64_t)s;
     else *(float *)out = (float)s / (float)n;
+  }
+}
+
+template <typename In, typename Acc, int OUT>
+__global__ void __launch_bounds__(1024)
+    k_reduce_one_block(const In *__restrict__ x, int64_t n, void *__restrict__ out) {
+  __shared__ Acc red[16];
+  reduce_one_block<In, Acc, OUT>(x, n, out, red);
+}
+
+// Several whole-array reductions in one launch: workgroup b runs item b
+// (items travel by value, so a captured launch keeps them).
+struct ReduceBatch {
+  abm_reduce_item items[ABM_REDUCE_BATCH_MAX];
+};
+
+__global__ void __launch_bounds__(1024) k_reduce_batch(ReduceBatch batch) {
+  __shared__ double redf[16];
+  __shared__ int64_t redi[16];
+  const abm_reduce_item it = batch.items[blockIdx.x];
+  switch (it.op) {  // block-uniform
+    case ABM_REDUCE_SUM_F32:
+      reduce_one_block<float, double, 0>((const float *)it.x, it.n, it.out, redf);
+      break;
+    case ABM_REDUCE_MEAN_F32:
+      reduce_one_block<float, double, 1>((const float *)it.x, it.n, it.out, redf);
+      break;
+    case ABM_REDUCE_SUM_I32:
+      reduce_one_block<int32_t, int64_t, 2>((const int32_t *)it.x, it.n, it.out, redi);
+      break;
+    case ABM_REDUCE_MEAN_I32:
+      reduce_one_block<int32_t, int64_t, 3>((const int32_t *)it.x, it.n, it.out, redi);
+      break;
+    default:
+      reduce_one_block<uint8_t, int64_t, 2>((const uint8_t *)it.x, it.n, it.out, redi);
+      break;
   }
 }
 
@@ -354,6 +393,23 @@ int abm_mean_i32(const int32_t *x, int64_t n, float *out, void *ws, size_t ws_by
   hipLaunchKernelGGL(k_mean_i32_final, dim3(1), dim3(1), 0, s, (const int64_t *)ws, (int)nb, n,
                      out);
   ABM_LAUNCH_CHECK("abm_mean_i32");
+  return kOk;
+}
+
+int abm_reduce_batch(const abm_reduce_item *items, int32_t count, void *stream) {
+  ABM_REQUIRE(items && count >= 1 && count <= ABM_REDUCE_BATCH_MAX,
+              "abm_reduce_batch: bad arguments (1 <= count <= ABM_REDUCE_BATCH_MAX)");
+  ReduceBatch batch;
+  for (int32_t i = 0; i < count; ++i) {
+    const abm_reduce_item &it = items[i];
+    ABM_REQUIRE(it.x && it.out && it.n >= 1 && it.n <= kSmallMax && it.op >= ABM_REDUCE_SUM_F32 &&
+                    it.op <= ABM_REDUCE_SUM_U8,
+                "abm_reduce_batch: bad item (1 <= n <= 65536, a known op)");
+    batch.items[i] = it;
+  }
+  hipLaunchKernelGGL(k_reduce_batch, dim3((unsigned)count), dim3(1024), 0, (hipStream_t)stream,
+                     batch);
+  ABM_LAUNCH_CHECK("abm_reduce_batch");
   return kOk;
 }
 

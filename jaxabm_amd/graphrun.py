@@ -44,6 +44,7 @@ import numpy as np
 import torch
 
 from . import random as jr
+from . import reductions
 from ._lib import MAX_COPY_ITEMS, CopyItem, check, lib, stream_handle
 
 _SCALARS = (bool, int, float)
@@ -112,13 +113,17 @@ class _Runner:
             c.update(current, keys[i], self.m.config)
         updated = {name: c.states for name, c in self.colls}
         env_new = dict(self.env)
-        if self.has_env:
-            env_new = self.m._update_state_fn(dict(self.env), updated, self.m._params,
-                                              keys[len(self.colls)])
-            if not isinstance(env_new, dict):
-                raise NotReplayable("update_state_fn did not return a dict")
-        metrics = self.m._metrics_fn(env_new, updated, self.m._params) if self.m._metrics_fn else {}
-        return updated, env_new, dict(metrics)
+        # the callbacks' whole-array sums and means go out as one launch
+        with reductions.deferring():
+            if self.has_env:
+                env_new = self.m._update_state_fn(dict(self.env), updated, self.m._params,
+                                                  keys[len(self.colls)])
+                if not isinstance(env_new, dict):
+                    raise NotReplayable("update_state_fn did not return a dict")
+            metrics = (self.m._metrics_fn(env_new, updated, self.m._params)
+                       if self.m._metrics_fn else {})
+        return (updated, reductions.materialize(env_new),
+                dict(reductions.materialize(metrics)))
 
     def _check(self, updated, env_new, metrics, first: bool):
         for name, _c in self.colls:

@@ -22,6 +22,7 @@ import torch
 
 from . import numpy as jnp
 from . import random as jr
+from . import reductions
 from .agent import AgentCollection, _fn_id, vmap_agents
 from .core import ModelConfig
 
@@ -92,13 +93,17 @@ class Model:
             step_key, coll_key = jr.host_split(step_key, 2)
             coll.update(current, _dev_key(coll_key), self.config)
         updated = {name: c.states for name, c in self._agent_collections.items()}
-        if self._update_state_fn:
-            step_key, update_key = jr.host_split(step_key, 2)
-            self._env_state = self._update_state_fn(self._env_state, updated, self._params,
-                                                    _dev_key(update_key))
         metrics = {}
-        if self._metrics_fn:
-            metrics = self._metrics_fn(self._env_state, updated, self._params)
+        # the callbacks' whole-array sums and means go out as one launch
+        with reductions.deferring():
+            if self._update_state_fn:
+                step_key, update_key = jr.host_split(step_key, 2)
+                self._env_state = self._update_state_fn(self._env_state, updated, self._params,
+                                                        _dev_key(update_key))
+            if self._metrics_fn:
+                metrics = self._metrics_fn(self._env_state, updated, self._params)
+        self._env_state = reductions.materialize(self._env_state)
+        metrics = reductions.materialize(metrics)
         self._time_step += 1
         if self.config.track_history and self._time_step % self.config.collect_interval == 0:
             self._history.append({"time_step": self._time_step, "metrics": metrics})
@@ -164,11 +169,13 @@ class Model:
                 updated[name] = vmap_agents(
                     lambda s, k: upd(s, model_state, self.config, k), states, jr.split(sub, n), n=n,
                     ident=_fn_id(upd))
-            if self._update_state_fn:
-                key, uk = jr.split(key)
-                env_state = self._update_state_fn(env_state, updated, params, uk)
-            metrics = self._metrics_fn(env_state, updated, params) if self._metrics_fn else {}
-            return env_state, updated, metrics, key
+            with reductions.deferring():
+                if self._update_state_fn:
+                    key, uk = jr.split(key)
+                    env_state = self._update_state_fn(env_state, updated, params, uk)
+                metrics = self._metrics_fn(env_state, updated, params) if self._metrics_fn else {}
+            return (reductions.materialize(env_state), updated, reductions.materialize(metrics),
+                    key)
 
         return _step_fn
 

@@ -17,7 +17,11 @@ replayed as a graph (``Model.run(graph=True)``).
   order is XLA's, so float means agree to rounding).
 * ``sum_all(x)`` / ``mean_all(x)``: the whole-array jnp.sum / jnp.mean that
   ``jaxabm_amd.numpy`` runs on plain GPU tensors (env and metrics callbacks):
-  one launch up to 65,536 values.
+  one launch up to 65,536 values.  Inside ``deferring()`` (a model step's
+  env and metrics callbacks) those launches wait and go out together, one
+  ``abm_reduce_batch`` launch, when the scope ends or when anything reads
+  one of their results, whichever comes first; each value is computed
+  exactly as its own launch computes it.
 * ``gini(x)``: int32 input, the reference's formula bit for bit (int32 sums
   with wrap-around, float32 quotient).
 * ``soft_histogram(x, bins, slope)``: float64 bin means of sigmoid
@@ -25,9 +29,14 @@ replayed as a graph (``Model.run(graph=True)``).
 """
 from __future__ import annotations
 
+import contextlib
+import threading
+
 import torch
 
-from ._lib import check, lib, ptr, require_gpu, stream_handle
+from ._lib import (REDUCE_BATCH_MAX, REDUCE_MEAN_F32, REDUCE_MEAN_I32, REDUCE_SUM_F32,
+                   REDUCE_SUM_I32, REDUCE_SUM_U8, ReduceItem, check, lib, ptr, require_gpu,
+                   stream_handle)
 
 
 
@@ -76,13 +85,100 @@ def _fn(name: str):
     return f
 
 
+# ---- batched launches of the step's whole-array reductions
+_tls = threading.local()
+_OPS = {"abm_sum_f32": REDUCE_SUM_F32, "abm_mean_f32": REDUCE_MEAN_F32,
+        "abm_sum_i32": REDUCE_SUM_I32, "abm_mean_i32": REDUCE_MEAN_I32,
+        "abm_sum_u8": REDUCE_SUM_U8}
+_METADATA = {torch.Tensor.dtype.__get__, torch.Tensor.shape.__get__,
+             torch.Tensor.device.__get__, torch.Tensor.is_cuda.__get__,
+             torch.Tensor.requires_grad.__get__, torch.Tensor.ndim.__get__, torch.Tensor.dim,
+             torch.Tensor.numel, torch.Tensor.size, torch.Tensor.is_floating_point}
+
+
+def _plain(obj):
+    if isinstance(obj, _Deferred):
+        return obj._abm_plain
+    if isinstance(obj, (list, tuple)):
+        return type(obj)(_plain(o) for o in obj)
+    if isinstance(obj, dict):
+        return {k: _plain(v) for k, v in obj.items()}
+    return obj
+
+
+class _Deferred(torch.Tensor):
+    """The 0-d result of a whole-array sum or mean whose launch waits for
+    the rest of the step's reductions (``deferring()``).  Any torch operation
+    on it launches the waiting batch first and then runs on the plain
+    tensor; reading its dtype, shape or device does not."""
+
+    @classmethod
+    def __torch_function__(cls, func, types, args=(), kwargs=None):
+        if func not in _METADATA:
+            flush()
+        return func(*_plain(args), **_plain(kwargs or {}))
+
+
+def _pending() -> list:
+    p = getattr(_tls, "pending", None)
+    if p is None:
+        p = _tls.pending = []
+    return p
+
+
+def flush() -> None:
+    """Launch the waiting reductions (abm_reduce_batch, up to 16 a launch)."""
+    pend = _pending()
+    if not pend:
+        return
+    _tls.pending = []
+    stream = pend[0][3]
+    for i in range(0, len(pend), REDUCE_BATCH_MAX):
+        part = pend[i:i + REDUCE_BATCH_MAX]
+        items = (ReduceItem * len(part))(*[ReduceItem(v.data_ptr(), v.numel(), out.data_ptr(),
+                                                      op, 0) for op, v, out, _s in part])
+        check(_fn("abm_reduce_batch")(items, len(part), stream), "reduce_batch")
+
+
+@contextlib.contextmanager
+def deferring():
+    """Batch the whole-array sums and means issued inside into one launch
+    (at the end of the outermost scope, or earlier when a result is read)."""
+    depth = getattr(_tls, "depth", 0)
+    _tls.depth = depth + 1
+    try:
+        yield
+    finally:
+        _tls.depth = depth
+        if depth == 0:
+            flush()
+
+
+def materialize(d):
+    """``d`` (a dict of callback results) with deferred results replaced by
+    plain tensors, the waiting reductions launched first."""
+    flush()
+    if isinstance(d, dict) and any(isinstance(v, _Deferred) for v in d.values()):
+        return {k: (v._abm_plain if isinstance(v, _Deferred) else v) for k, v in d.items()}
+    return d
+
+
 def _whole_call(name: str, v: torch.Tensor, out_dtype) -> torch.Tensor:
     n = v.numel()
     dev = v.device
     out = torch.empty((), dtype=out_dtype, device=dev)
-    ws = None if n <= _SUM_SMALL else _workspace(n, dev)
     stream = torch._C._cuda_getCurrentRawStream(dev.index if dev.index is not None
                                                 else torch.cuda.current_device())
+    if n <= _SUM_SMALL and getattr(_tls, "depth", 0):
+        pend = _pending()
+        if pend and pend[0][3] != stream:
+            flush()
+        # the input stays referenced until its launch is enqueued
+        _pending().append((_OPS[name], v, out, stream))
+        d = torch.Tensor._make_subclass(_Deferred, out)
+        d._abm_plain = out
+        return d
+    ws = None if n <= _SUM_SMALL else _workspace(n, dev)
     rc = _fn(name)(v.data_ptr(), n, out.data_ptr(), None if ws is None else ws.data_ptr(),
                    0 if ws is None else ws.numel(), stream)
     if rc:

@@ -144,6 +144,85 @@ def test_whole_array_sums_and_means(R, n):
     assert reductions.sum_all(ti).item() == jnp.sum(ti).item()
 
 
+def test_reduce_batch_equals_single_launches(R):
+    """abm_reduce_batch through the C ABI: every op at 1, 1,000 and 65,536
+    values, 15 items in one launch, each result bit-identical to its own
+    single-reduction call; bad items are refused."""
+    import ctypes as C
+
+    import torch
+
+    from jaxabm_amd._lib import (REDUCE_MEAN_F32, REDUCE_MEAN_I32, REDUCE_SUM_F32,
+                                 REDUCE_SUM_I32, REDUCE_SUM_U8, AbmError, ReduceItem, check, lib)
+
+    rs = np.random.default_rng(3)
+    single = {REDUCE_SUM_F32: ("abm_sum_f32", torch.float32),
+              REDUCE_MEAN_F32: ("abm_mean_f32", torch.float32),
+              REDUCE_SUM_I32: ("abm_sum_i32", torch.int32),
+              REDUCE_MEAN_I32: ("abm_mean_i32", torch.float32),
+              REDUCE_SUM_U8: ("abm_sum_u8", torch.int32)}
+    items, keep, want = [], [], []
+    for op, (name, odt) in single.items():
+        for n in (1, 1000, 65_536):
+            if op in (REDUCE_SUM_F32, REDUCE_MEAN_F32):
+                x = torch.from_numpy((rs.standard_normal(n) * 2.0 ** rs.integers(-20, 20, n))
+                                     .astype(np.float32)).cuda()
+            elif op == REDUCE_SUM_U8:
+                x = torch.from_numpy(rs.integers(0, 256, n).astype(np.uint8)).cuda()
+            else:
+                x = torch.from_numpy(rs.integers(-2**31, 2**31 - 1, n).astype(np.int32)).cuda()
+            out = torch.full((), -7, dtype=odt, device="cuda")
+            ref = torch.full((), -9, dtype=odt, device="cuda")
+            check(getattr(lib(), name)(x.data_ptr(), n, ref.data_ptr(), None, 0, None), name)
+            items.append(ReduceItem(x.data_ptr(), n, out.data_ptr(), op, 0))
+            keep.append((x, out))
+            want.append(ref)
+    arr = (ReduceItem * len(items))(*items)
+    check(lib().abm_reduce_batch(arr, len(items), None), "reduce_batch")
+    torch.cuda.synchronize()
+    for (x, out), ref in zip(keep, want):
+        assert out.view(torch.int32).item() == ref.view(torch.int32).item(), (x.dtype, x.numel())
+    bad = (ReduceItem * 1)(ReduceItem(keep[0][0].data_ptr(), 65_537, keep[0][1].data_ptr(),
+                                      REDUCE_SUM_F32, 0))
+    with pytest.raises(AbmError, match="bad item"):
+        check(lib().abm_reduce_batch(bad, 1, None), "reduce_batch")
+    with pytest.raises(AbmError, match="count"):
+        check(lib().abm_reduce_batch(arr, 17, None), "reduce_batch")
+
+
+def test_deferred_callback_reductions(R):
+    """Inside reductions.deferring() (a step's env and metrics callbacks)
+    whole-array sums and means wait and go out as one launch: reading a
+    result launches the waiting batch first; values equal the immediate
+    launches bit for bit; more than 16 waiting reductions take several
+    launches; materialize() hands back plain tensors."""
+    import torch
+
+    from jaxabm_amd import numpy as jnp
+    from jaxabm_amd import reductions
+
+    g = torch.Generator(device="cuda").manual_seed(5)
+    xs = [torch.randn(1000 + 37 * i, device="cuda", generator=g) * 1e3 for i in range(20)]
+    ws = torch.randint(-1000, 1000, (4096,), device="cuda", generator=g, dtype=torch.int32)
+    now = [jnp.mean(x) for x in xs] + [jnp.sum(ws), jnp.sum(ws > 3), jnp.mean(ws)]
+    with reductions.deferring():
+        later = [jnp.mean(x) for x in xs] + [jnp.sum(ws), jnp.sum(ws > 3), jnp.mean(ws)]
+        assert all(isinstance(v, reductions._Deferred) for v in later)
+        assert later[0].dtype == torch.float32 and later[-3].dtype == torch.int32
+        assert len(reductions._pending()) == 23        # metadata reads launch nothing
+        used = later[1] * 2.0 + 1.0                     # a read launches the batch
+        assert not reductions._pending() and type(used) is torch.Tensor
+        tail = jnp.sum(xs[0])
+        assert len(reductions._pending()) == 1
+        d = reductions.materialize({"a": later[0], "b": 3, "c": tail})
+        assert type(d["a"]) is torch.Tensor and d["b"] == 3 and not reductions._pending()
+    assert not reductions._pending()
+    for a, b in zip(now, later):
+        assert a.view(torch.int32).item() == b._abm_plain.view(torch.int32).item()
+    assert used.item() == (now[1] * 2.0 + 1.0).item()
+    assert d["c"].item() == jnp.sum(xs[0]).item()
+
+
 def test_reduction_routing_rules(R):
     """Axis reductions, keepdims, float64 and tensors that record gradients
     stay torch reductions (with JAX's x64-off integer dtypes)."""
