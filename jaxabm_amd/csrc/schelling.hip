@@ -233,7 +233,17 @@ struct SchellingStep {
   const uint32_t *ck;
   int mode;
   int32_t *happy_count;
+  // the previous step's happy count (null at a run's first step): when every
+  // agent was happy then, nobody moved, the grid is unchanged and this step
+  // is the identity
+  const int32_t *prev_happy;
 };
+
+// True when the step is the identity: the previous step found every agent
+// happy, so nothing moved since and every agent is happy again.
+__device__ __forceinline__ bool schelling_identity(const int32_t *prev_happy, int64_t n_agents) {
+  return prev_happy && (int64_t)*prev_happy == n_agents;
+}
 
 __global__ void __launch_bounds__(kTileDim * kTileDim) k_schelling_agents(SchellingStep a) {
   extern __shared__ int32_t empty_prefix_lds[];  // [nb + 1]
@@ -242,6 +252,12 @@ __global__ void __launch_bounds__(kTileDim * kTileDim) k_schelling_agents(Schell
   __shared__ int32_t carry;
   constexpr int kHalo = (kTileDim + 2) * (kTileDim + 2);  // 34 x 34 cells, <= 2 per thread
   const int tid = threadIdx.x;
+  if (schelling_identity(a.prev_happy, a.n_agents)) {  // grid-uniform
+    // every agent happy again; no target is drawn (the last step left each
+    // agent's target at -1) and the moves kernel exits too
+    if (blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) *a.happy_count = (int32_t)a.n_agents;
+    return;
+  }
   const int tx = tid % kTileDim, ty = tid / kTileDim;
   const int r0 = blockIdx.y * kTileDim, c0 = blockIdx.x * kTileDim;
   const int r = r0 + ty, c = c0 + tx;
@@ -306,12 +322,18 @@ __global__ void __launch_bounds__(kTileDim * kTileDim) k_schelling_agents(Schell
   }
 }
 
-__global__ void k_schelling_moves(int64_t n_agents, int8_t *__restrict__ cells,
+// one thread per agent; large workgroups keep the grid small, which is most
+// of the cost of a step that only checks the identity condition
+constexpr int kMovesThreads = 1024;
+
+__global__ void __launch_bounds__(kMovesThreads) k_schelling_moves(int64_t n_agents, int8_t *__restrict__ cells,
                                   int32_t *__restrict__ id, int32_t *__restrict__ claim,
                                   int32_t *__restrict__ pos, const int8_t *__restrict__ type,
                                   const int32_t *__restrict__ target, uint32_t *__restrict__ ebits,
                                   int32_t *__restrict__ ecount, int32_t nb,
-                                  int32_t *__restrict__ moves) {
+                                  int32_t *__restrict__ moves,
+                                  const int32_t *__restrict__ prev_happy) {
+  if (schelling_identity(prev_happy, n_agents)) return;  // grid-uniform: no move
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool moved = false;
   if (k < n_agents) {
@@ -337,12 +359,13 @@ __global__ void k_schelling_moves(int64_t n_agents, int8_t *__restrict__ cells,
       moved = true;
     }
   }
-  __shared__ int32_t wc[4];
+  __shared__ int32_t wc[kMovesThreads / 64];
   const uint64_t m = __ballot(moved);
   if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = (int32_t)__popcll(m);
   __syncthreads();
   if (threadIdx.x == 0) {
-    const int32_t s = wc[0] + wc[1] + wc[2] + wc[3];
+    int32_t s = 0;
+    for (int w = 0; w < kMovesThreads / 64; ++w) s += wc[w];
     if (s) atomicAdd(moves, s);
   }
 }
@@ -480,11 +503,14 @@ int abm_schelling_run(const abm_schelling_grid *g, const uint32_t *keys, int32_t
     a.ck = keys + 2 * (int64_t)t;
     a.mode = mode;
     a.happy_count = counters + 2 * (int64_t)t;
+    a.prev_happy = t ? counters + 2 * (int64_t)(t - 1) : nullptr;
     hipLaunchKernelGGL(k_schelling_agents, tiles, dim3(kTileDim * kTileDim), prefix_lds, s, a);
     if (g->n_agents > 0)
-      hipLaunchKernelGGL(k_schelling_moves, dim3(nblk(g->n_agents, 256)), dim3(256), 0, s,
+      hipLaunchKernelGGL(k_schelling_moves, dim3(nblk(g->n_agents, kMovesThreads)),
+                         dim3(kMovesThreads), 0, s,
                          g->n_agents, g->cells, g->id, g->claim, g->pos, g->type, g->target,
-                         ebits, g->block_counts, (int32_t)nb, counters + 2 * (int64_t)t + 1);
+                         ebits, g->block_counts, (int32_t)nb, counters + 2 * (int64_t)t + 1,
+                         a.prev_happy);
   }
   hipLaunchKernelGGL(k_schelling_record, dim3(nblk(steps, 256)), dim3(256), 0, s, counters,
                      steps, g->n_agents > 0 ? g->n_agents : 1, happy_frac, moves);
