@@ -648,6 +648,24 @@ def serving_summary(n_requests: int) -> dict:
     }
 
 
+def bench_serving(args) -> dict:
+    """The batched-serving line alone: the seeded trace of ``--serving-requests``
+    requests, saturated and at 200 requests/s, against the server at
+    max_active 1 and the plain engines."""
+    srv = serving_summary(args.serving_requests)
+    return {
+        "metric": "sims/s (batched serving of a seeded SIR + wealth request trace)",
+        "value": srv["sims_per_s"], "unit": "sims/s", "n_gpus": 1,
+        "steps": srv["requests"], "warmup": 0, "ms_per_step": 1e3 / srv["sims_per_s"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8+i32",
+        "data": "synthetic (seeded request trace)",
+        "config": {"workload": f"serving_trace_{srv['requests']}requests",
+                   "trace_checksum": srv["trace_checksum"],
+                   "bench_step": "one simulation request"},
+        "serving": srv,
+    }
+
+
 def sir_fixture() -> dict:
     """The committed checksum of the measured graph and the oracle's S/I/R
     rows of the measured run (tests/golden/sir_ba1M_m4_365steps.json)."""
@@ -870,7 +888,7 @@ def main() -> int:
     ap.add_argument("--serving-requests", type=int, default=400,
                     help="requests in the serving trace (0 = skip the serving measurement)")
     ap.add_argument("--workload", default="sir",
-                    choices=["sir", "deffuant", "wealth", "schelling", "generic"],
+                    choices=["sir", "deffuant", "wealth", "schelling", "generic", "serving"],
                     help="sir (default): the headline line, with the wealth, Schelling and "
                          "Deffuant programs embedded; the others print one program's line")
     ap.add_argument("--profile-run", action="store_true",
@@ -895,7 +913,8 @@ def main() -> int:
         return 2
     if args.workload != "sir":
         fn = {"deffuant": bench_deffuant, "wealth": bench_wealth,
-              "schelling": bench_schelling, "generic": bench_generic}[args.workload]
+              "schelling": bench_schelling, "generic": bench_generic,
+              "serving": bench_serving}[args.workload]
         print(json.dumps(fn(args)))
         return 0
     try:
