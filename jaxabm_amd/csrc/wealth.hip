@@ -187,28 +187,18 @@ __device__ __forceinline__ void wealth_steps(const abm_wealth_inst &in, float *g
 // of step t is taken by wave 0 alone from histogram t & 1 while the other
 // waves go on to step t + 1; that histogram's bins are cleared by their
 // owners after the next step's first barrier, when wave 0 has read them.
-__device__ __forceinline__ void wealth_steps_small(const abm_wealth_inst &in, int32_t *linc,
-                                                   int32_t *lhist2, uint32_t *red_c,
-                                                   uint32_t *red_t, uint32_t *red_s) {
-  (void)red_c;
-  (void)red_s;
+__device__ __forceinline__ void wealth_steps_small(const abm_wealth_inst &in, int32_t *linc2,
+                                                   int32_t *lhist2, uint32_t *red_t) {
   const int64_t n = in.n;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const bool agent = tid < n;
   float x = agent ? in.x[tid] : 0.0f, y = agent ? in.y[tid] : 0.0f;
   int32_t w = agent ? in.w[tid] : 0;
   if (tid <= n) {
-    linc[tid] = 0;
+    linc2[tid] = 0;
+    linc2[kWealthThreads + tid] = 0;
     lhist2[tid] = 0;
     lhist2[kWealthThreads + tid] = 0;
-  }
-  // the draws of step t + 1 are loaded during step t (barriers wait on LDS
-  // only, so the loads overlap the histogram phases)
-  uint32_t dx = 0u, dy = 0u, dt = 0u;
-  if (agent && in.steps > 0) {
-    dx = in.draws[tid];
-    dy = in.draws[n + tid];
-    dt = in.draws[2 * n + tid];
   }
   // T: the total wealth, the same at every step
   uint32_t tw = agent ? (uint32_t)w : 0u;
@@ -220,65 +210,85 @@ __device__ __forceinline__ void wealth_steps_small(const abm_wealth_inst &in, in
   for (int q = 0; q < kWealthThreads / 64; ++q) T += red_t[q];
   const float mean = (float)(int32_t)T / (float)n;
   const float cst = (float)((double)(n + 1) / (double)n);
-  for (int32_t t = 0; t < in.steps; ++t) {
-    int32_t *hist = lhist2 + (t & 1) * kWealthThreads;
-    int32_t *prev = lhist2 + ((t + 1) & 1) * kWealthThreads;
+  const int32_t steps = in.steps;
+  // Mean and Gini of step s from its histogram (wave 0), clearing the bins
+  // it reads: rank sum 64 bins a round until all n agents are counted (one
+  // round while every wealth is below 64); the nonzero bins all lie in the
+  // rounds read.
+  auto record = [&](int32_t s) {
+    int32_t *hist = lhist2 + (s & 1) * kWealthThreads;
+    uint32_t sp = 0, counted = 0;
+    for (int64_t b0 = 0; counted < (uint32_t)n && b0 <= n; b0 += 64) {
+      const int64_t wv = b0 + lane;
+      const uint32_t c = wv <= n ? (uint32_t)hist[wv] : 0u;
+      if (wv <= n) hist[wv] = 0;
+      uint32_t inc = c;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = __shfl_up(inc, off);
+        if (lane >= off) inc += v;
+      }
+      const uint32_t before = counted + inc - c;
+      if (c) {
+        const uint64_t ranksum = (uint64_t)c * before + (uint64_t)c * (c + 1) / 2;
+        sp += (uint32_t)wv * (uint32_t)ranksum;
+      }
+      counted += __shfl(inc, 63);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) sp += __shfl_xor(sp, off);
+    if (lane == 0) {
+      in.mean_out[s] = mean;
+      const int32_t Ti = (int32_t)T;
+      float g = 0.0f;
+      if ((float)Ti > 1e-6f) {
+        const int32_t num = (int32_t)(2u * sp);
+        const int32_t den = (int32_t)((uint32_t)n * T);
+        const float q = (float)num / (float)den;
+        g = q - cst;
+      }
+      in.gini_out[s] = g;
+    }
+  };
+  // Step t's agent phase (walk, one exchange unit into linc[t & 1]) runs in
+  // the same phase as step t - 1's exchange (which reads and clears
+  // linc[(t - 1) & 1]), one barrier per step; wave 0 records step t - 1
+  // after the next barrier, while the others go on.  Each agent's own draw
+  // words of step t + 1 are loaded a step ahead.
+  uint32_t dx = 0u, dy = 0u, dt = 0u;
+  if (agent && steps > 0) {
+    dx = in.draws[tid];
+    dy = in.draws[n + tid];
+    dt = in.draws[2 * n + tid];
+  }
+  auto agent_phase = [&](int32_t t) {
     const float mdx = __uint_as_float(dx), mdy = __uint_as_float(dy);
     const uint32_t target = dt;
-    if (agent && t + 1 < in.steps) {
+    if (t + 1 < steps) {
       const uint32_t *row = in.draws + (int64_t)(t + 1) * 3 * n;
       dx = row[tid];
       dy = row[n + tid];
       dt = row[2 * n + tid];
     }
+    x = mod1(x + mdx);
+    y = mod1(y + mdy);
+    if (w > 0) atomicAdd(&linc2[(t & 1) * kWealthThreads + target], 1);
+  };
+  if (agent && steps > 0) agent_phase(0);
+  for (int32_t t = 0; t < steps; ++t) {
+    __syncthreads();  // step t's exchange units are in; step t - 1's histogram is complete
+    if (wave == 0 && t > 0) record(t - 1);
     if (agent) {
-      x = mod1(x + mdx);
-      y = mod1(y + mdy);
-      if (w > 0) atomicAdd(&linc[target], 1);
-    }
-    __syncthreads();
-    if (tid <= n) prev[tid] = 0;  // step t - 1's bins, read by wave 0 before this barrier
-    if (agent) {
+      int32_t *linc = linc2 + (t & 1) * kWealthThreads;
       w = w - (w > 0 ? 1 : 0) + linc[tid];
       linc[tid] = 0;
-      atomicAdd(&hist[w], 1);  // 0 <= w <= n: wealth is conserved
+      atomicAdd(&lhist2[(t & 1) * kWealthThreads + w], 1);  // 0 <= w <= n: conserved
+      if (t + 1 < steps) agent_phase(t + 1);
     }
+  }
+  if (steps > 0) {
     __syncthreads();
-    if (wave == 0) {
-      // rank sum from the histogram, 64 bins a round until all n agents are
-      // counted (one round while every wealth is below 64)
-      uint32_t sp = 0, counted = 0;
-      for (int64_t b0 = 0; counted < (uint32_t)n && b0 <= n; b0 += 64) {
-        const int64_t wv = b0 + lane;
-        const uint32_t c = wv <= n ? (uint32_t)hist[wv] : 0u;
-        uint32_t inc = c;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-          const uint32_t v = __shfl_up(inc, off);
-          if (lane >= off) inc += v;
-        }
-        const uint32_t before = counted + inc - c;
-        if (c) {
-          const uint64_t ranksum = (uint64_t)c * before + (uint64_t)c * (c + 1) / 2;
-          sp += (uint32_t)wv * (uint32_t)ranksum;
-        }
-        counted += __shfl(inc, 63);
-      }
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) sp += __shfl_xor(sp, off);
-      if (lane == 0) {
-        in.mean_out[t] = mean;
-        const int32_t Ti = (int32_t)T;
-        float g = 0.0f;
-        if ((float)Ti > 1e-6f) {
-          const int32_t num = (int32_t)(2u * sp);
-          const int32_t den = (int32_t)((uint32_t)n * T);
-          const float q = (float)num / (float)den;
-          g = q - cst;
-        }
-        in.gini_out[t] = g;
-      }
-    }
+    if (wave == 0) record(steps - 1);
   }
   if (agent) {
     in.x[tid] = x;
@@ -300,14 +310,14 @@ __global__ void __launch_bounds__(kWealthThreads)
   const int64_t n = in.n;
   const int tid = threadIdx.x;
   if (in.draws && n + 1 <= kWealthThreads) {
-    // increments and two histograms: the dynamic LDS (at least 3 x
+    // two increment arrays and two histograms: the dynamic LDS (at least 4 x
     // kWealthThreads words in this launch), or a static array where the
     // launch has none
     if constexpr (kLds) {
-      wealth_steps_small(in, lds_agents, lds_agents + kWealthThreads, red, red_t, red_s);
+      wealth_steps_small(in, lds_agents, lds_agents + 2 * kWealthThreads, red_t);
     } else {
-      __shared__ int32_t small_lds[3 * kWealthThreads];
-      wealth_steps_small(in, small_lds, small_lds + kWealthThreads, red, red_t, red_s);
+      __shared__ int32_t small_lds[4 * kWealthThreads];
+      wealth_steps_small(in, small_lds, small_lds + 2 * kWealthThreads, red_t);
     }
   } else if (kLds && n <= max_lds) {
     float *lx = reinterpret_cast<float *>(lds_agents);
@@ -535,7 +545,7 @@ int abm_wealth_run_batch(const abm_wealth_inst *insts, int32_t count, int64_t ma
     size_t dyn = (size_t)(max_n * kWealthLdsBytesPerAgent + 4);
     // an instance of n < kWealthThreads keeps its increments and two
     // histograms there
-    if (dyn < 3 * kWealthThreads * sizeof(int32_t)) dyn = 3 * kWealthThreads * sizeof(int32_t);
+    if (dyn < 4 * kWealthThreads * sizeof(int32_t)) dyn = 4 * kWealthThreads * sizeof(int32_t);
     hipLaunchKernelGGL(k_wealth_run<true>, dim3((unsigned)count), dim3(kWealthThreads), dyn,
                        (hipStream_t)stream, insts, max_n, mode);
   } else {
