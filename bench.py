@@ -30,7 +30,10 @@ Also reported:
   * wealth, schelling, deffuant: the other programs' lines (1,000-agent
     100-step wealth exchange, 512 x 512 200-step Schelling, 65,536-agent x
     256-instance 200-step Deffuant gradient evaluation), each with its
-    cpu_baseline and roofline.
+    cpu_baseline and roofline;
+  * generic: a 10,000-agent 200-step model with no library kernel, run
+    eagerly and as graph replays, vmapped and fused (steps/s of each, the
+    value being graph replays of the fused update).
 """
 from __future__ import annotations
 
@@ -889,8 +892,9 @@ def main() -> int:
                     help="requests in the serving trace (0 = skip the serving measurement)")
     ap.add_argument("--workload", default="sir",
                     choices=["sir", "deffuant", "wealth", "schelling", "generic", "serving"],
-                    help="sir (default): the headline line, with the wealth, Schelling and "
-                         "Deffuant programs embedded; the others print one program's line")
+                    help="sir (default): the headline line, with the wealth, Schelling, "
+                         "Deffuant and generic-model lines embedded; the others print one "
+                         "program's line")
     ap.add_argument("--profile-run", action="store_true",
                     help="the SIR graph replays only (warm-up and timed), for rocprofv3: no "
                          "step-by-step replay, no other programs, no CPU baseline, no serving")
@@ -928,6 +932,7 @@ def main() -> int:
         out["wealth"] = sub_line(bench_wealth, args)
         out["schelling"] = sub_line(bench_schelling, args)
         out["deffuant"] = sub_line(bench_deffuant, args)
+        out["generic"] = sub_line(bench_generic, args)
     print(json.dumps(out))
     return 0
 
