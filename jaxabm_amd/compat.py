@@ -13,7 +13,9 @@ tests/integration/test_integration.py).  JAX is not part of this engine, so
 * ``jax`` -> a module with ``numpy``, ``random``, ``vmap``
   (``torch.func.vmap``), ``grad`` (``torch.func.grad``), ``jit`` (returns
   the function: steps are captured as HIP graphs by ``Model.run`` instead),
-  ``devices()`` and ``__version__``.
+  ``devices()``, ``__version__`` and ``Array`` (``torch.Tensor``: the type
+  the reference's signatures annotate keys and states with, e.g.
+  ``key: jax.Array`` in tests/integration/test_integration.py:33).
 
 Call it once, before the model's imports::
 
@@ -65,6 +67,7 @@ def _jax_module() -> types.ModuleType:
     jax.grad = torch.func.grad
     jax.jit = _jit
     jax.devices = _devices
+    jax.Array = torch.Tensor
     jax.__version__ = f"jaxabm_amd-{__version__}"
     jax.__path__ = []  # a package, so `import jax.numpy` resolves through sys.modules
     return jax

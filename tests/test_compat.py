@@ -36,6 +36,15 @@ assert double is not None and jit(double, static_argnums=0) is double
 assert torch.equal(vmap(lambda r: r.sum())(torch.ones(3, 4)), torch.full((3,), 4.0))
 assert float(grad(lambda x: (x * x).sum())(torch.tensor([3.0]))[0]) == 6.0
 assert jax.devices() and jax.__version__.startswith("jaxabm_amd")
+# the reference annotates keys and states with jax.Array
+# (tests/integration/test_integration.py:33): class bodies must evaluate
+assert jax.Array is torch.Tensor
+class Walker(AgentType):
+    def init_state(self, model_config: ModelConfig, key: jax.Array) -> dict:
+        return {"x": random.uniform(key)}
+    def update(self, state, model_state, model_config, key: jax.Array) -> dict:
+        return state
+assert Walker.update.__annotations__["key"] is torch.Tensor
 compat.uninstall()
 import sys
 assert "jax" not in sys.modules and "jaxabm" not in sys.modules

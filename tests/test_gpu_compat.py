@@ -11,12 +11,12 @@ MODEL = r'''
 IMPORTS
 
 class Trader(AgentType):
-    def init_state(self, model_config, key):
+    def init_state(self, model_config, key: jax.Array):
         k1, k2 = random.split(key)
         return {"cash": random.uniform(k1) * 10.0,
                 "risk": random.randint(k2, (), 0, 5)}
 
-    def update(self, state, model_state, model_config, key):
+    def update(self, state, model_state, model_config, key: jax.Array):
         k1, k2 = random.split(key)
         price = model_state["env"]["price"]
         buy = random.uniform(k1) < 0.5 / (1.0 + state["risk"])
@@ -56,6 +56,7 @@ from jaxabm.model import Model
 """
 
 NATIVE_IMPORTS = """
+import torch as jax   # jax.Array -> torch.Tensor, as compat.install() maps it
 import jaxabm_amd as jx
 from jaxabm_amd import numpy as jnp
 from jaxabm_amd import random
