@@ -1050,6 +1050,29 @@ class _Trace:
             idxs.append(Sym(self, ix, "i64", x.varying))
         return SymVec(self, vals), SymVec(self, idxs)
 
+    def vany(self, x: SymVec, every: bool) -> Sym:
+        """torch.any / torch.all of a per-agent vector: a value counts as
+        true when it is nonzero (NaN included)."""
+        op = " && " if every else " || "
+        terms = [f"((bool)({v.name}))" for v in x.items]
+        return Sym(self, self.tmp("bool", "(" + op.join(terms) + ")"), "bool", x.varying)
+
+    def vcount(self, x: SymVec) -> Sym:
+        """torch.count_nonzero of a per-agent vector (int64; -0 counts as
+        zero, NaN as nonzero)."""
+        terms = [f"(int64_t)({v.name} != 0)" for v in x.items]
+        return Sym(self, self.tmp("int64_t", "(" + " + ".join(terms) + ")"), "i64", x.varying)
+
+    def vcat(self, parts) -> SymVec:
+        """torch.cat / jnp.concatenate of per-agent (or uniform) vectors of
+        one dtype, at most 16 values in all."""
+        vecs = [self.voperand(p) for p in parts]
+        if not vecs or not all(isinstance(v, SymVec) for v in vecs):
+            raise Unsupported("a concatenation of other than per-agent vectors")
+        if len({v.kind for v in vecs}) != 1:
+            raise Unsupported("a concatenation of vectors of different dtypes")
+        return SymVec(self, [e for v in vecs for e in v.items])
+
     def vextreme(self, x: SymVec, largest: bool, along_dim: bool) -> Sym:
         """torch.max / torch.min (whole vector) and torch.amax / torch.amin
         (over its dim) of a per-agent vector, as torch's reductions give
@@ -1195,6 +1218,16 @@ class _Trace:
                 expr = {"and": "{} & {}", "or": "{} | {}", "xor": "{} ^ {}"}[op]
             ca, cb = self._as(a, kind), self._as(b, kind)
             return Sym(self, self.tmp(_CT[kind], "(" + expr.format(ca, cb) + ")"), kind, varying)
+        if op in ("atan2", "hypot"):
+            # torch's element functions: atan2f / hypotf of the promoted
+            # floats (integers and booleans promote to float32)
+            if weak:
+                raise Unsupported(f"{op} of two Python scalars")
+            if kind not in _FLOATS:
+                kind = "f32"
+            fn = op + ("f" if kind == "f32" else "")
+            ca, cb = self._as(a, kind), self._as(b, kind)
+            return Sym(self, self.tmp(_CT[kind], f"{fn}({ca}, {cb})"), kind, varying)
         if kind in ("bool", "pyb"):
             if weak and op in ("add", "sub", "mul", "floordiv", "mod"):
                 kind = "pyi"          # Python: True + True == 2
@@ -1471,7 +1504,8 @@ class _Trace:
                   "multiply": "mul", "div": "div", "true_divide": "div", "divide": "div",
                   "floor_divide": "floordiv", "remainder": "mod", "lt": "lt", "less": "lt",
                   "le": "le", "gt": "gt", "greater": "gt", "ge": "ge", "eq": "eq", "ne": "ne",
-                  "bitwise_and": "and", "bitwise_or": "or", "bitwise_xor": "xor"}
+                  "bitwise_and": "and", "bitwise_or": "or", "bitwise_xor": "xor",
+                  "atan2": "atan2", "arctan2": "atan2", "hypot": "hypot"}
         if name in binary and len(args) == 2 and not kwargs:
             return self.binary(binary[name], args[0], args[1])
         dunder = {"__add__": "add", "__sub__": "sub", "__mul__": "mul", "__truediv__": "div",
@@ -1563,6 +1597,28 @@ class _Trace:
             if dim in (0, -1):
                 values, indices = self.vsort(args[0])
                 return indices if name == "argsort" else torch.return_types.sort((values, indices))
+        if name in ("any", "all") and len(args) == 1 and isinstance(args[0], SymVec) and \
+                set(kwargs) <= {"dim", "keepdim"} and kwargs.get("dim") in (None, 0, -1) and \
+                not kwargs.get("keepdim"):
+            return self.vany(args[0], every=name == "all")
+        if name == "count_nonzero" and len(args) == 1 and isinstance(args[0], SymVec) and \
+                set(kwargs) <= {"dim"} and kwargs.get("dim") in (None, 0, -1):
+            return self.vcount(args[0])
+        if name in ("cat", "concatenate", "concat") and len(args) >= 1 and \
+                isinstance(args[0], (list, tuple)) and \
+                kwargs.get("dim", args[1] if len(args) > 1 else 0) in (0, -1):
+            return self.vcat(args[0])
+        if name == "flip" and len(args) in (1, 2) and isinstance(args[0], SymVec):
+            dims = args[1] if len(args) == 2 else kwargs.get("dims")
+            if dims in ((0,), [0], (-1,), [-1], 0, -1):
+                return SymVec(self, list(reversed(args[0].items)))
+        if name == "roll" and len(args) in (2, 3) and isinstance(args[0], SymVec) and \
+                isinstance(args[1], int) and not isinstance(args[1], bool):
+            dims = args[2] if len(args) == 3 else kwargs.get("dims")
+            if dims in (None, 0, -1, (0,), [0], (-1,), [-1]):
+                items, k = args[0].items, len(args[0])
+                sh = args[1] % k
+                return SymVec(self, items[k - sh:] + items[:k - sh])
         if name in ("var", "std") and len(args) == 1 and isinstance(args[0], SymVec) and \
                 set(kwargs) <= {"dim", "keepdim", "correction", "unbiased"} and \
                 kwargs.get("dim") in (None, 0, -1, (0,), (-1,), [0], [-1]) and \
@@ -1915,6 +1971,11 @@ def run_agents(fn, n: int, key: torch.Tensor, mode: int, states: Optional[Dict[s
         name, src, tr, outs = trace(n, key, mode, states, model_state, call)
     except Unsupported as e:
         _refused[sig] = str(e)
+        return None
+    except Exception as e:  # noqa: BLE001 - anything the tracer cannot follow
+        # runs vmapped, where the function executes for real (and raises
+        # there if it is itself in error)
+        _refused[sig] = f"{type(e).__name__}: {e}"
         return None
     kern = _kernels.get(src)
     if kern is None:

@@ -575,6 +575,15 @@ def take(x, indices, axis=None):
     """jnp.take with its default mode "fill": negative indices count from the
     end, and an index still outside the axis yields the fill value (NaN,
     the most negative integer, the largest unsigned, True)."""
+    if getattr(x, "_abm_sym", False) and isinstance(indices, int) and \
+            not isinstance(indices, bool) and axis in (None, 0, -1) and hasattr(x, "items"):
+        # a per-agent vector inside a traced agent function, a constant index
+        k = len(x)
+        i = indices + k if indices < 0 else indices
+        if not 0 <= i < k:
+            from .fuse import Unsupported
+            raise Unsupported("jnp.take out of range (the fill value) in a traced function")
+        return x[i]
     x = _t(x)
     idx = _t(indices).to(device=x.device, dtype=torch.long)
     src = x.reshape(-1) if axis is None else x
@@ -590,6 +599,46 @@ def take(x, indices, axis=None):
         *x.shape[:ax], *idx.shape, *x.shape[ax + 1:])
     mask = valid.reshape((1,) * ax + tuple(idx.shape) + (1,) * (x.dim() - ax - 1))
     return torch.where(mask, out, torch.full_like(out, _take_fill(x.dtype)))
+
+
+def flip(x, axis=None):
+    """jnp.flip: reverse the order along ``axis`` (every axis when None)."""
+    x = _t(x)
+    dims = tuple(range(x.dim())) if axis is None else \
+        (tuple(axis) if isinstance(axis, (tuple, list)) else (axis,))
+    return torch.flip(x, dims)
+
+
+def roll(x, shift, axis=None):
+    """jnp.roll: shift along ``axis``; with no axis the flattened array is
+    rolled and the shape kept."""
+    x = _t(x)
+    if axis is None:
+        return torch.roll(x, shift)
+    return torch.roll(x, shift, axis)
+
+
+def median(x, axis=None, keepdims=False):
+    """jnp.median: the middle value, the mean of the two middle values for
+    an even count (numpy's rule); integers promote to float32; a NaN gives
+    NaN."""
+    x = _t(x)
+    if not x.is_floating_point():
+        x = x.to(torch.float32)
+    if axis is None:
+        return torch.quantile(x.reshape(-1), 0.5)
+    return torch.quantile(x, 0.5, dim=axis, keepdim=keepdims)
+
+
+def hypot(a, b):
+    """jnp.hypot: sqrt(a^2 + b^2) without overflow; integers promote to
+    float32."""
+    a, b = _t(a), _t(b)
+    if not a.is_floating_point():
+        a = a.to(torch.float32)
+    if not b.is_floating_point():
+        b = b.to(torch.float32)
+    return torch.hypot(a, b)
 
 
 def dot(a, b):

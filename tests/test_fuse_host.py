@@ -197,6 +197,49 @@ def test_sorts_trace_and_compile():
             _trace(bad, st, {"env": {}})
 
 
+def test_vector_logic_concat_flip_roll_take_atan2_trace_and_compile():
+    """jnp.any / all / count_nonzero, concatenate, flip, roll and a constant
+    take of per-agent vectors, arctan2 and hypot: one generated kernel;
+    an out-of-range constant take (JAX's fill value) is refused."""
+    st = {"p": torch.rand(N, 5) - 0.5, "q": torch.randint(-3, 4, (N, 3), dtype=torch.int32),
+          "x": torch.rand(N)}
+
+    def upd(s, m, key):
+        v = s["p"]
+        return {"a": jnp.any(v > 0.25), "l": jnp.all(v > -0.4), "c": jnp.count_nonzero(s["q"]),
+                "cat": jnp.concatenate([v, v * 2.0]), "f": jnp.flip(v), "r": jnp.roll(v, 2),
+                "t": jnp.take(v, -1), "y": jnp.arctan2(s["x"], v[0]),
+                "h": jnp.hypot(s["x"], 3.0)}
+
+    name, src, tr, outs = _trace(upd, st, {"env": {}})
+    assert {kk: (vv.dtype, tuple(vv.shape)) for kk, vv in outs.items()} == {
+        "a": (torch.bool, (N,)), "l": (torch.bool, (N,)), "c": (torch.int32, (N,)),
+        "cat": (torch.float32, (N, 10)), "f": (torch.float32, (N, 5)),
+        "r": (torch.float32, (N, 5)), "t": (torch.float32, (N,)), "y": (torch.float32, (N,)),
+        "h": (torch.float32, (N,))}
+    assert "atan2f(" in src and "hypotf(" in src
+    fuse.compile_source(src)
+    with pytest.raises(fuse.Unsupported, match="out of range"):
+        _trace(lambda s, m, k: {"t": jnp.take(s["p"], 7)}, st, {"env": {}})
+
+
+def test_numpy_flip_roll_median_hypot_follow_numpy():
+    """jnp.flip / roll / median / hypot on host tensors equal numpy's."""
+    rs = np.random.default_rng(2)
+    a = rs.standard_normal((4, 6)).astype(np.float32)
+    t = torch.from_numpy(a)
+    np.testing.assert_array_equal(jnp.flip(t).numpy(), np.flip(a))
+    np.testing.assert_array_equal(jnp.flip(t, 1).numpy(), np.flip(a, 1))
+    np.testing.assert_array_equal(jnp.roll(t, 3).numpy(), np.roll(a, 3))
+    np.testing.assert_array_equal(jnp.roll(t, -2, 1).numpy(), np.roll(a, -2, 1))
+    np.testing.assert_allclose(float(jnp.median(t)), np.median(a), rtol=4e-6)
+    np.testing.assert_allclose(jnp.median(t, axis=1).numpy(), np.median(a, axis=1), rtol=4e-6)
+    assert float(jnp.median(torch.tensor([1, 2, 3, 10], dtype=torch.int32))) == 2.5
+    np.testing.assert_allclose(jnp.hypot(t, torch.tensor(2.0)).numpy(),
+                               np.hypot(a, np.float32(2.0)), rtol=1e-6)
+    assert jnp.hypot(torch.tensor([3]), torch.tensor([4])).item() == 5.0
+
+
 def test_walker_with_vectors_traces_and_compiles():
     """A bouncing walker in the style of the reference's basic example
     (examples/basic_example.py:20-70): 2-vectors of position and velocity,

@@ -233,7 +233,7 @@ OPS = {
     "delta": lambda s, m: s["x"] + m["env"]["delta"], "invert": lambda s, m: ~s["b"],
     "half": lambda s, m: s["x"] * 0.5, "sub_t": lambda s, m: s["x"] - s["x"] * 0.5,
     "pow_mhalf": lambda s, m: s["x"] ** -0.5, "pow_w_mhalf": lambda s, m: s["w"] ** -0.5,
-    "rsqrt": "rsqrt",
+    "rsqrt": "rsqrt", "atan2": "atan2", "atan2_w": "atan2_w", "hypot": "hypot",
 }
 
 
@@ -244,6 +244,9 @@ def _jnp_op(name):
 
     return {
         "rsqrt": lambda s, m: torch.rsqrt(s["x"]),
+        "atan2": lambda s, m: jnp.arctan2(s["x"], s["x"] * 0.5 - 1.0),
+        "atan2_w": lambda s, m: jnp.arctan2(s["w"], s["x"]),
+        "hypot": lambda s, m: jnp.hypot(s["x"], s["x"] * 0.25 + 1.0),
         "log": lambda s, m: jnp.log(s["x"]),
         "min_nan": lambda s, m: jnp.minimum(jnp.log(s["x"]), s["x"] * 0.0),
         "max_sz": lambda s, m: jnp.maximum(s["x"] * 0.0, -(s["x"] * 0.0)),
@@ -599,6 +602,53 @@ def test_sorts_fused_equal_vmapped(J, k):
                                           b[kk].view(torch.int32).cpu().numpy(), err_msg=kk)
         else:
             np.testing.assert_array_equal(a[kk].cpu().numpy(), b[kk].cpu().numpy(), err_msg=kk)
+
+
+def test_vector_logic_concat_flip_roll_take_fused_equal_vmapped(J):
+    """jnp.any / all / count_nonzero (NaN counts as true, -0 as zero),
+    concatenate, flip, roll and constant takes of per-agent vectors, fused
+    and vmapped alike."""
+    import torch
+
+    from jaxabm_amd import fuse
+    from jaxabm_amd import numpy as jnp
+    from jaxabm_amd import random as jr
+
+    g = torch.Generator(device="cuda").manual_seed(17)
+    n = 20_000
+    pool = torch.tensor([0.0, -0.0, 1.0, -2.5, float("nan"), float("inf")], device="cuda")
+    p = pool[torch.randint(0, 6, (n, 4), device="cuda", generator=g)]
+    q = torch.randint(-1, 2, (n, 3), device="cuda", generator=g, dtype=torch.int32)
+
+    class Logic:
+        def init_state(self, cfg, key):
+            return {"x": 0.0}
+
+        def update(self, s, ms, cfg, key):
+            v, w = s["p"], s["q"]
+            return {"a": jnp.any(v), "l": jnp.all(v), "c": jnp.count_nonzero(v),
+                    "ci": jnp.count_nonzero(w), "ai": jnp.any(w > 0),
+                    "cat": jnp.concatenate([v, v * 2.0]), "f": jnp.flip(v),
+                    "r": jnp.roll(v, 1), "rn": jnp.roll(w, -1), "t": jnp.take(v, -1),
+                    "t0": jnp.take(w, 0)}
+
+    res = []
+    for fused in (False, True):
+        fuse.enable(fused)
+        try:
+            before = fuse.stats()["launches"]
+            c = J.AgentCollection(Logic(), n)
+            c.model_config = J.ModelConfig()
+            c._states = {"p": p.clone(), "q": q.clone()}
+            c.update({"env": {}}, jr.PRNGKey(1), J.ModelConfig())
+            assert fuse.stats()["launches"] - before == (1 if fused else 0), fuse.refused()
+            res.append({kk: vv.clone() for kk, vv in c.states.items()})
+        finally:
+            fuse.enable(True)
+    a, b = res
+    for kk in a:
+        assert a[kk].dtype == b[kk].dtype and a[kk].shape == b[kk].shape, kk
+        np.testing.assert_array_equal(bits(a[kk]), bits(b[kk]), err_msg=kk)
 
 
 def test_vector_at_updates_and_bool_subtraction_fused_equal_vmapped(J):
