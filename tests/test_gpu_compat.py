@@ -56,7 +56,8 @@ from jaxabm.model import Model
 """
 
 NATIVE_IMPORTS = """
-import torch as jax   # jax.Array -> torch.Tensor, as compat.install() maps it
+import types, torch
+jax = types.SimpleNamespace(Array=torch.Tensor)   # as compat.install() maps jax.Array
 import jaxabm_amd as jx
 from jaxabm_amd import numpy as jnp
 from jaxabm_amd import random

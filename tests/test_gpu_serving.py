@@ -46,6 +46,13 @@ def requests(sv):
         R("sir", n=5003, steps=33, seed=3, graph_seed=8, beta=0.05, gamma=0.07, n_infected=20),
         R("wealth", n=2, steps=9, seed=4),
         R("sir", n=700, steps=20, seed=5, graph_seed=9, m=3, beta=0.2, gamma=0.3),
+        # a stepwise wealth step t is applied at sub-step t + 1 and recorded at
+        # t + 2: runs whose last record falls at, before and after the end of
+        # a 7-step tick, and a one-step run
+        R("wealth", n=1500, steps=1, seed=6),
+        R("wealth", n=1501, steps=5, seed=7),
+        R("wealth", n=1502, steps=6, seed=8),
+        R("wealth", n=1503, steps=12, seed=9),
     ]
 
 
