@@ -45,9 +45,9 @@ constexpr int kWordsPerBlock = kCellsPerBlock / 32;
 // block prefix of the empty counts in LDS: at most 64 KB
 constexpr int64_t kMaxEmptyBlocks = 16383;
 // copies of the per-block empty counts: the moves kernel's workgroup w updates
-// copy w % kCountCopies, so the count atomics of a busy step spread over 8x
+// copy w % kCountCopies, so the count atomics of a busy step spread over 16x
 // the words (each word's device-scope atomics serialise); readers sum them
-constexpr int kCountCopies = 8;
+constexpr int kCountCopies = 16;
 
 // number of cells with (cells[c] == 0) == want_empty per 1024-cell block
 __global__ void __launch_bounds__(kCellsPerBlock)
