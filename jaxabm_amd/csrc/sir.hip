@@ -416,23 +416,27 @@ __global__ void __launch_bounds__(kSirThreads) k_sir_step(SirStep a) {
       lds_barrier();
       // phase 2: scans (heavy rows by waves, light rows by lanes) against the
       // LDS copy of the bitmap when the tile has enough rows, else the global one
+      // (block-uniform: a tile with no row to scan writes no other thread's
+      // outb byte, so it needs no barrier before phase 3)
       {
         const int32_t nl = n_light, nh = n_heavy;
-        if constexpr (kLds) {
-          if (!staged && nl + nh >= kStageItems) {
-            const uint4 *src = reinterpret_cast<const uint4 *>(a.bm_in);
-            for (int64_t i = tid; i < (a.words >> 2); i += kSirThreads) lds_bm4[i] = src[i];
-            staged = true;
-            lds_barrier();
+        if (nl + nh > 0) {
+          if constexpr (kLds) {
+            if (!staged && nl + nh >= kStageItems) {
+              const uint4 *src = reinterpret_cast<const uint4 *>(a.bm_in);
+              for (int64_t i = tid; i < (a.words >> 2); i += kSirThreads) lds_bm4[i] = src[i];
+              staged = true;
+              lds_barrier();
+            }
           }
+          if (kLds && staged)
+            scan_items<kMode>(a, reinterpret_cast<const uint32_t *>(lds_bm4), sub, nl, nh,
+                              items, outb, kinf, &next_heavy);
+          else
+            scan_items<kMode>(a, a.bm_in, sub, nl, nh, items, outb, kinf, &next_heavy);
+          lds_barrier();
         }
-        if (kLds && staged)
-          scan_items<kMode>(a, reinterpret_cast<const uint32_t *>(lds_bm4), sub, nl, nh,
-                            items, outb, kinf, &next_heavy);
-        else
-          scan_items<kMode>(a, a.bm_in, sub, nl, nh, items, outb, kinf, &next_heavy);
       }
-      lds_barrier();
       // phase 3: outputs (one 8-byte state store when anything changed), the
       // next bitmap (one byte = these 8 agents), counts, the mark worklist
       const uint2 rw = *reinterpret_cast<const uint2 *>(outb + tid * kPerThread);
@@ -474,24 +478,28 @@ __global__ void __launch_bounds__(kSirThreads) k_sir_step(SirStep a) {
       push_items(lmask, hmask, items, &m_light, &m_heavy);
       lds_barrier();
       // phase 4: marks of newly infected agents and periodic refreshes
+      // (block-uniform: with none, no thread reads the worklist or its
+      // counters, and the next tile's reset only rewrites zeros)
       {
         const int32_t nl = m_light, nh = m_heavy;
-        for (;;) {
-          int32_t k = 0;
-          if (lane == 0) k = atomicAdd(&next_mark, 1);
-          k = __shfl(k, 0);
-          if (k >= nh) break;
-          const int64_t u = sub + items[kTile - 1 - k];
-          mark_heavy(a.col, a.mark, a.row_ptr[u], a.row_ptr[u + 1], new_mark);
-        }
-        for (int32_t k0 = tid; k0 < nl; k0 += kSirThreads * kScanGroup) {
-          int loc[kScanGroup];
-          int32_t e[kScanGroup], hi[kScanGroup];
-          light_group_rows(a, sub, k0, nl, items, loc, e, hi);
-          mark_light_group(a.col, a.mark, e, hi, new_mark);
+        if (nl + nh > 0) {
+          for (;;) {
+            int32_t k = 0;
+            if (lane == 0) k = atomicAdd(&next_mark, 1);
+            k = __shfl(k, 0);
+            if (k >= nh) break;
+            const int64_t u = sub + items[kTile - 1 - k];
+            mark_heavy(a.col, a.mark, a.row_ptr[u], a.row_ptr[u + 1], new_mark);
+          }
+          for (int32_t k0 = tid; k0 < nl; k0 += kSirThreads * kScanGroup) {
+            int loc[kScanGroup];
+            int32_t e[kScanGroup], hi[kScanGroup];
+            light_group_rows(a, sub, k0, nl, items, loc, e, hi);
+            mark_light_group(a.col, a.mark, e, hi, new_mark);
+          }
+          lds_barrier();
         }
       }
-      lds_barrier();
     }
   }
   // block counts: per-thread counts -> wave sums -> LDS -> one atomic per class
