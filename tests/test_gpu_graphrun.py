@@ -178,6 +178,40 @@ def test_large_model_graph_matches_eager_every_replay(J):
     assert out["t"] == list(range(1, 31))
 
 
+def test_batched_replays_match_eager(J, monkeypatch):
+    """A run long enough for graphs of several step bodies (here 4 a launch
+    from 40 steps, 43 steps: ten 4-step replays and one 1-step replay, with
+    eager work and a synchronisation between replays) equals the eager loop,
+    history and collect_interval included, and a continued run too."""
+    import torch
+
+    from jaxabm_amd import graphrun
+
+    monkeypatch.setattr(graphrun, "STEPS_PER_LAUNCH", 4)
+    monkeypatch.setattr(graphrun, "BATCH_MIN_STEPS", 40)
+    a, b = make_trader(J, n=20_000, interval=2), make_trader(J, n=20_000, interval=2)
+    ra = a.run(43)
+    b.initialize()
+    b._history = []
+    scratch = torch.rand(1 << 20, device="cuda")
+    calls = []
+
+    def between():
+        calls.append(1)
+        scratch.mul_(1.0001).sum()
+        torch.cuda.synchronize()
+
+    info = graphrun.run_graph(b, 43, between=between)
+    assert info["mode"] == "graph" and info["steps_per_launch"] == 4 and info["steps_done"] == 43
+    assert len(calls) == 10       # 11 replays
+    out = {"step": [h["time_step"] for h in b._history]}
+    for k in b._history[0]["metrics"]:
+        out[k] = [h["metrics"][k] for h in b._history]
+    assert_same_run(a, b, dict(ra), out)
+    assert_same_run(a, b, a.run(45), b.run(45, graph=True))
+    assert b.last_run_info["steps_per_launch"] == 4
+
+
 def test_collect_interval_and_continuation(J):
     a, b = make_trader(J, n=5000, interval=3), make_trader(J, n=5000, interval=3)
     assert_same_run(a, b, a.run(10), b.run(10, graph=True))
