@@ -260,8 +260,11 @@ int abm_wealth_run_batch(const abm_wealth_inst *insts, int32_t count, int64_t ma
  * with the previous step's mean and Gini recorded by workgroup 0; then the
  * exchange applied, the wealth histogram, total and maximum).  inst: the
  * descriptor in host memory (passed to the kernels by value); its draws
- * field is not used.  acc: device scratch uint32[2].  Same results as
+ * field is not used.  acc: device scratch uint32[ABM_WEALTH_GRID_ACC_WORDS]
+ * (the total and 16 copies of the 64 smallest histogram bins, so that no
+ * word takes an atomic from every workgroup).  Same results as
  * abm_wealth_run_batch. */
+#define ABM_WEALTH_GRID_ACC_WORDS 1026
 int abm_wealth_run_grid(const abm_wealth_inst *inst, uint32_t *acc, int mode, void *stream);
 
 /* ---- Moore-8 neighbourhood reductions over a (rows, cols) row-major Grid

@@ -26,6 +26,7 @@ class CopyItem(C.Structure):
                 ("row_stride", C.c_int64), ("row_limit", C.c_int64)]
 
 REDUCE_BATCH_MAX = 16              # abm_reduce_batch
+WEALTH_GRID_ACC_WORDS = 1026       # abm_wealth_run_grid's acc scratch
 (REDUCE_SUM_F32, REDUCE_MEAN_F32, REDUCE_SUM_I32, REDUCE_MEAN_I32,
  REDUCE_SUM_U8) = range(5)
 

@@ -85,8 +85,8 @@ struct Inst {
   int32_t *counts;           // SIR [steps][3]
   float *x, *y;
   // wealth: w[n]; the stepwise path keeps two copies of the exchange
-  // increments inc[2][n], the histogram hist[2][n+1] and acc[2] = (max wealth,
-  // total), one per step parity (the resident path uses copy 0 of hist)
+  // increments inc[2][n] and of the histogram hist[2][n+1], one per step
+  // parity, and the total in acc[0] (the resident path uses copy 0 of hist)
   int32_t *w, *inc, *hist, *acc;
   float *mean, *gini;             // wealth [steps]
 };
@@ -105,19 +105,6 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t *red) {
   uint32_t s = 0;
 #pragma unroll
   for (int w = 0; w < THREADS / 64; ++w) s += red[w];
-  __syncthreads();
-  return s;
-}
-
-template <int THREADS>
-__device__ __forceinline__ int32_t block_max(int32_t v, uint32_t *red) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = (uint32_t)v;
-  __syncthreads();
-  int32_t s = INT_MIN;
-#pragma unroll
-  for (int w = 0; w < THREADS / 64; ++w) s = max(s, (int32_t)red[w]);
   __syncthreads();
   return s;
 }
@@ -160,6 +147,42 @@ __device__ uint32_t rank_sum_and_clear(uint32_t *lh, int32_t lbins, int32_t *gh,
     __syncthreads();
     if (tid == THREADS - 1) *carry = before + c;
     __syncthreads();
+  }
+  return S;
+}
+
+// The same rank sum over a global histogram gh[0 .. n], read THREADS bins a
+// round until all n agents are counted (every nonzero bin lies in the
+// rounds read), so no largest wealth is needed.  Clears the bins it reads.
+template <int THREADS>
+__device__ uint32_t rank_sum_until_n(int32_t *gh, int64_t n, uint32_t *red, uint32_t *carry) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) *carry = 0;
+  __syncthreads();
+  uint32_t S = 0;
+  for (int64_t base = 0; base <= n; base += THREADS) {
+    const int64_t w = base + tid;
+    const uint32_t c = w <= n ? (uint32_t)gh[w] : 0u;
+    uint32_t inc = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(inc, off);
+      if (lane >= off) inc += y;
+    }
+    if (lane == 63) red[wave] = inc;
+    __syncthreads();
+    uint32_t before = *carry;
+    for (int q = 0; q < wave; ++q) before += red[q];
+    before += inc - c;
+    if (c) {
+      const uint64_t rs = (uint64_t)c * before + (uint64_t)c * (c + 1) / 2;
+      S += (uint32_t)w * (uint32_t)rs;
+      gh[w] = 0;
+    }
+    __syncthreads();
+    if (tid == THREADS - 1) *carry = before + c;
+    __syncthreads();
+    if ((int64_t)*carry >= n) break;  // block-uniform
   }
   return S;
 }
@@ -419,21 +442,15 @@ __global__ void __launch_bounds__(kResThreads)
 
 // ---------------------------------------------------------------- stepwise path
 // Mean and Gini of wealth step t (one workgroup): rank sum of the step's
-// histogram copy, the total and the largest wealth from its acc copy; clears
-// both for step t + 2.
+// histogram copy (cleared for step t + 2) and the total, which wealth
+// conservation keeps at acc[0] from step 0 on.
 __device__ void wealth_stepwise_record(const Inst &in, int32_t t, uint32_t *red,
                                        uint32_t *carry) {
   int32_t *hist = in.hist + (int64_t)(t & 1) * (in.n + 1);
-  int32_t *acc = in.acc + 2 * (t & 1);
-  const int32_t maxw = acc[0];
-  const uint32_t T = (uint32_t)acc[1];
-  uint32_t S = rank_sum_and_clear<kStepThreads>(nullptr, 0, hist, maxw, red, carry);
+  const uint32_t T = (uint32_t)in.acc[0];
+  uint32_t S = rank_sum_until_n<kStepThreads>(hist, in.n, red, carry);
   S = block_sum<kStepThreads>(S, red);
-  if (threadIdx.x == 0) {
-    wealth_record(in.n, T, S, in.mean, in.gini, t);
-    acc[0] = 0;
-    acc[1] = 0;
-  }
+  if (threadIdx.x == 0) wealth_record(in.n, T, S, in.mean, in.gini, t);
 }
 
 // Wealth sub-step t of range r (see k_serve_phase1).  The exchange of step
@@ -450,7 +467,6 @@ __device__ void wealth_stepwise(const Inst &in, const Range &r, int32_t t, int m
   int32_t *inc_old = in.inc + (int64_t)((t + 1) & 1) * n;  // step t - 1's increments
   int32_t *inc_new = in.inc + (int64_t)(t & 1) * n;
   int32_t *hist = in.hist + (int64_t)((t + 1) & 1) * (n + 1);  // step t - 1's histogram
-  int32_t *acc = in.acc + 2 * ((t + 1) & 1);
   if (apply) {
     bins[tid] = 0;
     __syncthreads();
@@ -462,7 +478,6 @@ __device__ void wealth_stepwise(const Inst &in, const Range &r, int32_t t, int m
   }
   const uint32_t span = (uint32_t)(n - 1);
   uint32_t total = 0;
-  int32_t mx = 0;
   for (int64_t i = r.begin + tid; i < r.end; i += kStepThreads) {
     if (apply) {
       const int32_t w0 = in.w[i];
@@ -470,7 +485,6 @@ __device__ void wealth_stepwise(const Inst &in, const Range &r, int32_t t, int m
       in.w[i] = wi;
       inc_old[i] = 0;
       total += (uint32_t)wi;
-      mx = max(mx, wi);
       if (wi < kStepBins)
         atomicAdd(&bins[wi], 1u);
       else
@@ -484,11 +498,9 @@ __device__ void wealth_stepwise(const Inst &in, const Range &r, int32_t t, int m
   if (apply) {
     __syncthreads();
     if (bins[tid]) atomicAdd(&hist[tid], (int32_t)bins[tid]);
-    const uint32_t T = block_sum<kStepThreads>(total, red);
-    const int32_t m = block_max<kStepThreads>(mx, red);
-    if (tid == 0) {
-      atomicAdd(&acc[1], (int32_t)T);
-      atomicMax(&acc[0], m);
+    if (t == 1) {  // after step 0: the total, conserved from here on
+      const uint32_t T = block_sum<kStepThreads>(total, red);
+      if (tid == 0) atomicAdd(&in.acc[0], (int32_t)T);
     }
   }
 }

@@ -349,30 +349,51 @@ __global__ void __launch_bounds__(kWealthThreads)
 // launches per step.  K1(t): every workgroup draws step t for its agents and
 // applies the draws (positions; one exchange atomic per giving agent), and
 // workgroup 0 first records step t-1 from the histogram.  K2(t): the
-// exchange is applied, the wealth histogram built (per-workgroup LDS bins,
-// flushed with one global atomic per bin in use: wealth stays bunched on a
-// few small values, so per-agent global atomics would all hit the same few
-// words), and the total and the largest wealth accumulated.  The same integer
-// and IEEE float operations as the one-workgroup run, so the same bits.
+// exchange is applied and the wealth histogram built.  Wealth stays bunched
+// on a few small values, so the histogram is gathered without hot words:
+// per-workgroup LDS bins, flushed with one atomic per bin in use, the
+// values below kGridHot into one of kGridCopies copies (chosen by workgroup)
+// and the rest into the instance's histogram.  The total T is conserved, so
+// it is summed once, at step 0; the record stops once it has counted all n
+// agents, so no running maximum is kept.  The same integer and IEEE float
+// operations as the one-workgroup run, so the same bits.
 constexpr int kGridThreads = 256;
 constexpr int kGridBins = 1024;     // LDS histogram bins per workgroup
 constexpr int64_t kGridMaxBlocks = 1024;
+constexpr int kGridHot = 64;        // values with their own histogram copies
+constexpr int kGridCopies = 16;
+static_assert(ABM_WEALTH_GRID_ACC_WORDS == 2 + kGridCopies * kGridHot,
+              "acc layout: T, a spare word, then the hot-bin copies");
 
-// Mean and Gini of step t from hist[0 .. maxw] and the total; clears the
-// bins and the accumulators (one workgroup).
+// Mean and Gini of step t from the histogram and the total; clears the bins
+// it reads (one workgroup).  Bins are read 256 a round until all n agents
+// are counted, which covers every nonzero bin.
 __device__ void wealth_grid_record(const abm_wealth_inst &in, int32_t t, uint32_t *acc,
                                    uint32_t *red) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t n = in.n;
   const uint32_t T = acc[0];
-  const int64_t maxw = (int64_t)acc[1];
+  uint32_t *hot = acc + 2;
   __shared__ uint32_t carry;
   if (tid == 0) carry = 0;
   __syncthreads();
   uint32_t S_part = 0;
-  for (int64_t base = 0; base <= maxw; base += kGridThreads) {
+  for (int64_t base = 0; base <= n; base += kGridThreads) {
     const int64_t w = base + tid;
-    const uint32_t c = w <= maxw ? (uint32_t)in.hist[w] : 0u;
+    uint32_t c = 0;
+    if (w < kGridHot) {
+      uint32_t v[kGridCopies];
+#pragma unroll
+      for (int k = 0; k < kGridCopies; ++k) v[k] = hot[k * kGridHot + w];
+#pragma unroll
+      for (int k = 0; k < kGridCopies; ++k) {
+        c += v[k];
+        if (v[k]) hot[k * kGridHot + w] = 0;
+      }
+    } else if (w <= n) {
+      c = (uint32_t)in.hist[w];
+      if (c) in.hist[w] = 0;
+    }
     uint32_t inc = c;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -387,11 +408,11 @@ __device__ void wealth_grid_record(const abm_wealth_inst &in, int32_t t, uint32_
     if (c) {
       const uint64_t ranksum = (uint64_t)c * before + (uint64_t)c * (c + 1) / 2;
       S_part += (uint32_t)w * (uint32_t)ranksum;
-      in.hist[w] = 0;
     }
     __syncthreads();
     if (tid == kGridThreads - 1) carry = before + c;
     __syncthreads();
+    if ((int64_t)carry >= n) break;  // block-uniform
   }
   const uint32_t S = block_sum_u32(S_part, red);
   if (tid == 0) {
@@ -406,8 +427,6 @@ __device__ void wealth_grid_record(const abm_wealth_inst &in, int32_t t, uint32_
       g = q - cst;
     }
     in.gini_out[t] = g;
-    acc[0] = 0;
-    acc[1] = 0;
   }
 }
 
@@ -427,7 +446,7 @@ __global__ void __launch_bounds__(kGridThreads)
 }
 
 __global__ void __launch_bounds__(kGridThreads)
-    k_wealth_grid_apply(abm_wealth_inst in, uint32_t *acc) {
+    k_wealth_grid_apply(abm_wealth_inst in, uint32_t *acc, int32_t t) {
   __shared__ uint32_t red[kGridThreads / 64];
   __shared__ uint32_t bins[kGridBins];
   for (int b = threadIdx.x; b < kGridBins; b += kGridThreads) bins[b] = 0;
@@ -435,7 +454,6 @@ __global__ void __launch_bounds__(kGridThreads)
   const int64_t n = in.n;
   const int64_t stride = (int64_t)gridDim.x * kGridThreads;
   uint32_t total = 0;
-  int32_t mx = 0;
   for (int64_t i = (int64_t)blockIdx.x * kGridThreads + threadIdx.x; i < n; i += stride) {
     const int32_t w0 = in.w[i];
     const int32_t wi = w0 - (w0 > 0 ? 1 : 0) + in.inc[i];
@@ -447,19 +465,25 @@ __global__ void __launch_bounds__(kGridThreads)
     else
       atomicAdd(&in.hist[wi], 1);
     total += (uint32_t)wi;
-    mx = max(mx, wi);
   }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off));
-  const uint32_t T = block_sum_u32(total, red);  // its barriers order the bins too
-  for (int b = threadIdx.x; b < kGridBins; b += kGridThreads)
-    if (bins[b]) atomicAdd(&in.hist[b], (int32_t)bins[b]);
-  if ((threadIdx.x & 63) == 0) atomicMax(&acc[1], (uint32_t)mx);
-  if (threadIdx.x == 0) atomicAdd(&acc[0], T);
+  __syncthreads();
+  uint32_t *hot = acc + 2 + (int64_t)(blockIdx.x % kGridCopies) * kGridHot;
+  for (int b = threadIdx.x; b < kGridBins; b += kGridThreads) {
+    const uint32_t c = bins[b];
+    if (!c) continue;
+    if (b < kGridHot)
+      atomicAdd(&hot[b], c);
+    else
+      atomicAdd(&in.hist[b], (int32_t)c);
+  }
+  if (t == 0) {  // the total, conserved from here on
+    const uint32_t T = block_sum_u32(total, red);
+    if (threadIdx.x == 0) atomicAdd(&acc[0], T);
+  }
 }
 
 __global__ void k_wealth_zero_acc(uint32_t *acc) {
-  if (threadIdx.x < 2) acc[threadIdx.x] = 0;
+  for (int i = threadIdx.x; i < ABM_WEALTH_GRID_ACC_WORDS; i += blockDim.x) acc[i] = 0;
 }
 
 __global__ void k_wealth_init(const uint32_t *__restrict__ init_key, int64_t n,
@@ -519,7 +543,7 @@ int abm_wealth_run_grid(const abm_wealth_inst *inst, uint32_t *acc, int mode, vo
               "abm_wealth_run_grid: incomplete instance");
   hipStream_t s = (hipStream_t)stream;
   if (in.steps == 0) return kOk;
-  hipLaunchKernelGGL(k_wealth_zero_acc, dim3(1), dim3(64), 0, s, acc);
+  hipLaunchKernelGGL(k_wealth_zero_acc, dim3(1), dim3(256), 0, s, acc);
   int64_t blocks = (in.n + kGridThreads - 1) / kGridThreads;
   if (blocks > kGridMaxBlocks) blocks = kGridMaxBlocks;
   for (int32_t t = 0; t <= in.steps; ++t) {
@@ -527,7 +551,7 @@ int abm_wealth_run_grid(const abm_wealth_inst *inst, uint32_t *acc, int mode, vo
                        in, t, acc, mode);
     if (t < in.steps)
       hipLaunchKernelGGL(k_wealth_grid_apply, dim3((unsigned)blocks), dim3(kGridThreads), 0, s,
-                         in, acc);
+                         in, acc, t);
   }
   ABM_LAUNCH_CHECK("abm_wealth_run_grid");
   return kOk;

@@ -19,7 +19,8 @@ import numpy as np
 import torch
 
 from .. import random as jr
-from .._lib import WEALTH_INST, check, lib, ptr, require_gpu, stream_handle
+from .._lib import (WEALTH_GRID_ACC_WORDS, WEALTH_INST, check, lib, ptr, require_gpu,
+                    stream_handle)
 
 
 # above the LDS capacity of one workgroup (8,000 agents), a run spreads over
@@ -84,7 +85,7 @@ class WealthEngine:
         if self.n > GRID_MIN_AGENTS:
             # draws computed inline by the step kernels: no draw buffer
             host = self.descriptor(keys, steps, mean, gini)
-            acc = torch.empty(2, dtype=torch.int32, device=self.device)
+            acc = torch.empty(WEALTH_GRID_ACC_WORDS, dtype=torch.int32, device=self.device)
             check(lib().abm_wealth_run_grid(host.ctypes.data, ptr(acc), self.mode,
                                             stream_handle()), "wealth_run_grid")
         else:
