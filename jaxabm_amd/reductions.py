@@ -132,11 +132,17 @@ def flush() -> None:
     if not pend:
         return
     _tls.pending = []
+    for _op, v, _out, _s, version in pend:
+        if v._version != version:
+            # the eager launch would have read the values before the change
+            raise RuntimeError("an array was modified in place after a whole-array sum or mean "
+                               "of it inside a model callback; compute the reduction after the "
+                               "update, or update a copy")
     stream = pend[0][3]
     for i in range(0, len(pend), REDUCE_BATCH_MAX):
         part = pend[i:i + REDUCE_BATCH_MAX]
         items = (ReduceItem * len(part))(*[ReduceItem(v.data_ptr(), v.numel(), out.data_ptr(),
-                                                      op, 0) for op, v, out, _s in part])
+                                                      op, 0) for op, v, out, _s, _ver in part])
         check(_fn("abm_reduce_batch")(items, len(part), stream), "reduce_batch")
 
 
@@ -173,8 +179,9 @@ def _whole_call(name: str, v: torch.Tensor, out_dtype) -> torch.Tensor:
         pend = _pending()
         if pend and pend[0][3] != stream:
             flush()
-        # the input stays referenced until its launch is enqueued
-        _pending().append((_OPS[name], v, out, stream))
+        # the input stays referenced until its launch is enqueued, and must
+        # not change before then (checked by its version counter)
+        _pending().append((_OPS[name], v, out, stream, v._version))
         d = torch.Tensor._make_subclass(_Deferred, out)
         d._abm_plain = out
         return d

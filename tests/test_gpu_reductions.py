@@ -221,6 +221,14 @@ def test_deferred_callback_reductions(R):
         assert a.view(torch.int32).item() == b._abm_plain.view(torch.int32).item()
     assert used.item() == (now[1] * 2.0 + 1.0).item()
     assert d["c"].item() == jnp.sum(xs[0]).item()
+    # an input changed in place before the batch goes out is an error, not a
+    # silently different sum
+    y = xs[3].clone()
+    with pytest.raises(RuntimeError, match="modified in place"):
+        with reductions.deferring():
+            jnp.sum(y)
+            y.add_(1.0)
+    assert not reductions._pending()
 
 
 def test_reduction_routing_rules(R):
