@@ -579,15 +579,49 @@ def bench_schelling(args) -> dict:
     enqueue_run()
     check(lib().abm_graph_end(sh, C.byref(g)), "graph_end")
 
-    def run():
+    def run_one_graph():
         check(lib().abm_graph_launch(g, sh), "graph_launch")
 
     for _ in range(args.warmup):
-        run()
-    t = _timed(stream, run, args.steps)
+        run_one_graph()
+    t_one = _timed(stream, run_one_graph, args.steps)
     same = bool(np.array_equal(happy.cpu().numpy(), ref[0]) and
                 np.array_equal(moves.cpu().numpy(), ref[1]))
     check(lib().abm_graph_destroy(g), "graph_destroy")
+
+    # the measured run: launches of 16 steps, each replayed as its own graph
+    # (the state restore in the first), stopped once the grid has settled
+    chunk = SchellingEngine.SETTLE_CHUNK
+    graphs = []
+    for s0 in range(0, S, chunk):
+        s1 = min(s0 + chunk, S)
+        gi = C.c_void_p()
+        check(lib().abm_graph_begin(sh), "graph_begin")
+        if s0 == 0:
+            for k, t0 in init.items():
+                check(lib().abm_memcpy_d2d(ptr(getattr(eng, k)), ptr(t0),
+                                           C.c_size_t(t0.numel() * t0.element_size()), sh),
+                      "reset")
+        eng.launch(keys[s0:], s1 - s0, counters[s0:], happy[s0:], moves[s0:], stream=stream)
+        check(lib().abm_graph_end(sh, C.byref(gi)), "graph_end")
+        graphs.append(gi)
+    launched = []
+
+    def run():
+        launched.append(eng.run_settling(
+            keys, S, counters, happy, moves, chunk, stream,
+            launch=lambda i, s0, s1: check(lib().abm_graph_launch(graphs[i], sh), "graph_launch")))
+
+    with torch.cuda.stream(stream):
+        happy.fill_(-1.0)   # the measured runs must write every record again
+        moves.fill_(-1)
+    for _ in range(args.warmup):
+        run()
+    t = _timed(stream, run, args.steps)
+    same = same and bool(np.array_equal(happy.cpu().numpy(), ref[0]) and
+                         np.array_equal(moves.cpu().numpy(), ref[1]))
+    for gi in graphs:
+        check(lib().abm_graph_destroy(gi), "graph_destroy")
     # identity steps (the previous step found every agent happy) read one
     # count and move nothing: only computed steps are charged the 10.8 B
     hf = np.asarray(ref[0])
@@ -602,8 +636,16 @@ def bench_schelling(args) -> dict:
         "config": {"workload": "schelling_512x512_200steps", "cells": H * W, "sim_steps": S,
                    "hipgraph": True,
                    "bench_step": "one full simulation from the seeded grid (state restored "
-                                 "by device copies, then 200 steps)"},
+                                 "by device copies, then 200 steps)",
+                   "launch": f"{chunk} steps per graph replay; after each the host reads the "
+                             "last step's happy count back, and once a launch ends with every "
+                             "agent happy the steps after the one queued behind it are "
+                             "recorded as identity steps without a launch"},
         "replays_identical": same,
+        "steps_launched": launched[-1],
+        "one_graph_ms_per_run": t_one * 1e3,
+        "one_graph_note": "all 200 steps as one graph replay (identity steps launched, each "
+                          "exiting after reading the previous step's count)",
         "cpu_baseline": _schelling_cpu(H, W, args.cpu_runs),
         "final_happy_fraction": float(ref[0][-1]), "total_moves": int(ref[1].sum()),
         "computed_steps": computed, "identity_steps": S - computed,

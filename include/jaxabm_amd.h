@@ -299,7 +299,7 @@ typedef struct abm_schelling_grid {
                             during a run, in 8 copies summed by the readers) */
 } abm_schelling_grid;
 /* Length in int32 of the block_counts scratch for a grid of n_cells cells:
- * 8 x ceil(n_cells / 1024). */
+ * 16 x ceil(n_cells / 1024). */
 int64_t abm_schelling_block_count(int64_t n_cells);
 /* Cells from uniform(key, (H*W,)) and the row-major agent numbering; sets
  * g->n_agents (synchronises the stream once). */
@@ -310,6 +310,18 @@ int abm_schelling_init(abm_schelling_grid *g, const uint32_t *key, int mode, int
  * to 16,776,192 cells. */
 int abm_schelling_run(const abm_schelling_grid *g, const uint32_t *keys, int32_t steps, int mode,
                       int32_t *counters, float *happy_frac, int32_t *moves, void *stream);
+/* Records `steps` settled steps without launching them.  A step whose
+ * previous step found every agent happy is the identity (nothing moves, so
+ * every agent is happy again), and so is every step after it.  prev_counters:
+ * device [2], the counters of the step before the first one recorded; it must
+ * hold a happy count of g->n_agents.  Writes counters (n_agents, 0), happy
+ * fraction 1 and 0 moves for each step, exactly what abm_schelling_run gives
+ * for those steps; when prev_counters holds another count, counters get -1,
+ * the happy fraction NaN and moves -1 instead.  Lets a caller that reads the
+ * count back stop launching a run once its grid has settled (the step loop is
+ * jaxabm/model.py:146-216, which has no such exit). */
+int abm_schelling_settled(const abm_schelling_grid *g, const int32_t *prev_counters, int32_t steps,
+                          int32_t *counters, float *happy_frac, int32_t *moves, void *stream);
 
 /* ---- differentiable Deffuant model, batch of (eps, mu) instances
  *      (loss and gradient set-up jaxabm/analysis.py:489-531; step schedule

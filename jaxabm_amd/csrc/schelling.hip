@@ -380,6 +380,22 @@ __global__ void k_schelling_record(const int32_t *__restrict__ counters, int32_t
   moves[t] = counters[2 * t + 1];
 }
 
+// settled steps: each is the identity when the step before them found every
+// agent happy; otherwise the records are marked invalid (-1, NaN, -1)
+__global__ void k_schelling_settled(const int32_t *__restrict__ prev_counters, int32_t steps,
+                                    int64_t n_agents, int32_t *__restrict__ counters,
+                                    float *__restrict__ happy_frac,
+                                    int32_t *__restrict__ moves) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= steps) return;
+  const bool ok = (int64_t)prev_counters[0] == n_agents;
+  counters[2 * t] = ok ? (int32_t)n_agents : -1;
+  counters[2 * t + 1] = ok ? 0 : -1;
+  happy_frac[t] = ok ? (float)n_agents / (float)(n_agents > 0 ? n_agents : 1)
+                     : __int_as_float(0x7fc00000);
+  moves[t] = ok ? 0 : -1;
+}
+
 __global__ void k_schelling_init_cells(const uint32_t *__restrict__ key, int64_t nc,
                                        int8_t *__restrict__ cells, int32_t *__restrict__ claim,
                                        int mode) {
@@ -515,6 +531,19 @@ int abm_schelling_run(const abm_schelling_grid *g, const uint32_t *keys, int32_t
   hipLaunchKernelGGL(k_schelling_record, dim3(nblk(steps, 256)), dim3(256), 0, s, counters,
                      steps, g->n_agents > 0 ? g->n_agents : 1, happy_frac, moves);
   ABM_LAUNCH_CHECK("abm_schelling_run");
+  return kOk;
+}
+
+int abm_schelling_settled(const abm_schelling_grid *g, const int32_t *prev_counters, int32_t steps,
+                          int32_t *counters, float *happy_frac, int32_t *moves, void *stream) {
+  if (int rc = check_grid(g, "abm_schelling_settled")) return rc;
+  ABM_REQUIRE(prev_counters && counters && happy_frac && moves && steps >= 0,
+              "abm_schelling_settled: bad arguments");
+  if (steps == 0) return kOk;
+  hipLaunchKernelGGL(k_schelling_settled, dim3(nblk(steps, 256)), dim3(256), 0,
+                     (hipStream_t)stream, prev_counters, steps, g->n_agents, counters, happy_frac,
+                     moves);
+  ABM_LAUNCH_CHECK("abm_schelling_settled");
   return kOk;
 }
 

@@ -24,6 +24,8 @@ from .._lib import SchellingGrid, check, lib, ptr, require_gpu, stream_handle
 
 
 class SchellingEngine:
+    SETTLE_CHUNK = 16   # steps per launch when a run may stop once the grid settles
+
     def __init__(self, height: int = 512, width: int = 512, seed: int = 0,
                  mode: int | None = None, device: str = "cuda"):
         require_gpu()
@@ -46,6 +48,7 @@ class SchellingEngine:
                                 self.target.data_ptr(), self.empty.data_ptr(),
                                 self.block_counts.data_ptr())
         self.n_agents = 0
+        self._peek = None
         self.rng = np.zeros(2, np.uint32)
         self.reset(seed)
 
@@ -67,14 +70,60 @@ class SchellingEngine:
                                       ptr(counters), ptr(happy), ptr(moves),
                                       stream_handle(stream)), "schelling_run")
 
-    def run(self, steps: int):
-        """Run ``steps`` steps; returns host (fraction happy, moves) arrays."""
+    def run(self, steps: int, chunk: int = SETTLE_CHUNK):
+        """Run ``steps`` steps; returns host (fraction happy, moves) arrays.
+        ``chunk``: steps per launch, with the settled-grid exit of
+        run_settling (0: all steps in one launch)."""
         keys = self.key_table(steps)
         counters = torch.empty((max(steps, 1), 2), dtype=torch.int32, device=self.device)
         happy = torch.empty(max(steps, 1), dtype=torch.float32, device=self.device)
         moves = torch.empty(max(steps, 1), dtype=torch.int32, device=self.device)
-        self.launch(keys, steps, counters, happy, moves)
+        self.run_settling(keys, steps, counters, happy, moves, chunk)
         return happy[:steps].cpu().numpy(), moves[:steps].cpu().numpy()
+
+    def run_settling(self, keys, steps, counters, happy, moves, chunk=SETTLE_CHUNK, stream=None,
+                     launch=None) -> int:
+        """Launch the steps ``chunk`` at a time and stop once the grid settles.
+
+        After each launch the happy count of its last step is copied back
+        (pinned memory, an event); the host reads launch i's count before it
+        queues launch i + 2.  A step whose previous step found every agent
+        happy is the identity, and so is every step after it: once launch i
+        ends with all agents happy, launch i + 1 (already queued) runs identity
+        steps and the records of the steps after it are written by one
+        abm_schelling_settled call instead, the values those steps would
+        record.  ``launch(i, s0, s1)`` queues steps s0..s1-1 as launch i
+        (default: abm_schelling_run on this stream).  Returns the number of
+        steps launched."""
+        st = stream or torch.cuda.current_stream(self.device)
+        if launch is None:
+            def launch(i, s0, s1):
+                self.launch(keys[s0:], s1 - s0, counters[s0:], happy[s0:], moves[s0:], st)
+        chunk = chunk if chunk > 0 else max(steps, 1)
+        bounds = [(s0, min(s0 + chunk, steps)) for s0 in range(0, steps, chunk)]
+        if len(bounds) <= 2:   # nothing to stop early
+            for i, (s0, s1) in enumerate(bounds):
+                launch(i, s0, s1)
+            return steps
+        if self._peek is None or self._peek.numel() < len(bounds):
+            self._peek = torch.empty(len(bounds), dtype=torch.int32, pin_memory=True)
+        events = []
+        with torch.cuda.stream(st):
+            for i, (s0, s1) in enumerate(bounds):
+                if i >= 2:
+                    events[i - 2].synchronize()
+                    if int(self._peek[i - 2]) == self.n_agents:
+                        check(lib().abm_schelling_settled(
+                            C.byref(self._g), ptr(counters[s0 - 1]), steps - s0, ptr(counters[s0:]),
+                            ptr(happy[s0:]), ptr(moves[s0:]), stream_handle(st)),
+                            "schelling_settled")
+                        return s0
+                launch(i, s0, s1)
+                self._peek[i:i + 1].copy_(counters[s1 - 1, 0:1], non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record(st)
+                events.append(ev)
+        return steps
 
     def grid(self) -> np.ndarray:
         return self.cells.cpu().numpy().reshape(self.H, self.W)

@@ -22,17 +22,19 @@ def schelling():
 
 
 @pytest.mark.parametrize("mode", MODES)
-@pytest.mark.parametrize("H,W,steps,seed", [(64, 64, 30, 0), (100, 37, 20, 3), (5, 7, 10, 1),
-                                            (128, 96, 12, 2), (1030, 1030, 3, 7)])
-def test_schelling_matches_oracle(schelling, mode, H, W, steps, seed):
+@pytest.mark.parametrize("H,W,steps,seed,chunk", [(64, 64, 30, 0, 8), (64, 64, 30, 0, 0),
+                                                  (64, 64, 30, 0, 1), (100, 37, 20, 3, 3),
+                                                  (5, 7, 10, 1, 2), (128, 96, 12, 2, 8),
+                                                  (1030, 1030, 3, 7, 8)])
+def test_schelling_matches_oracle(schelling, mode, H, W, steps, seed, chunk):
     """1030 x 1030: more than 1,024 blocks of 1,024 cells, the chunked
-    empty-cell prefix."""
+    empty-cell prefix.  chunk: steps per launch (0: one launch)."""
     eng = schelling.SchellingEngine(H, W, seed=seed, mode=mode)
     cells, pos, typ, rng = O.schelling_init(H, W, seed, mode)
     assert eng.n_agents == len(pos)
     np.testing.assert_array_equal(eng.grid().reshape(-1), cells)
     np.testing.assert_array_equal(eng.positions(), pos)
-    hf, mv = eng.run(steps)
+    hf, mv = eng.run(steps, chunk)
     ohf, omv = O.schelling_run(H, W, cells, pos, typ, rng, steps, mode)
     np.testing.assert_array_equal(mv, omv)
     np.testing.assert_array_equal(hf, ohf)
@@ -65,3 +67,60 @@ def test_schelling_full_size(schelling):
     assert ohf[-1] > ohf[0]
     settled = int(np.argmax(ohf == 1.0))
     assert 10 < settled < 100 and (ohf[settled:] == 1.0).all() and (omv[settled:] == 0).all()
+
+
+@pytest.mark.parametrize("chunk", [8, 5, 1])
+def test_schelling_settled_run_stops_launching(schelling, chunk):
+    """512 x 512, 200 steps launched ``chunk`` steps at a time: once a launch
+    ends with every agent happy, the steps after the launch queued behind it
+    are recorded without a launch.  Records, grid and positions equal the
+    oracle's and the one-launch run's; fewer than 200 steps were launched."""
+    import torch
+
+    H = W = 512
+    S = 200
+    cells, pos, typ, rng = O.schelling_init(H, W, 0, O.MODE_PARTITIONABLE)
+    ohf, omv = O.schelling_run(H, W, cells, pos, typ, rng, S, O.MODE_PARTITIONABLE)
+    settled = int(np.argmax(ohf == 1.0))
+    eng = schelling.SchellingEngine(H, W, seed=0)
+    keys = eng.key_table(S)
+    counters = torch.full((S, 2), -7, dtype=torch.int32, device="cuda")
+    happy = torch.full((S,), -7.0, dtype=torch.float32, device="cuda")
+    moves = torch.full((S,), -7, dtype=torch.int32, device="cuda")
+    launched = eng.run_settling(keys, S, counters, happy, moves, chunk)
+    np.testing.assert_array_equal(happy.cpu().numpy(), ohf)
+    np.testing.assert_array_equal(moves.cpu().numpy(), omv)
+    np.testing.assert_array_equal(eng.grid().reshape(-1), cells)
+    np.testing.assert_array_equal(eng.positions(), pos)
+    # launch i ends settled once its last step is at or past the first
+    # all-happy step; launch i + 1 is then in flight
+    first = -(-(settled + 1) // chunk) - 1
+    assert launched == min(S, (first + 2) * chunk)
+    assert launched < S
+
+
+def test_schelling_settled_rejects_an_unsettled_count(schelling):
+    """abm_schelling_settled writes the identity records only when the count
+    it is given equals the number of agents; another count gives records
+    marked invalid (-1, NaN, -1), not identity steps."""
+    import ctypes as C
+
+    import torch
+
+    from jaxabm_amd._lib import check, lib, ptr, stream_handle
+
+    eng = schelling.SchellingEngine(16, 16, seed=1)
+    n = eng.n_agents
+    for prev, ok in ((n, True), (n - 1, False)):
+        pc = torch.tensor([prev, 0], dtype=torch.int32, device="cuda")
+        counters = torch.zeros((5, 2), dtype=torch.int32, device="cuda")
+        happy = torch.zeros(5, dtype=torch.float32, device="cuda")
+        moves = torch.zeros(5, dtype=torch.int32, device="cuda")
+        check(lib().abm_schelling_settled(C.byref(eng._g), ptr(pc), 5, ptr(counters), ptr(happy),
+                                          ptr(moves), stream_handle()), "settled")
+        c, h, m = counters.cpu().numpy(), happy.cpu().numpy(), moves.cpu().numpy()
+        if ok:
+            assert (c[:, 0] == n).all() and (c[:, 1] == 0).all()
+            assert (h == 1.0).all() and (m == 0).all()
+        else:
+            assert (c == -1).all() and np.isnan(h).all() and (m == -1).all()
