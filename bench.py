@@ -602,7 +602,8 @@ def bench_schelling(args) -> dict:
                 check(lib().abm_memcpy_d2d(ptr(getattr(eng, k)), ptr(t0),
                                            C.c_size_t(t0.numel() * t0.element_size()), sh),
                       "reset")
-        eng.launch(keys[s0:], s1 - s0, counters[s0:], happy[s0:], moves[s0:], stream=stream)
+        eng.launch(keys[s0:], s1 - s0, counters[s0:], happy[s0:], moves[s0:], stream=stream,
+                   prev=counters[s0 - 1] if s0 else None)
         check(lib().abm_graph_end(sh, C.byref(gi)), "graph_end")
         graphs.append(gi)
     launched = []

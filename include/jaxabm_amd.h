@@ -310,6 +310,16 @@ int abm_schelling_init(abm_schelling_grid *g, const uint32_t *key, int mode, int
  * to 16,776,192 cells. */
 int abm_schelling_run(const abm_schelling_grid *g, const uint32_t *keys, int32_t steps, int mode,
                       int32_t *counters, float *happy_frac, int32_t *moves, void *stream);
+/* Continues a run: the steps of abm_schelling_run, for a grid whose cells
+ * have not changed since the last abm_schelling_run / abm_schelling_run_from
+ * call on it ended (that call left the grid's bitmap and block counts current,
+ * so they are not rebuilt).  prev_counters: device [2], the counters of the
+ * step before the first one here, which the first step's identity test reads
+ * (a step after one that found every agent happy is the identity).  The step
+ * loop is jaxabm/model.py:146-216 split over several calls. */
+int abm_schelling_run_from(const abm_schelling_grid *g, const uint32_t *keys, int32_t steps,
+                           int mode, const int32_t *prev_counters, int32_t *counters,
+                           float *happy_frac, int32_t *moves, void *stream);
 /* Records `steps` settled steps without launching them.  A step whose
  * previous step found every agent happy is the identity (nothing moves, so
  * every agent is happy again), and so is every step after it.  prev_counters:

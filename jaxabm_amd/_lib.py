@@ -201,6 +201,8 @@ SIGNATURES = {
     "abm_schelling_block_count": (_i64, [_i64]),
     "abm_schelling_init": (_int, [C.POINTER(SchellingGrid), _p, _int, C.POINTER(_i32), _p]),
     "abm_schelling_run": (_int, [C.POINTER(SchellingGrid), _p, _i32, _int, _p, _p, _p, _p]),
+    "abm_schelling_run_from": (_int, [C.POINTER(SchellingGrid), _p, _i32, _int, _p, _p, _p, _p,
+                                       _p]),
     "abm_schelling_settled": (_int, [C.POINTER(SchellingGrid), _p, _i32, _p, _p, _p, _p]),
     "abm_unit_threshold": (_u32, [_f32]),
     "abm_fill_i32": (_int, [_p, _i64, _i32, _p]),

@@ -471,10 +471,15 @@ int abm_schelling_init(abm_schelling_grid *g, const uint32_t *key, int mode,
   return kOk;
 }
 
-int abm_schelling_run(const abm_schelling_grid *g, const uint32_t *keys, int32_t steps,
-                      int mode, int32_t *counters, float *happy_frac, int32_t *moves,
-                      void *stream) {
-  if (int rc = check_grid(g, "abm_schelling_run")) return rc;
+}  // extern "C"
+
+// the step loop of a run; prev0: the counters of the step before the first
+// (nullptr: a fresh run, whose bitmap and block counts are built from the
+// cells first)
+static int schelling_steps(const abm_schelling_grid *g, const uint32_t *keys, int32_t steps,
+                           int mode, const int32_t *prev0, int32_t *counters, float *happy_frac,
+                           int32_t *moves, void *stream, const char *what) {
+  if (int rc = check_grid(g, what)) return rc;
   ABM_REQUIRE(keys && counters && happy_frac && moves && steps >= 0 &&
                   (mode == 0 || mode == 1),
               "abm_schelling_run: bad arguments");
@@ -491,14 +496,16 @@ int abm_schelling_run(const abm_schelling_grid *g, const uint32_t *keys, int32_t
               "abm_schelling_run: the empty scratch must be 16-byte aligned");
   hipLaunchKernelGGL(k_fill_i32_s, dim3(nblk(2 * (int64_t)steps, 256)), dim3(256), 0, s,
                      counters, 2 * (int64_t)steps, 0);
-  hipLaunchKernelGGL(k_empty_bits, dim3(nb), dim3(kCellsPerBlock), 0, s, g->cells, nc, nwords,
-                     ebits);
-  // copy 0 = the grid's empty counts, the other copies start at zero
-  hipLaunchKernelGGL(k_cells_count, dim3(nb), dim3(kCellsPerBlock), 0, s, g->cells, nc, 1,
-                     g->block_counts);
-  const int64_t zero_words = (int64_t)(kCountCopies - 1) * nb;
-  hipLaunchKernelGGL(k_fill_i32_s, dim3(nblk(zero_words, 256)), dim3(256), 0, s,
-                     g->block_counts + nb, zero_words, 0);
+  if (!prev0) {
+    hipLaunchKernelGGL(k_empty_bits, dim3(nb), dim3(kCellsPerBlock), 0, s, g->cells, nc, nwords,
+                       ebits);
+    // copy 0 = the grid's empty counts, the other copies start at zero
+    hipLaunchKernelGGL(k_cells_count, dim3(nb), dim3(kCellsPerBlock), 0, s, g->cells, nc, 1,
+                       g->block_counts);
+    const int64_t zero_words = (int64_t)(kCountCopies - 1) * nb;
+    hipLaunchKernelGGL(k_fill_i32_s, dim3(nblk(zero_words, 256)), dim3(256), 0, s,
+                       g->block_counts + nb, zero_words, 0);
+  }
   const dim3 tiles((unsigned)((g->width + kTileDim - 1) / kTileDim),
                    (unsigned)((g->height + kTileDim - 1) / kTileDim));
   const size_t prefix_lds = (size_t)(nb + 1) * sizeof(int32_t);
@@ -519,7 +526,7 @@ int abm_schelling_run(const abm_schelling_grid *g, const uint32_t *keys, int32_t
     a.ck = keys + 2 * (int64_t)t;
     a.mode = mode;
     a.happy_count = counters + 2 * (int64_t)t;
-    a.prev_happy = t ? counters + 2 * (int64_t)(t - 1) : nullptr;
+    a.prev_happy = t ? counters + 2 * (int64_t)(t - 1) : prev0;
     hipLaunchKernelGGL(k_schelling_agents, tiles, dim3(kTileDim * kTileDim), prefix_lds, s, a);
     if (g->n_agents > 0)
       hipLaunchKernelGGL(k_schelling_moves, dim3(nblk(g->n_agents, kMovesThreads)),
@@ -530,8 +537,25 @@ int abm_schelling_run(const abm_schelling_grid *g, const uint32_t *keys, int32_t
   }
   hipLaunchKernelGGL(k_schelling_record, dim3(nblk(steps, 256)), dim3(256), 0, s, counters,
                      steps, g->n_agents > 0 ? g->n_agents : 1, happy_frac, moves);
-  ABM_LAUNCH_CHECK("abm_schelling_run");
+  ABM_LAUNCH_CHECK(what);
   return kOk;
+}
+
+extern "C" {
+
+int abm_schelling_run(const abm_schelling_grid *g, const uint32_t *keys, int32_t steps,
+                      int mode, int32_t *counters, float *happy_frac, int32_t *moves,
+                      void *stream) {
+  return schelling_steps(g, keys, steps, mode, nullptr, counters, happy_frac, moves, stream,
+                         "abm_schelling_run");
+}
+
+int abm_schelling_run_from(const abm_schelling_grid *g, const uint32_t *keys, int32_t steps,
+                           int mode, const int32_t *prev_counters, int32_t *counters,
+                           float *happy_frac, int32_t *moves, void *stream) {
+  ABM_REQUIRE(prev_counters, "abm_schelling_run_from: prev_counters is null");
+  return schelling_steps(g, keys, steps, mode, prev_counters, counters, happy_frac, moves,
+                         stream, "abm_schelling_run_from");
 }
 
 int abm_schelling_settled(const abm_schelling_grid *g, const int32_t *prev_counters, int32_t steps,

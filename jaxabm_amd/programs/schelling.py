@@ -65,10 +65,19 @@ class SchellingEngine:
         ck = np.ascontiguousarray(tab[:, 0:2])
         return torch.from_numpy(ck.view(np.int32)).to(self.device)
 
-    def launch(self, keys, steps, counters, happy, moves, stream=None):
-        check(lib().abm_schelling_run(C.byref(self._g), ptr(keys), int(steps), self.mode,
-                                      ptr(counters), ptr(happy), ptr(moves),
-                                      stream_handle(stream)), "schelling_run")
+    def launch(self, keys, steps, counters, happy, moves, stream=None, prev=None):
+        """abm_schelling_run; with ``prev`` (the counters of the step before,
+        a device [2] view) abm_schelling_run_from, which continues the run the
+        last call left."""
+        if prev is None:
+            check(lib().abm_schelling_run(C.byref(self._g), ptr(keys), int(steps), self.mode,
+                                          ptr(counters), ptr(happy), ptr(moves),
+                                          stream_handle(stream)), "schelling_run")
+        else:
+            check(lib().abm_schelling_run_from(C.byref(self._g), ptr(keys), int(steps),
+                                               self.mode, ptr(prev), ptr(counters), ptr(happy),
+                                               ptr(moves), stream_handle(stream)),
+                  "schelling_run_from")
 
     def run(self, steps: int, chunk: int = SETTLE_CHUNK):
         """Run ``steps`` steps; returns host (fraction happy, moves) arrays.
@@ -93,12 +102,14 @@ class SchellingEngine:
         steps and the records of the steps after it are written by one
         abm_schelling_settled call instead, the values those steps would
         record.  ``launch(i, s0, s1)`` queues steps s0..s1-1 as launch i
-        (default: abm_schelling_run on this stream).  Returns the number of
-        steps launched."""
+        (default on this stream: abm_schelling_run for the first,
+        abm_schelling_run_from for the others).  Returns the number of steps
+        launched."""
         st = stream or torch.cuda.current_stream(self.device)
         if launch is None:
             def launch(i, s0, s1):
-                self.launch(keys[s0:], s1 - s0, counters[s0:], happy[s0:], moves[s0:], st)
+                self.launch(keys[s0:], s1 - s0, counters[s0:], happy[s0:], moves[s0:], st,
+                            prev=counters[s0 - 1] if s0 else None)
         chunk = chunk if chunk > 0 else max(steps, 1)
         bounds = [(s0, min(s0 + chunk, steps)) for s0 in range(0, steps, chunk)]
         if len(bounds) <= 2:   # nothing to stop early
