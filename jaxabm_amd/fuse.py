@@ -94,6 +94,7 @@ _TORCH = {"f32": torch.float32, "f64": torch.float64, "i32": torch.int32, "i64":
           "u8": torch.uint8, "bool": torch.bool}
 _KIND = {v: k for k, v in _TORCH.items()}
 _WEAK = ("pyf", "pyi", "pyb")
+_PROMOTED: Dict[tuple, str] = {}   # (kind a, varying a, kind b, varying b) -> result kind
 _FLOATS = ("f32", "f64", "pyf")
 _DEFAULT = {"pyf": "f32", "pyi": "i32", "pyb": "bool"}   # JAX's defaults for Python scalars
 
@@ -1187,7 +1188,13 @@ class _Trace:
             if "pyi" in (a.kind, b.kind):
                 return "pyi"
             return "pyb"
-        return _KIND[torch.result_type(self._stand_in(a), self._stand_in(b))]
+        # torch's promotion depends on the kinds and on which operands are
+        # 0-d (uniform) or 1-d (varying) only: memoised
+        k = (a.kind, a.varying, b.kind, b.varying)
+        r = _PROMOTED.get(k)
+        if r is None:
+            r = _PROMOTED[k] = _KIND[torch.result_type(self._stand_in(a), self._stand_in(b))]
+        return r
 
     # -- element operations
     def binary(self, op: str, a, b):
