@@ -307,7 +307,7 @@ int abm_schelling_init(abm_schelling_grid *g, const uint32_t *key, int mode, int
                        void *stream);
 /* Runs `steps` steps; keys: device [steps][2] collection keys ck.  counters:
  * device scratch [steps][2]; happy_frac, moves: device [steps].  Grids of up
- * to 16,776,192 cells. */
+ * to 15,360,000 cells. */
 int abm_schelling_run(const abm_schelling_grid *g, const uint32_t *keys, int32_t steps, int mode,
                       int32_t *counters, float *happy_frac, int32_t *moves, void *stream);
 /* Continues a run: the steps of abm_schelling_run, for a grid whose cells

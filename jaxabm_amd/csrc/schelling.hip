@@ -42,8 +42,9 @@ namespace abm {
 constexpr int kTileDim = 32;
 constexpr int kCellsPerBlock = 1024;
 constexpr int kWordsPerBlock = kCellsPerBlock / 32;
-// block prefix of the empty counts in LDS: at most 64 KB
-constexpr int64_t kMaxEmptyBlocks = 16383;
+// block prefix of the empty counts in LDS: with the agents kernel's 5.3 KB of
+// static LDS, at most 64 KB in all
+constexpr int64_t kMaxEmptyBlocks = 15000;
 // copies of the per-block empty counts: the moves kernel's workgroup w updates
 // copy w % kCountCopies, so the count atomics of a busy step spread over 16x
 // the words (each word's device-scope atomics serialise); readers sum them
@@ -250,6 +251,8 @@ __global__ void __launch_bounds__(kTileDim * kTileDim) k_schelling_agents(Schell
   __shared__ int8_t tile[kTileDim + 2][kTileDim + 2];
   __shared__ int32_t wc[16];
   __shared__ int32_t carry;
+  __shared__ int32_t work[kTileDim * kTileDim];  // the tile's agents that draw a target
+  __shared__ int32_t nwork;
   constexpr int kHalo = (kTileDim + 2) * (kTileDim + 2);  // 34 x 34 cells, <= 2 per thread
   const int tid = threadIdx.x;
   if (schelling_identity(a.prev_happy, a.n_agents)) {  // grid-uniform
@@ -284,8 +287,9 @@ __global__ void __launch_bounds__(kTileDim * kTileDim) k_schelling_agents(Schell
     const int q = tid + i * (int)blockDim.x;
     if (q < kHalo) tile[q / (kTileDim + 2)][q % (kTileDim + 2)] = tv[i];
   }
+  if (tid == 0) nwork = 0;  // visible after the prefix's barriers
   empty_prefix(a.ecount, a.nb, first, empty_prefix_lds, wc, &carry);
-  bool happy = false;
+  bool happy = false, draws = false;
   if (inside) {
     const int8_t me = tile[ty + 1][tx + 1];
     if (me) {
@@ -300,17 +304,30 @@ __global__ void __launch_bounds__(kTileDim * kTileDim) k_schelling_agents(Schell
           same += v == me;
         }
       happy = 8 * same >= 3 * occ;
-      int32_t tg = -1;
-      if (!happy && a.n_empty > 0) {
-        Key k1, k2;
-        split2(split_at(Key{a.ck[0], a.ck[1]}, a.n_agents, k, a.mode), k1, k2, a.mode);
-        const uint32_t j = randint_offset(bits_scalar(k1, a.mode), bits_scalar(k2, a.mode),
-                                          (uint32_t)a.n_empty);
-        tg = jth_empty(j, empty_prefix_lds, a.nb, a.ebits, a.nwords);
-        if (tg >= 0) atomicMin(&a.claim[tg], k);
-      }
-      a.target[k] = tg;
+      draws = !happy && a.n_empty > 0;
+      if (!draws) a.target[k] = -1;
     }
+  }
+  // The agents that draw go to the front of the workgroup, so that the draw
+  // and the search run on full waves, not on every wave that holds one.
+  const int lane = tid & 63;
+  const uint64_t dm = __ballot(draws);
+  if (dm) {  // wave-uniform
+    int32_t base = 0;
+    if (lane == 0) base = atomicAdd(&nwork, (int32_t)__popcll(dm));
+    base = __shfl(base, 0);
+    if (draws) work[base + (int32_t)__popcll(dm & ((1ull << lane) - 1ull))] = k;
+  }
+  __syncthreads();
+  if (tid < nwork) {
+    const int32_t kd = work[tid];
+    Key k1, k2;
+    split2(split_at(Key{a.ck[0], a.ck[1]}, a.n_agents, kd, a.mode), k1, k2, a.mode);
+    const uint32_t j = randint_offset(bits_scalar(k1, a.mode), bits_scalar(k2, a.mode),
+                                      (uint32_t)a.n_empty);
+    const int32_t tg = jth_empty(j, empty_prefix_lds, a.nb, a.ebits, a.nwords);
+    if (tg >= 0) atomicMin(&a.claim[tg], kd);
+    a.target[kd] = tg;
   }
   const uint64_t m = __ballot(happy);
   if ((tid & 63) == 0) wc[tid >> 6] = (int32_t)__popcll(m);
@@ -489,7 +506,7 @@ static int schelling_steps(const abm_schelling_grid *g, const uint32_t *keys, in
   const int64_t n_empty = nc - g->n_agents;
   const unsigned nb = nblk(nc, kCellsPerBlock);
   ABM_REQUIRE(nb <= kMaxEmptyBlocks,
-              "abm_schelling_run: grids of more than 16,776,192 cells are not supported");
+              "abm_schelling_run: grids of more than 15,360,000 cells are not supported");
   const int64_t nwords = (nc + 31) / 32;
   uint32_t *ebits = reinterpret_cast<uint32_t *>(g->empty);  // nwords <= H*W words
   ABM_REQUIRE(((uintptr_t)ebits & 15u) == 0,
