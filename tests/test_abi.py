@@ -120,3 +120,29 @@ def test_unit_threshold_is_exact():
         ms = np.arange(max(0, thr - 3), min(1 << 23, thr + 3))
         u = (np.uint32(0x3F800000) | ms.astype(np.uint32)).view(np.float32) - np.float32(1.0)
         np.testing.assert_array_equal(ms < thr, u < pf)
+
+
+def test_schelling_entry_points_reject_bad_arguments_without_a_device():
+    """The Schelling entry points validate on the host before any launch: a
+    continuing run needs the previous step's counters, the settled records
+    need a grid, and grids above 15,360,000 cells are refused (the empty-cell
+    block prefix and the agents kernel's static LDS stay within 64 KB)."""
+    from jaxabm_amd._lib import AbmError, SchellingGrid, check, lib
+
+    fake = 0x1000   # never dereferenced: every call fails before a launch
+    L = lib()
+
+    def grid(h, w):
+        return SchellingGrid(h, w, 0, *([fake] * 8))
+
+    g = grid(64, 64)
+    with pytest.raises(AbmError, match="abm_schelling_run_from: prev_counters is null"):
+        check(L.abm_schelling_run_from(C.byref(g), fake, 4, 0, None, fake, fake, fake, None),
+              "run_from")
+    with pytest.raises(AbmError, match="abm_schelling_settled: bad grid descriptor"):
+        check(L.abm_schelling_settled(None, fake, 4, fake, fake, fake, None), "settled")
+    big = grid(4000, 3841)   # 15,364,000 cells: 15,004 blocks of 1,024
+    for rc in (L.abm_schelling_run(C.byref(big), fake, 4, 0, fake, fake, fake, None),
+               L.abm_schelling_run_from(C.byref(big), fake, 4, 0, fake, fake, fake, fake, None)):
+        with pytest.raises(AbmError, match="15,360,000 cells"):
+            check(rc, "run")
