@@ -124,3 +124,19 @@ def test_schelling_settled_rejects_an_unsettled_count(schelling):
             assert (h == 1.0).all() and (m == 0).all()
         else:
             assert (c == -1).all() and np.isnan(h).all() and (m == -1).all()
+
+
+def test_schelling_near_the_grid_size_limit(schelling):
+    """3,900 x 3,900 cells (14,854 blocks of 1,024, close to the 15,000-block
+    limit): the agents kernel's largest prefix plus its static LDS fit the
+    workgroup; 2 steps exact against the oracle."""
+    H = W = 3900
+    eng = schelling.SchellingEngine(H, W, seed=5)
+    cells, pos, typ, rng = O.schelling_init(H, W, 5, O.MODE_PARTITIONABLE)
+    assert eng.n_agents == len(pos)
+    hf, mv = eng.run(2)
+    ohf, omv = O.schelling_run(H, W, cells, pos, typ, rng, 2, O.MODE_PARTITIONABLE)
+    np.testing.assert_array_equal(mv, omv)
+    np.testing.assert_array_equal(hf, ohf)
+    np.testing.assert_array_equal(eng.grid().reshape(-1), cells)
+    np.testing.assert_array_equal(eng.positions(), pos)
