@@ -28,8 +28,10 @@ from .core import ModelConfig
 
 
 def _dev_key(k: np.ndarray) -> torch.Tensor:
-    return torch.from_numpy(np.ascontiguousarray(k, np.uint32).view(np.int32).copy()).to(
-        jr._device())
+    """A host key on the device, staged in pinned memory and copied on the
+    current stream without waiting for the work queued before it."""
+    host = torch.from_numpy(np.ascontiguousarray(k, np.uint32).view(np.int32).copy())
+    return host.pin_memory().to(jr._device(), non_blocking=True)
 
 
 class Model:
